@@ -1,0 +1,378 @@
+"""BriefMatch optical flow on AMD MI355X (gfx950): Python host interface.
+
+A thin ctypes layer over the C ABI in include/briefmatch_amd.h
+(libbriefmatch_amd.so, built by ``python -m briefmatch_amd.build``).  It mirrors
+the reference's C++ interface: ``Params`` is BMParams (briefmatch.h:65-89),
+``derive_params`` is the CLI derivation of OFPipeline::setParams
+(of_pipeline.cpp:178-190), ``Context`` is BriefMatch (briefmatch.h:108-155) with
+one method per stage, and ``BriefMatchError`` is BriefMatchException
+(briefmatch.h:91-106).  There is no CPU fallback: if the HIP library is missing
+or the GPU is unavailable, calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["BriefMatchError", "Params", "derive_params", "default_params", "DeviceBuffer", "Context",
+           "lib", "LIB_PATH", "philox4x32_10", "epe"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbriefmatch_amd.so")
+
+BM_OK, BM_E_PARAM, BM_E_ALLOC, BM_E_LAUNCH, BM_E_IO, BM_E_STATE = range(6)
+# stage ids of bm_profile_read (BM_STAGE_* in briefmatch_amd.h)
+STAGES = ("gray", "upsample", "brief", "init", "level", "level_rs", "random_search", "downsample", "median",
+          "trilateral", "copy")
+H2D, D2H, D2D, H2H = range(4)
+
+
+class BriefMatchError(RuntimeError):
+    """A non-zero status from the C ABI; ``code`` is the BM_E_* value."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class Params(C.Structure):
+    """bm_params: the fields of BMParams (briefmatch.h:65-89)."""
+    _fields_ = [(n, C.c_uint) for n in ("sx", "sy", "sx_in", "sy_in", "N", "iterations", "jfMax", "P",
+                                        "fsize")] + \
+               [(n, C.c_float) for n in ("patchArea", "sigma", "sEPE", "sI", "sS", "maxmot")] + \
+               [(n, C.c_int) for n in ("doubleProp", "filter", "tInit")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+    def copy(self):
+        p = Params()
+        C.memmove(C.byref(p), C.byref(self), C.sizeof(Params))
+        return p
+
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library; raises BriefMatchError when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BriefMatchError(BM_E_STATE, "HIP library %s is not built; run `python -m briefmatch_amd.build`"
+                              % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    V, I, U32, U64, F = C.c_void_p, C.c_int, C.c_uint32, C.c_uint64, C.c_float
+    sig = {
+        "bm_params_default": (None, [C.POINTER(Params)]),
+        "bm_params_derive": (I, [C.c_uint, C.c_uint, F, F, F, F, F, F, C.POINTER(Params)]),
+        "bm_ctx_create": (I, [C.POINTER(Params), U64, C.POINTER(V)]),
+        "bm_ctx_destroy": (None, [V]),
+        "bm_ctx_get_params": (I, [V, C.POINTER(Params)]),
+        "bm_ctx_stream": (V, [V]),
+        "bm_last_error": (C.c_char_p, [V]),
+        "bm_sync": (I, [V]),
+        "bm_philox4x32_10": (I, [V, V, V]),
+        "bm_descriptor_stride": (C.c_uint, [V]),
+        "bm_get_pattern": (I, [V, V]),
+        "bm_set_pattern": (I, [V, V]),
+        "bm_get_gauss_kernel": (I, [V, V, C.POINTER(C.c_uint)]),
+        "bm_ctx_field": (I, [V, C.POINTER(V), C.POINTER(V)]),
+        "bm_malloc": (I, [C.POINTER(V), C.c_size_t]),
+        "bm_free": (I, [V]),
+        "bm_memcpy": (I, [V, V, C.c_size_t, I]),
+        "bm_memset": (I, [V, I, C.c_size_t]),
+        "bm_gray": (I, [V, V, V, V]),
+        "bm_upsample": (I, [V, V, V]),
+        "bm_brief": (I, [V, V, V]),
+        "bm_brief_gray": (I, [V, V, V]),
+        "bm_init": (I, [V, V, V, I, V, V, V]),
+        "bm_propagate_level": (I, [V, V, V, V, V, V, V, I, I]),
+        "bm_propagate": (I, [V, V, V, V, V, I]),
+        "bm_random_search": (I, [V, V, V, V, V, U32, U32]),
+        "bm_match": (I, [V, V, V, I, V, U32, V, V]),
+        "bm_downsample": (I, [V, V, V, V]),
+        "bm_median1d": (I, [V, V, V, I]),
+        "bm_trilateral": (I, [V, V, V, V, V, V, V, V]),
+        "bm_refine": (I, [V, V, V, V, V, V]),
+        "bm_flow_pair": (I, [V, V, V, U32, V, V]),
+        "bm_stream_frame": (I, [V, V, V, V, C.POINTER(I)]),
+        "bm_stream_reset": (I, [V]),
+        "bm_timer_start": (I, [V]),
+        "bm_timer_stop": (I, [V, C.POINTER(F)]),
+        "bm_epe": (I, [V, V, V, V, V, C.c_size_t, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "bm_profile_enable": (I, [V, I]),
+        "bm_profile_read": (I, [V, V, V]),
+        "bm_device_copy": (I, [V, V, V, C.c_size_t]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, DeviceBuffer):
+        return a.ptr
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(C.c_void_p)
+    return a
+
+
+def _check(status, ctx=None):
+    if status != BM_OK:
+        msg = lib().bm_last_error(ctx)
+        raise BriefMatchError(status, (msg or b"").decode(errors="replace"))
+
+
+def default_params():
+    p = Params()
+    lib().bm_params_default(C.byref(p))
+    return p
+
+
+def derive_params(W, H, ux=0.0, uy=0.0, patch_radius=3.0, sigma_ds=0.025, median_size=1.0, range_spatial=0.85,
+                  **overrides):
+    """OFPipeline::setParams derivation (of_pipeline.cpp:178-190); keyword
+    overrides set BMParams fields afterwards (e.g. N=32, iterations=1)."""
+    p = Params()
+    _check(lib().bm_params_derive(W, H, ux, uy, patch_radius, sigma_ds, median_size, range_spatial, C.byref(p)))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def philox4x32_10(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    _check(lib().bm_philox4x32_10(_ptr(c), _ptr(k), _ptr(out)))
+    return out
+
+
+def epe(U, V, gt, occ=None):
+    """(EPE-all, EPE-noc) of a flow against ground truth (BriefMatch::error)."""
+    U = np.ascontiguousarray(U, np.float32)
+    V = np.ascontiguousarray(V, np.float32)
+    gu = np.ascontiguousarray(gt[..., 0], np.float32)
+    gv = np.ascontiguousarray(gt[..., 1], np.float32)
+    o = None if occ is None else np.ascontiguousarray(occ, np.uint8)
+    a, n = C.c_double(), C.c_double()
+    _check(lib().bm_epe(_ptr(U), _ptr(V), _ptr(gu), _ptr(gv), _ptr(o), U.size, C.byref(a), C.byref(n)))
+    return a.value, n.value
+
+
+class DeviceBuffer:
+    """Device allocation (CUDA_constructor / CUDA_destructor, util_cuda.cu:206-216)."""
+
+    def __init__(self, nbytes=None, array=None):
+        if array is not None:
+            array = np.ascontiguousarray(array)
+            nbytes = array.nbytes
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check(lib().bm_malloc(C.byref(p), max(self.nbytes, 1)))
+        self.ptr = p
+        if array is not None:
+            self.upload(array)
+
+    @classmethod
+    def like(cls, shape, dtype):
+        return cls(int(np.prod(shape)) * np.dtype(dtype).itemsize)
+
+    def upload(self, array):
+        array = np.ascontiguousarray(array)
+        if array.nbytes > self.nbytes:
+            raise BriefMatchError(BM_E_PARAM, "upload larger than the buffer")
+        _check(lib().bm_memcpy(self.ptr, _ptr(array), array.nbytes, H2D))
+
+    def download(self, shape, dtype):
+        out = np.empty(shape, dtype)
+        if out.nbytes > self.nbytes:
+            raise BriefMatchError(BM_E_PARAM, "download larger than the buffer")
+        _check(lib().bm_memcpy(_ptr(out), self.ptr, out.nbytes, D2H))
+        return out
+
+    def zero(self):
+        _check(lib().bm_memset(self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().bm_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _download_raw(ptr, shape, dtype):
+    out = np.empty(shape, dtype)
+    _check(lib().bm_memcpy(_ptr(out), ptr, out.nbytes, D2H))
+    return out
+
+
+class Context:
+    """One configuration (BriefMatch after setup(), briefmatch.cpp:72-214)."""
+
+    def __init__(self, params, seed=0):
+        self.params = params.copy()
+        h = C.c_void_p()
+        _check(lib().bm_ctx_create(C.byref(self.params), seed, C.byref(h)))
+        self.h = h
+        got = Params()
+        _check(lib().bm_ctx_get_params(self.h, C.byref(got)), self.h)
+        self.params = got
+        self.seed = seed
+        p = self.params
+        self.W, self.H, self.sx, self.sy = p.sx_in, p.sy_in, p.sx, p.sy
+        self.N = p.N
+        self.NS = (p.N + 31) // 32
+        self.NSP = lib().bm_descriptor_stride(self.h)
+
+    # -- plumbing ---------------------------------------------------------
+    def _c(self, status):
+        _check(status, self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        self._c(lib().bm_sync(self.h))
+
+    def pattern(self):
+        out = np.zeros(4 * self.N, np.int16)
+        self._c(lib().bm_get_pattern(self.h, _ptr(out)))
+        return out
+
+    def set_pattern(self, pattern):
+        pat = np.ascontiguousarray(pattern, np.int16)
+        self._c(lib().bm_set_pattern(self.h, _ptr(pat)))
+
+    def gauss_kernel(self):
+        out = np.zeros(128, np.float32)
+        P = C.c_uint()
+        self._c(lib().bm_get_gauss_kernel(self.h, _ptr(out), C.byref(P)))
+        return out[:P.value].copy()
+
+    def field(self):
+        """The context's last match field as numpy (off uint32, cost uint16)."""
+        o, c = C.c_void_p(), C.c_void_p()
+        self._c(lib().bm_ctx_field(self.h, C.byref(o), C.byref(c)))
+        self.sync()
+        return (_download_raw(o, (self.sy, self.sx), np.uint32), _download_raw(c, (self.sy, self.sx), np.uint16))
+
+    # buffer helpers
+    def rgb_buffer(self):
+        return DeviceBuffer(self.W * self.H * 3)
+
+    def image_buffer(self):
+        return DeviceBuffer(self.W * self.H * 4)
+
+    def desc_buffer(self):
+        return DeviceBuffer(self.sx * self.sy * self.NSP * 4)
+
+    def field_buffers(self):
+        return DeviceBuffer(self.sx * self.sy * 4), DeviceBuffer(self.sx * self.sy * 2)
+
+    def download_desc(self, buf):
+        return buf.download((self.sy, self.sx, self.NSP), np.uint32)
+
+    # -- stages -------------------------------------------------------------
+    def gray(self, rgb, g8=None, g32=None):
+        self._c(lib().bm_gray(self.h, _ptr(rgb), _ptr(g8), _ptr(g32)))
+
+    def upsample(self, g8, L):
+        self._c(lib().bm_upsample(self.h, _ptr(g8), _ptr(L)))
+
+    def brief(self, L, F):
+        self._c(lib().bm_brief(self.h, _ptr(L), _ptr(F)))
+
+    def brief_gray(self, g8, F):
+        self._c(lib().bm_brief_gray(self.h, _ptr(g8), _ptr(F)))
+
+    def init(self, F1, F2, off, cost, first=True, prev_off=None):
+        self._c(lib().bm_init(self.h, _ptr(F1), _ptr(F2), int(first), _ptr(prev_off), _ptr(off), _ptr(cost)))
+
+    def propagate_level(self, F1, F2, off_in, cost_in, off_out, cost_out, p, pd):
+        self._c(lib().bm_propagate_level(self.h, _ptr(F1), _ptr(F2), _ptr(off_in), _ptr(cost_in), _ptr(off_out),
+                                         _ptr(cost_out), p, pd))
+
+    def propagate(self, F1, F2, off, cost, pd):
+        self._c(lib().bm_propagate(self.h, _ptr(F1), _ptr(F2), _ptr(off), _ptr(cost), pd))
+
+    def random_search(self, F1, F2, off, cost, iteration, pair=0):
+        self._c(lib().bm_random_search(self.h, _ptr(F1), _ptr(F2), _ptr(off), _ptr(cost), iteration, pair))
+
+    def match(self, F1, F2, off, cost, first=True, prev_off=None, pair=0):
+        self._c(lib().bm_match(self.h, _ptr(F1), _ptr(F2), int(first), _ptr(prev_off), pair, _ptr(off),
+                               _ptr(cost)))
+
+    def downsample(self, off, U, V):
+        self._c(lib().bm_downsample(self.h, _ptr(off), _ptr(U), _ptr(V)))
+
+    def median1d(self, src, dst, direction):
+        self._c(lib().bm_median1d(self.h, _ptr(src), _ptr(dst), direction))
+
+    def trilateral(self, Ui, Vi, Um, Vm, I, Uo, Vo):
+        self._c(lib().bm_trilateral(self.h, *[_ptr(a) for a in (Ui, Vi, Um, Vm, I, Uo, Vo)]))
+
+    def refine(self, U, V, I, Uo, Vo):
+        self._c(lib().bm_refine(self.h, *[_ptr(a) for a in (U, V, I, Uo, Vo)]))
+
+    def flow_pair(self, rgb1, rgb2, U, V, pair=0):
+        self._c(lib().bm_flow_pair(self.h, _ptr(rgb1), _ptr(rgb2), pair, _ptr(U), _ptr(V)))
+
+    def stream_frame(self, rgb, U, V):
+        has = C.c_int()
+        self._c(lib().bm_stream_frame(self.h, _ptr(rgb), _ptr(U), _ptr(V), C.byref(has)))
+        return bool(has.value)
+
+    def stream_reset(self):
+        self._c(lib().bm_stream_reset(self.h))
+
+    def profile_enable(self, on=True):
+        self._c(lib().bm_profile_enable(self.h, int(on)))
+
+    def profile_read(self):
+        """{stage: (ms, launches)} since the last read (waits for the stream)."""
+        ms = np.zeros(len(STAGES), np.float64)
+        n = np.zeros(len(STAGES), np.uint32)
+        self._c(lib().bm_profile_read(self.h, _ptr(ms), _ptr(n)))
+        return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(STAGES)}
+
+    def device_copy(self, dst, src, nbytes):
+        self._c(lib().bm_device_copy(self.h, _ptr(dst), _ptr(src), nbytes))
+
+    def timer_start(self):
+        self._c(lib().bm_timer_start(self.h))
+
+    def timer_stop(self):
+        ms = C.c_float()
+        self._c(lib().bm_timer_stop(self.h, C.byref(ms)))
+        return ms.value
+
+    # -- convenience ---------------------------------------------------------
+    def flow_numpy(self, frame1, frame2, pair=0):
+        """Host RGB uint8 frames -> host (U, V) FP32 via bm_flow_pair."""
+        r1 = DeviceBuffer(array=np.ascontiguousarray(frame1, np.uint8))
+        r2 = DeviceBuffer(array=np.ascontiguousarray(frame2, np.uint8))
+        U, V = self.image_buffer(), self.image_buffer()
+        self.flow_pair(r1, r2, U, V, pair=pair)
+        self.sync()
+        return U.download((self.H, self.W), np.float32), V.download((self.H, self.W), np.float32)

@@ -1,0 +1,890 @@
+// C-ABI of the BriefMatch flow engine: parameters, context (tables, device
+// buffers, stream), stage entry points, the whole-pair path and sequence mode.
+// Host code only; the kernels are in bm_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "bm_internal.h"
+
+namespace {
+
+const int32_t kNdtriQ16[4096] = {
+#include "bm_ndtri_q16.inc"
+};
+
+thread_local std::string g_thread_error;
+
+void philox_host(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+}  // namespace
+
+struct bm_ctx {
+    bm_params p{};
+    bm::Geometry g{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    std::vector<int16_t> pattern;
+    std::vector<float> gauss;
+    // tables
+    int16_t *d_pattern = nullptr;
+    uint32_t *d_up_x = nullptr, *d_up_y = nullptr, *d_nn_x = nullptr, *d_nn_y = nullptr;
+    float *d_gauss = nullptr, *d_spt = nullptr;
+    // work buffers
+    uint8_t *d_g8[2] = {nullptr, nullptr};
+    float *d_g32[2] = {nullptr, nullptr};
+    uint32_t *d_F[2] = {nullptr, nullptr};
+    uint32_t *d_off[2] = {nullptr, nullptr};
+    uint16_t *d_cost[2] = {nullptr, nullptr};
+    float *d_U0 = nullptr, *d_V0 = nullptr, *d_tU = nullptr, *d_tV = nullptr, *d_Um = nullptr, *d_Vm = nullptr;
+    int field = 0;  // d_off[field] holds the last match field
+    // sequence mode
+    int have_prev = 0, cur = 0, field_valid = 0;
+    uint32_t frame_counter = 0;
+    std::vector<void *> allocs;
+    // per-stage event timing
+    struct ProfEntry {
+        int stage;
+        hipEvent_t a, b;
+    };
+    bool prof = false;
+    std::vector<ProfEntry> pending;
+    std::vector<hipEvent_t> pool;
+    double prof_ms[BM_STAGE_COUNT] = {};
+    unsigned prof_n[BM_STAGE_COUNT] = {};
+};
+
+namespace {
+
+int fail(bm_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx)
+        ctx->err = msg;
+    else
+        g_thread_error = msg;
+    return code;
+}
+
+int hip_fail(bm_ctx *ctx, hipError_t e, const char *what)
+{
+    return fail(ctx, BM_E_LAUNCH, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define BM_HIP(ctx, expr)                                        \
+    do {                                                         \
+        hipError_t e_ = (expr);                                  \
+        if (e_ != hipSuccess) return hip_fail((ctx), e_, #expr); \
+    } while (0)
+
+hipEvent_t take_event(bm_ctx *ctx)
+{
+    if (!ctx->pool.empty()) {
+        hipEvent_t e = ctx->pool.back();
+        ctx->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Enqueue one kernel; while profiling, bracket it with events on the stream.
+#define BM_LAUNCH(ctx, stage, call)                                                  \
+    do {                                                                             \
+        hipEvent_t a_ = nullptr;                                                     \
+        if ((ctx)->prof && (a_ = take_event(ctx)) != nullptr)                        \
+            (void)hipEventRecord(a_, (ctx)->stream);                                 \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) return hip_fail((ctx), e_, #call);                     \
+        if (a_) {                                                                    \
+            hipEvent_t b_ = take_event(ctx);                                         \
+            if (b_) {                                                                \
+                (void)hipEventRecord(b_, (ctx)->stream);                             \
+                (ctx)->pending.push_back({(stage), a_, b_});                         \
+            } else {                                                                 \
+                (ctx)->pool.push_back(a_);                                           \
+            }                                                                        \
+        }                                                                            \
+    } while (0)
+
+template <class T>
+int dev_alloc(bm_ctx *ctx, T **ptr, size_t count)
+{
+    void *p = nullptr;
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, BM_E_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    ctx->allocs.push_back(p);
+    *ptr = static_cast<T *>(p);
+    return BM_OK;
+}
+
+template <class T>
+int upload(bm_ctx *ctx, T *dst, const std::vector<T> &src)
+{
+    BM_HIP(ctx, hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return BM_OK;
+}
+
+unsigned next_pow2(unsigned v)
+{
+    unsigned p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+int make_gauss(float sigma, std::vector<float> &k)
+{
+    const unsigned n = (unsigned)(std::ceil(sigma * 3.0f) * 2.0f + 1.0f);
+    if (n > BM_MAX_GAUSS_TAPS) return -1;
+    const int P2 = (int)n / 2;
+    k.assign(n, 0.0f);
+    float w = 0.0f;
+    for (int i = -P2; i <= P2; ++i) {
+        k[i + P2] = std::exp(-(float)(i * i) / (2.0f * sigma * sigma));
+        w += k[i + P2];
+    }
+    for (auto &v : k) v /= w;
+    return 0;
+}
+
+void make_pattern(uint64_t seed, unsigned N, float patchArea, std::vector<int16_t> &pat)
+{
+    pat.assign(4 * (size_t)N, 0);
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (unsigned s = 0; s < N; ++s) {
+        const uint32_t ctr[4] = {s, 0u, 0u, 1u};
+        uint32_t r[4];
+        philox_host(ctr, key, r);
+        for (int c = 0; c < 4; ++c) {
+            const float v = 0.2f * ((float)kNdtriQ16[r[c] >> 20] * (1.0f / 65536.0f));
+            const float sq = v * v;
+            float val = v > 0.0f ? sq : -sq;
+            val = std::max(-1.0f, std::min(1.0f, val));
+            pat[4 * s + c] = (int16_t)(int)(patchArea * val);
+        }
+    }
+}
+
+int set_pattern_internal(bm_ctx *ctx, const std::vector<int16_t> &pat)
+{
+    ctx->pattern = pat;
+    int rmax = 0;
+    for (int16_t v : pat) rmax = std::max(rmax, std::abs((int)v));
+    ctx->g.brief_radius = std::min(rmax, bm::brief_max_radius());
+    if (ctx->d_pattern) return upload(ctx, ctx->d_pattern, pat);
+    return BM_OK;
+}
+
+int check(bm_ctx *ctx, const void *p, const char *name)
+{
+    if (!ctx) return fail(nullptr, BM_E_PARAM, "null context");
+    if (!p) return fail(ctx, BM_E_PARAM, std::string("null pointer: ") + name);
+    return BM_OK;
+}
+
+// Ping-pong propagation of one iteration over the context's field buffers,
+// starting from ctx->field; the random search of `iteration` is fused into the
+// last level when fuse_rs.
+int run_iteration(bm_ctx *ctx, const uint32_t *F1, const uint32_t *F2, int pd, int iteration, uint32_t pair,
+                  bool fuse_rs)
+{
+    const bm::Geometry &g = ctx->g;
+    for (int p = g.jfMax; p > 0; p /= 2) {
+        const int rs = (fuse_rs && p == 1) ? iteration : -1;
+        const int a = ctx->field, b = 1 - a;
+        BM_LAUNCH(ctx, (rs >= 0 ? BM_STAGE_LEVEL_RS : BM_STAGE_LEVEL), bm::launch_level(g, F1, F2, ctx->d_off[a], ctx->d_cost[a], ctx->d_off[b], ctx->d_cost[b], p, pd,
+                                     rs, pair, ctx->stream));
+        ctx->field = b;
+    }
+    if (g.jfMax == 0 && fuse_rs) {
+        BM_LAUNCH(ctx, BM_STAGE_RANDOM_SEARCH, bm::launch_random_search(g, F1, F2, ctx->d_off[ctx->field], ctx->d_cost[ctx->field],
+                                             (uint32_t)iteration, pair, ctx->stream));
+    }
+    return BM_OK;
+}
+
+// init + iterations into the context field buffers.
+int run_match(bm_ctx *ctx, const uint32_t *F1, const uint32_t *F2, int first, const uint32_t *prev, uint32_t pair)
+{
+    const bm::Geometry &g = ctx->g;
+    int dst = ctx->field;
+    if (!first && prev == ctx->d_off[dst]) dst = 1 - dst;
+    BM_LAUNCH(ctx, BM_STAGE_INIT, bm::launch_init(g, F1, F2, first, prev, ctx->d_off[dst], ctx->d_cost[dst], ctx->stream));
+    ctx->field = dst;
+    int pd = -1;
+    for (int i = 0; i < g.iterations; ++i) {
+        int r = run_iteration(ctx, F1, F2, pd, i, pair, true);
+        if (r != BM_OK) return r;
+        pd = -pd;
+    }
+    return BM_OK;
+}
+
+int run_refine(bm_ctx *ctx, const float *U, const float *V, const float *I, float *Uo, float *Vo)
+{
+    const bm::Geometry &g = ctx->g;
+    BM_LAUNCH(ctx, BM_STAGE_MEDIAN, bm::launch_median(g, U, V, ctx->d_tU, ctx->d_tV, 1, ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_MEDIAN, bm::launch_median(g, ctx->d_tU, ctx->d_tV, ctx->d_Um, ctx->d_Vm, 0, ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_TRILATERAL, bm::launch_trilateral(g, U, V, ctx->d_Um, ctx->d_Vm, I, Uo, Vo, ctx->stream));
+    return BM_OK;
+}
+
+int run_flow_tail(bm_ctx *ctx, const float *guide, float *U, float *V)
+{
+    const bm::Geometry &g = ctx->g;
+    if (!ctx->p.filter) {
+        BM_LAUNCH(ctx, BM_STAGE_DOWNSAMPLE, bm::launch_downsample(g, ctx->d_off[ctx->field], U, V, ctx->stream));
+        return BM_OK;
+    }
+    BM_LAUNCH(ctx, BM_STAGE_DOWNSAMPLE, bm::launch_downsample(g, ctx->d_off[ctx->field], ctx->d_U0, ctx->d_V0, ctx->stream));
+    return run_refine(ctx, ctx->d_U0, ctx->d_V0, guide, U, V);
+}
+
+}  // namespace
+
+extern "C" {
+
+void bm_params_default(bm_params *p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->sx = p->sy = p->sx_in = p->sy_in = 64;
+    p->iterations = 2;
+    p->jfMax = 256;
+    p->doubleProp = 1;
+    p->filter = 1;
+    p->tInit = 1;
+    p->sigma = 0.5f;
+    p->P = 11;
+    p->fsize = 13;
+    p->N = 128;
+    p->patchArea = 75.0f;
+    p->sEPE = 0.4f;
+    p->sI = 0.1f;
+    p->sS = 6.0f;
+    p->maxmot = 5.0f;
+}
+
+int bm_params_derive(unsigned W, unsigned H, float ux, float uy, float patchRad, float sigmaRad, float fsizeRad,
+                     float sSRad, bm_params *p)
+{
+    if (!p) return fail(nullptr, BM_E_PARAM, "null params");
+    if (W == 0 || H == 0) return fail(nullptr, BM_E_PARAM, "empty input resolution");
+    bm_params_default(p);
+    if (ux < 0.1f) ux = 2.740f;
+    if (uy < 0.1f) uy = 3.052f;
+    const unsigned sx = (unsigned)(ux * (float)W), sy = (unsigned)(uy * (float)H);
+    p->sx = sx;
+    p->sy = sy;
+    p->sx_in = W;
+    p->sy_in = H;
+    const double diag = std::sqrt((double)(sx * sx + sy * sy));
+    const double diag_in = std::sqrt((double)(W * W + H * H));
+    p->patchArea = (float)((double)(0.01f * patchRad) * diag);
+    p->sigma = (float)((double)(0.01f * sigmaRad) * diag);
+    p->fsize = (unsigned)(2.0 * std::round((double)(0.005f * fsizeRad) * diag_in) + 1.0);
+    p->sS = (float)((double)(0.01f * sSRad) * diag_in);
+    p->P = (unsigned)(std::ceil(p->sigma * 3.0f) * 2.0f + 1.0f);
+    return BM_OK;
+}
+
+const char *bm_last_error(const bm_ctx *ctx) { return ctx ? ctx->err.c_str() : g_thread_error.c_str(); }
+
+int bm_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    if (!ctr || !key || !out) return fail(nullptr, BM_E_PARAM, "null pointer");
+    philox_host(ctr, key, out);
+    return BM_OK;
+}
+
+int bm_ctx_create(const bm_params *params, uint64_t seed, bm_ctx **out)
+{
+    if (!params || !out) return fail(nullptr, BM_E_PARAM, "null argument");
+    *out = nullptr;
+    const bm_params &p = *params;
+    char m[512];
+    if (p.N < 2 || p.N > BM_MAX_FEATURE_BITS) {
+        std::snprintf(m, sizeof m, "Feature length exceeds maximum value. Current size is %u and maximum is %d.", p.N,
+                      BM_MAX_FEATURE_BITS);
+        return fail(nullptr, BM_E_PARAM, m);
+    }
+    if (p.fsize > BM_MAX_MEDIAN) {
+        std::snprintf(m, sizeof m, "Median filter size exceeds maximum value. Current size is %u and maximum is %d.",
+                      p.fsize, BM_MAX_MEDIAN);
+        return fail(nullptr, BM_E_PARAM, m);
+    }
+    if (p.filter && (p.fsize % 2) == 0)
+        return fail(nullptr, BM_E_PARAM, "Median filter size must be odd.");
+    if (std::ceil(p.sigma * 3.0f) * 2.0f + 1.0f > BM_MAX_GAUSS_TAPS) {
+        std::snprintf(m, sizeof m,
+                      "Maximum size of Gaussian down-sampling filter exceeded. Currently using sigma = %0.4f and "
+                      "maximum is %0.4f.",
+                      p.sigma, std::floor((BM_MAX_GAUSS_TAPS - 1.0f) / 2.0f) / 3.0f);
+        return fail(nullptr, BM_E_PARAM, m);
+    }
+    if (p.sx_in == 0 || p.sy_in == 0 || p.sx == 0 || p.sy == 0)
+        return fail(nullptr, BM_E_PARAM, "empty resolution");
+    if (p.sx > 32767 || p.sy > 32767)
+        return fail(nullptr, BM_E_PARAM, "matched resolution above 32767 (offsets are int16)");
+    if (p.sigma <= 0.0f) return fail(nullptr, BM_E_PARAM, "down-sampling sigma must be positive");
+    if (p.filter && (p.sEPE <= 0.0f || p.sI <= 0.0f || p.sS <= 0.0f))
+        return fail(nullptr, BM_E_PARAM, "trilateral sigmas must be positive");
+    const size_t M = (size_t)p.sx * p.sy;
+    const unsigned NS = (p.N + 31) / 32, NSP = next_pow2(NS);
+    if ((double)M * NSP >= 4294967296.0 || (double)p.jfMax * p.sx + (double)M >= 2147483647.0)
+        return fail(nullptr, BM_E_PARAM, "matched resolution too large for 32-bit indexing");
+
+    bm_ctx *ctx = new (std::nothrow) bm_ctx;
+    if (!ctx) return fail(nullptr, BM_E_ALLOC, "out of host memory");
+    ctx->p = p;
+    bm::Geometry &g = ctx->g;
+    g.W = (int)p.sx_in;
+    g.H = (int)p.sy_in;
+    g.sx = (int)p.sx;
+    g.sy = (int)p.sy;
+    g.N = (int)p.N;
+    g.NS = (int)NS;
+    g.NSP = (int)NSP;
+    g.fsize = (int)p.fsize;
+    g.jfMax = (int)p.jfMax;
+    g.iterations = (int)p.iterations;
+    g.doubleProp = p.doubleProp ? 1 : 0;
+    g.seed = seed;
+    const float rs_sigma = std::sqrt(std::ceil((float)p.sx * 0.5f / 3.0f));
+    g.rs_sigma_q16 = (int64_t)std::llround((double)rs_sigma * 65536.0);
+
+    make_gauss(p.sigma, ctx->gauss);
+    g.P = (int)ctx->gauss.size();
+    ctx->p.P = (unsigned)g.P;
+    std::vector<int16_t> pat;
+    make_pattern(seed, p.N, p.patchArea, pat);
+    set_pattern_internal(ctx, pat);
+
+    // trilateral constants (sigmas squared as BriefMatch::process passes them)
+    const float sE = p.sEPE * p.sEPE, sI = p.sI * p.sI, sS = p.sS * p.sS;
+    g.tri_sigma_epe = sE;
+    g.tri_sigma_i = sI;
+    g.tri_sigma_s = sS;
+    const double log2e = 1.4426950408889634;
+    g.tri_s = p.filter ? (int)std::ceil(2.5 * (double)std::sqrt(sS)) : 0;
+    g.tri_a = p.filter ? (float)std::sqrt(log2e / (2.0 * (double)sE)) : 1.0f;
+    g.tri_b = p.filter ? (float)(std::sqrt(log2e / (2.0 * (double)sI)) / 255.0) : 1.0f;
+    if (p.filter && (g.tri_s < 1 || g.tri_s > 2000)) {
+        delete ctx;
+        return fail(nullptr, BM_E_PARAM, "trilateral window outside [1, 2000]");
+    }
+    std::vector<float> spt(std::max(2 * g.tri_s, 1));
+    for (int m_ = 0; m_ < 2 * g.tri_s; ++m_) {
+        const double i = m_ - g.tri_s;
+        spt[m_] = (float)(log2e * i * i / (2.0 * (double)sS));
+    }
+
+    // coordinate tables
+    std::vector<uint32_t> upx(p.sx), upy(p.sy), nnx(p.sx_in), nny(p.sy_in);
+    for (unsigned x = 0; x < p.sx; ++x) upx[x] = (uint32_t)((uint64_t)x * p.sx_in * 256u / p.sx);
+    for (unsigned y = 0; y < p.sy; ++y) upy[y] = (uint32_t)((uint64_t)y * p.sy_in * 256u / p.sy);
+    for (unsigned x = 0; x < p.sx_in; ++x)
+        nnx[x] = (uint32_t)std::min<uint64_t>((2ull * x * p.sx + p.sx_in) / (2ull * p.sx_in), p.sx - 1);
+    for (unsigned y = 0; y < p.sy_in; ++y)
+        nny[y] = (uint32_t)std::min<uint64_t>((2ull * y * p.sy + p.sy_in) / (2ull * p.sy_in), p.sy - 1);
+
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+    if (e != hipSuccess) {
+        std::string msg = std::string("HIP initialisation failed: ") + hipGetErrorString(e);
+        bm_ctx_destroy(ctx);
+        return fail(nullptr, BM_E_LAUNCH, msg);
+    }
+
+    const size_t Win = (size_t)p.sx_in * p.sy_in;
+    int r = BM_OK;
+#define BM_ALLOC(ptr, n) \
+    if (r == BM_OK) r = dev_alloc(ctx, &(ptr), (n))
+    BM_ALLOC(ctx->d_pattern, pat.size());
+    BM_ALLOC(ctx->d_up_x, upx.size());
+    BM_ALLOC(ctx->d_up_y, upy.size());
+    BM_ALLOC(ctx->d_nn_x, nnx.size());
+    BM_ALLOC(ctx->d_nn_y, nny.size());
+    BM_ALLOC(ctx->d_gauss, ctx->gauss.size());
+    BM_ALLOC(ctx->d_spt, spt.size());
+    for (int k = 0; k < 2; ++k) {
+        BM_ALLOC(ctx->d_g8[k], Win);
+        BM_ALLOC(ctx->d_g32[k], Win);
+        BM_ALLOC(ctx->d_F[k], M * NSP);
+        BM_ALLOC(ctx->d_off[k], M);
+        BM_ALLOC(ctx->d_cost[k], M);
+    }
+    BM_ALLOC(ctx->d_U0, Win);
+    BM_ALLOC(ctx->d_V0, Win);
+    BM_ALLOC(ctx->d_tU, Win);
+    BM_ALLOC(ctx->d_tV, Win);
+    BM_ALLOC(ctx->d_Um, Win);
+    BM_ALLOC(ctx->d_Vm, Win);
+#undef BM_ALLOC
+    if (r == BM_OK) r = upload(ctx, ctx->d_pattern, pat);
+    if (r == BM_OK) r = upload(ctx, ctx->d_up_x, upx);
+    if (r == BM_OK) r = upload(ctx, ctx->d_up_y, upy);
+    if (r == BM_OK) r = upload(ctx, ctx->d_nn_x, nnx);
+    if (r == BM_OK) r = upload(ctx, ctx->d_nn_y, nny);
+    if (r == BM_OK) r = upload(ctx, ctx->d_gauss, ctx->gauss);
+    if (r == BM_OK) r = upload(ctx, ctx->d_spt, spt);
+    if (r != BM_OK) {
+        std::string msg = ctx->err;
+        bm_ctx_destroy(ctx);
+        return fail(nullptr, r, msg);
+    }
+    g.pattern = ctx->d_pattern;
+    g.up_x = ctx->d_up_x;
+    g.up_y = ctx->d_up_y;
+    g.nn_x = ctx->d_nn_x;
+    g.nn_y = ctx->d_nn_y;
+    g.gauss = ctx->d_gauss;
+    g.tri_spatial = ctx->d_spt;
+    *out = ctx;
+    return BM_OK;
+}
+
+void bm_ctx_destroy(bm_ctx *ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void *p : ctx->allocs) (void)hipFree(p);
+    for (auto &pe : ctx->pending) {
+        (void)hipEventDestroy(pe.a);
+        (void)hipEventDestroy(pe.b);
+    }
+    for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int bm_ctx_get_params(const bm_ctx *ctx, bm_params *out)
+{
+    if (!ctx || !out) return fail(nullptr, BM_E_PARAM, "null argument");
+    *out = ctx->p;
+    return BM_OK;
+}
+
+void *bm_ctx_stream(bm_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int bm_sync(bm_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, BM_E_PARAM, "null context");
+    BM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return BM_OK;
+}
+
+unsigned bm_descriptor_stride(const bm_ctx *ctx) { return ctx ? (unsigned)ctx->g.NSP : 0u; }
+
+int bm_get_pattern(const bm_ctx *ctx, int16_t *out)
+{
+    if (!ctx || !out) return fail(nullptr, BM_E_PARAM, "null argument");
+    std::memcpy(out, ctx->pattern.data(), ctx->pattern.size() * sizeof(int16_t));
+    return BM_OK;
+}
+
+int bm_set_pattern(bm_ctx *ctx, const int16_t *pat)
+{
+    if (!ctx || !pat) return fail(nullptr, BM_E_PARAM, "null argument");
+    std::vector<int16_t> v(pat, pat + 4 * (size_t)ctx->p.N);
+    BM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return set_pattern_internal(ctx, v);
+}
+
+int bm_get_gauss_kernel(const bm_ctx *ctx, float *out, unsigned *P)
+{
+    if (!ctx || !out || !P) return fail(nullptr, BM_E_PARAM, "null argument");
+    std::memcpy(out, ctx->gauss.data(), ctx->gauss.size() * sizeof(float));
+    *P = (unsigned)ctx->gauss.size();
+    return BM_OK;
+}
+
+int bm_ctx_field(bm_ctx *ctx, uint32_t **off, uint16_t **cost)
+{
+    if (!ctx || !off || !cost) return fail(nullptr, BM_E_PARAM, "null argument");
+    *off = ctx->d_off[ctx->field];
+    *cost = ctx->d_cost[ctx->field];
+    return BM_OK;
+}
+
+int bm_malloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return fail(nullptr, BM_E_PARAM, "null argument");
+    hipError_t e = hipMalloc(ptr, std::max<size_t>(bytes, 1));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        return fail(nullptr, BM_E_ALLOC, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    return BM_OK;
+}
+
+int bm_free(void *ptr)
+{
+    if (!ptr) return BM_OK;
+    // the buffer may still be read by queued work on any stream
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipFree(ptr);
+    if (e != hipSuccess) return fail(nullptr, BM_E_LAUNCH, std::string("hipFree failed: ") + hipGetErrorString(e));
+    return BM_OK;
+}
+
+int bm_memcpy(void *dst, const void *src, size_t bytes, int kind)
+{
+    if ((!dst || !src) && bytes) return fail(nullptr, BM_E_PARAM, "null pointer");
+    hipMemcpyKind k;
+    switch (kind) {
+    case BM_HOST_TO_DEVICE: k = hipMemcpyHostToDevice; break;
+    case BM_DEVICE_TO_HOST: k = hipMemcpyDeviceToHost; break;
+    case BM_DEVICE_TO_DEVICE: k = hipMemcpyDeviceToDevice; break;
+    case BM_HOST_TO_HOST: k = hipMemcpyHostToHost; break;
+    default: return fail(nullptr, BM_E_PARAM, "unknown copy kind");
+    }
+    hipError_t e = hipMemcpy(dst, src, bytes, k);
+    if (e != hipSuccess) return fail(nullptr, BM_E_LAUNCH, std::string("hipMemcpy failed: ") + hipGetErrorString(e));
+    return BM_OK;
+}
+
+int bm_memset(void *ptr, int value, size_t bytes)
+{
+    if (!ptr && bytes) return fail(nullptr, BM_E_PARAM, "null pointer");
+    hipError_t e = hipMemset(ptr, value, bytes);
+    if (e != hipSuccess) return fail(nullptr, BM_E_LAUNCH, std::string("hipMemset failed: ") + hipGetErrorString(e));
+    return BM_OK;
+}
+
+int bm_gray(bm_ctx *ctx, const uint8_t *rgb, uint8_t *g8, float *g32)
+{
+    int r = check(ctx, rgb, "rgb");
+    if (r) return r;
+    if (!g8 && !g32) return fail(ctx, BM_E_PARAM, "no output");
+    BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(ctx->g, rgb, g8, g32, ctx->stream));
+    return BM_OK;
+}
+
+int bm_upsample(bm_ctx *ctx, const uint8_t *g8, float *L)
+{
+    int r = check(ctx, g8, "gray8");
+    if (!r) r = check(ctx, L, "L");
+    if (r) return r;
+    BM_LAUNCH(ctx, BM_STAGE_UPSAMPLE, bm::launch_upsample(ctx->g, g8, L, ctx->stream));
+    return BM_OK;
+}
+
+int bm_brief(bm_ctx *ctx, const float *L, uint32_t *F)
+{
+    int r = check(ctx, L, "L");
+    if (!r) r = check(ctx, F, "F");
+    if (r) return r;
+    BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(ctx->g, L, nullptr, F, ctx->stream));
+    return BM_OK;
+}
+
+int bm_brief_gray(bm_ctx *ctx, const uint8_t *g8, uint32_t *F)
+{
+    int r = check(ctx, g8, "gray8");
+    if (!r) r = check(ctx, F, "F");
+    if (r) return r;
+    BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(ctx->g, nullptr, g8, F, ctx->stream));
+    return BM_OK;
+}
+
+int bm_init(bm_ctx *ctx, const uint32_t *F1, const uint32_t *F2, int first, const uint32_t *prev, uint32_t *off,
+            uint16_t *cost)
+{
+    int r = check(ctx, F1, "F1");
+    if (!r) r = check(ctx, F2, "F2");
+    if (!r) r = check(ctx, off, "off");
+    if (!r) r = check(ctx, cost, "cost");
+    if (r) return r;
+    if (!first && (!prev || prev == off)) return fail(ctx, BM_E_PARAM, "temporal init needs a separate prev_off");
+    BM_LAUNCH(ctx, BM_STAGE_INIT, bm::launch_init(ctx->g, F1, F2, first, prev, off, cost, ctx->stream));
+    return BM_OK;
+}
+
+int bm_propagate_level(bm_ctx *ctx, const uint32_t *F1, const uint32_t *F2, const uint32_t *oi, const uint16_t *ci,
+                       uint32_t *oo, uint16_t *co, int p, int pd)
+{
+    int r = check(ctx, F1, "F1");
+    if (!r) r = check(ctx, F2, "F2");
+    if (!r) r = check(ctx, oi, "off_in");
+    if (!r) r = check(ctx, ci, "cost_in");
+    if (!r) r = check(ctx, oo, "off_out");
+    if (!r) r = check(ctx, co, "cost_out");
+    if (r) return r;
+    if ((const void *)oi == (const void *)oo || (const void *)ci == (const void *)co)
+        return fail(ctx, BM_E_PARAM, "level input and output must differ");
+    if (p < 1 || (pd != 1 && pd != -1)) return fail(ctx, BM_E_PARAM, "bad level step or direction");
+    BM_LAUNCH(ctx, BM_STAGE_LEVEL, bm::launch_level(ctx->g, F1, F2, oi, ci, oo, co, p, pd, -1, 0, ctx->stream));
+    return BM_OK;
+}
+
+int bm_propagate(bm_ctx *ctx, const uint32_t *F1, const uint32_t *F2, uint32_t *off, uint16_t *cost, int pd)
+{
+    int r = check(ctx, F1, "F1");
+    if (!r) r = check(ctx, F2, "F2");
+    if (!r) r = check(ctx, off, "off");
+    if (!r) r = check(ctx, cost, "cost");
+    if (r) return r;
+    if (pd != 1 && pd != -1) return fail(ctx, BM_E_PARAM, "pd must be +1 or -1");
+    const size_t M = (size_t)ctx->g.sx * ctx->g.sy;
+    // the caller's field is the first snapshot; levels ping-pong through the context
+    uint32_t *o[2] = {off, ctx->d_off[1 - ctx->field]};
+    uint16_t *c[2] = {cost, ctx->d_cost[1 - ctx->field]};
+    int a = 0;
+    for (int p = ctx->g.jfMax; p > 0; p /= 2) {
+        BM_LAUNCH(ctx, BM_STAGE_LEVEL, bm::launch_level(ctx->g, F1, F2, o[a], c[a], o[1 - a], c[1 - a], p, pd, -1, 0, ctx->stream));
+        a = 1 - a;
+    }
+    if (a != 0) {
+        BM_HIP(ctx, hipMemcpyAsync(off, o[a], M * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
+        BM_HIP(ctx, hipMemcpyAsync(cost, c[a], M * sizeof(uint16_t), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return BM_OK;
+}
+
+int bm_random_search(bm_ctx *ctx, const uint32_t *F1, const uint32_t *F2, uint32_t *off, uint16_t *cost,
+                     uint32_t iteration, uint32_t pair)
+{
+    int r = check(ctx, F1, "F1");
+    if (!r) r = check(ctx, F2, "F2");
+    if (!r) r = check(ctx, off, "off");
+    if (!r) r = check(ctx, cost, "cost");
+    if (r) return r;
+    BM_LAUNCH(ctx, BM_STAGE_RANDOM_SEARCH, bm::launch_random_search(ctx->g, F1, F2, off, cost, iteration, pair, ctx->stream));
+    return BM_OK;
+}
+
+int bm_match(bm_ctx *ctx, const uint32_t *F1, const uint32_t *F2, int first, const uint32_t *prev, uint32_t pair,
+             uint32_t *off, uint16_t *cost)
+{
+    int r = check(ctx, F1, "F1");
+    if (!r) r = check(ctx, F2, "F2");
+    if (r) return r;
+    if (!first && !prev) return fail(ctx, BM_E_PARAM, "temporal init needs prev_off");
+    r = run_match(ctx, F1, F2, first, prev, pair);
+    if (r) return r;
+    const size_t M = (size_t)ctx->g.sx * ctx->g.sy;
+    if (off && off != ctx->d_off[ctx->field])
+        BM_HIP(ctx, hipMemcpyAsync(off, ctx->d_off[ctx->field], M * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+    if (cost && cost != ctx->d_cost[ctx->field])
+        BM_HIP(ctx, hipMemcpyAsync(cost, ctx->d_cost[ctx->field], M * sizeof(uint16_t), hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+    return BM_OK;
+}
+
+int bm_downsample(bm_ctx *ctx, const uint32_t *off, float *U, float *V)
+{
+    int r = check(ctx, off, "off");
+    if (!r) r = check(ctx, U, "U");
+    if (!r) r = check(ctx, V, "V");
+    if (r) return r;
+    BM_LAUNCH(ctx, BM_STAGE_DOWNSAMPLE, bm::launch_downsample(ctx->g, off, U, V, ctx->stream));
+    return BM_OK;
+}
+
+int bm_median1d(bm_ctx *ctx, const float *in, float *out, int dir)
+{
+    int r = check(ctx, in, "in");
+    if (!r) r = check(ctx, out, "out");
+    if (r) return r;
+    if (in == out) return fail(ctx, BM_E_PARAM, "median input and output must differ");
+    BM_LAUNCH(ctx, BM_STAGE_MEDIAN, bm::launch_median(ctx->g, in, nullptr, out, nullptr, dir ? 1 : 0, ctx->stream));
+    return BM_OK;
+}
+
+int bm_trilateral(bm_ctx *ctx, const float *Ui, const float *Vi, const float *Um, const float *Vm, const float *I,
+                  float *Uo, float *Vo)
+{
+    int r = check(ctx, Ui, "Ui");
+    if (!r) r = check(ctx, Vi, "Vi");
+    if (!r) r = check(ctx, Um, "Um");
+    if (!r) r = check(ctx, Vm, "Vm");
+    if (!r) r = check(ctx, I, "I");
+    if (!r) r = check(ctx, Uo, "Uo");
+    if (!r) r = check(ctx, Vo, "Vo");
+    if (r) return r;
+    if (!ctx->p.filter) return fail(ctx, BM_E_STATE, "context created with filter = 0");
+    BM_LAUNCH(ctx, BM_STAGE_TRILATERAL, bm::launch_trilateral(ctx->g, Ui, Vi, Um, Vm, I, Uo, Vo, ctx->stream));
+    return BM_OK;
+}
+
+int bm_refine(bm_ctx *ctx, const float *U, const float *V, const float *I, float *Uo, float *Vo)
+{
+    int r = check(ctx, U, "U");
+    if (!r) r = check(ctx, V, "V");
+    if (!r) r = check(ctx, I, "I");
+    if (!r) r = check(ctx, Uo, "Uo");
+    if (!r) r = check(ctx, Vo, "Vo");
+    if (r) return r;
+    if (!ctx->p.filter) return fail(ctx, BM_E_STATE, "context created with filter = 0");
+    return run_refine(ctx, U, V, I, Uo, Vo);
+}
+
+int bm_flow_pair(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, uint32_t pair, float *U, float *V)
+{
+    int r = check(ctx, rgb1, "rgb1");
+    if (!r) r = check(ctx, rgb2, "rgb2");
+    if (!r) r = check(ctx, U, "U");
+    if (!r) r = check(ctx, V, "V");
+    if (r) return r;
+    const bm::Geometry &g = ctx->g;
+    BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb1, ctx->d_g8[0], ctx->d_g32[0], ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb2, ctx->d_g8[1], nullptr, ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(g, nullptr, ctx->d_g8[0], ctx->d_F[0], ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(g, nullptr, ctx->d_g8[1], ctx->d_F[1], ctx->stream));
+    r = run_match(ctx, ctx->d_F[0], ctx->d_F[1], 1, nullptr, pair);
+    if (r) return r;
+    // an isolated pair breaks any running sequence
+    ctx->have_prev = 0;
+    ctx->field_valid = 0;
+    return run_flow_tail(ctx, ctx->d_g32[0], U, V);
+}
+
+int bm_stream_reset(bm_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, BM_E_PARAM, "null context");
+    ctx->have_prev = 0;
+    ctx->field_valid = 0;
+    ctx->frame_counter = 0;
+    return BM_OK;
+}
+
+int bm_stream_frame(bm_ctx *ctx, const uint8_t *rgb, float *U, float *V, int *has_flow)
+{
+    int r = check(ctx, rgb, "rgb");
+    if (!r) r = check(ctx, has_flow, "has_flow");
+    if (r) return r;
+    const bm::Geometry &g = ctx->g;
+    const int cur = ctx->cur, prev = 1 - cur;
+    BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb, ctx->d_g8[cur], ctx->d_g32[cur], ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(g, nullptr, ctx->d_g8[cur], ctx->d_F[cur], ctx->stream));
+    *has_flow = 0;
+    if (ctx->have_prev) {
+        if (!U || !V) return fail(ctx, BM_E_PARAM, "null flow output");
+        const int first = (ctx->p.tInit && ctx->field_valid) ? 0 : 1;
+        r = run_match(ctx, ctx->d_F[prev], ctx->d_F[cur], first, first ? nullptr : ctx->d_off[ctx->field],
+                      ctx->frame_counter);
+        if (r) return r;
+        r = run_flow_tail(ctx, ctx->d_g32[prev], U, V);
+        if (r) return r;
+        ctx->field_valid = 1;
+        ctx->frame_counter++;
+        *has_flow = 1;
+    }
+    ctx->have_prev = 1;
+    ctx->cur = prev;
+    return BM_OK;
+}
+
+int bm_timer_start(bm_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, BM_E_PARAM, "null context");
+    BM_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    return BM_OK;
+}
+
+int bm_timer_stop(bm_ctx *ctx, float *ms)
+{
+    if (!ctx || !ms) return fail(nullptr, BM_E_PARAM, "null argument");
+    BM_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    BM_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    BM_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return BM_OK;
+}
+
+int bm_profile_enable(bm_ctx *ctx, int on)
+{
+    if (!ctx) return fail(nullptr, BM_E_PARAM, "null context");
+    ctx->prof = on != 0;
+    return BM_OK;
+}
+
+int bm_profile_read(bm_ctx *ctx, double *ms, unsigned *launches)
+{
+    if (!ctx) return fail(nullptr, BM_E_PARAM, "null context");
+    BM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto &pe : ctx->pending) {
+        float t = 0.0f;
+        BM_HIP(ctx, hipEventElapsedTime(&t, pe.a, pe.b));
+        ctx->prof_ms[pe.stage] += t;
+        ctx->prof_n[pe.stage] += 1;
+        ctx->pool.push_back(pe.a);
+        ctx->pool.push_back(pe.b);
+    }
+    ctx->pending.clear();
+    for (int i = 0; i < BM_STAGE_COUNT; ++i) {
+        if (ms) ms[i] = ctx->prof_ms[i];
+        if (launches) launches[i] = ctx->prof_n[i];
+        ctx->prof_ms[i] = 0.0;
+        ctx->prof_n[i] = 0;
+    }
+    return BM_OK;
+}
+
+int bm_device_copy(bm_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    int r = check(ctx, dst, "dst");
+    if (!r) r = check(ctx, src, "src");
+    if (r) return r;
+    if (bytes % 16) return fail(ctx, BM_E_PARAM, "copy size must be a multiple of 16 bytes");
+    BM_LAUNCH(ctx, BM_STAGE_COPY, bm::launch_copy(src, dst, bytes, ctx->stream));
+    return BM_OK;
+}
+
+int bm_epe(const float *U, const float *V, const float *gu, const float *gv, const uint8_t *occ, size_t n,
+           double *epe_all, double *epe_noc)
+{
+    if (!U || !V || !gu || !gv || !epe_all || !epe_noc) return fail(nullptr, BM_E_PARAM, "null argument");
+    double sa = 0.0, sn = 0.0;
+    size_t na = 0, nn = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const float a = gu[i], b = gv[i];
+        if (std::fabs(a) > 1e9f || std::fabs(b) > 1e9f || std::isnan(a) || std::isnan(b)) continue;
+        const double du = (double)a - U[i], dv = (double)b - V[i];
+        const double e = std::sqrt(du * du + dv * dv);
+        sa += e;
+        ++na;
+        if (!occ || !occ[i]) {
+            sn += e;
+            ++nn;
+        }
+    }
+    *epe_all = na ? sa / (double)na : 0.0;
+    *epe_noc = nn ? sn / (double)nn : 0.0;
+    return BM_OK;
+}
+
+}  // extern "C"

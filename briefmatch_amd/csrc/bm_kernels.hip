@@ -1,0 +1,740 @@
+// HIP kernels of the BriefMatch flow path for gfx950 (CDNA4, wave64).
+//
+// Every stage of the reference's device code (kernels/briefmatch_cuda.cu,
+// kernels/util_cuda.cu) has a kernel here, written for this hardware:
+//   * one work-group per tile, one-dimensional grids whose work-group order is
+//     remapped so each of the 8 XCDs (own L2) walks a contiguous band of rows;
+//   * BRIEF stages the (up-sampled) image tile plus a halo in LDS, and the
+//     fused variant computes the bilinear up-sampling while filling LDS so the
+//     up-sampled frame never reaches HBM;
+//   * jump flooding is level-synchronous (one launch per level, ping-pong
+//     buffers), with all four candidate gathers issued before any cost is
+//     compared, and the random search fused into the last level;
+//   * the down-sampling chain is one kernel that evaluates the separable
+//     Gaussian only at the nearest-neighbour sample points;
+//   * the trilateral filter streams bands of pre-scaled (u, v, I) through LDS
+//     and evaluates one exp2 per tap.
+#include <algorithm>
+
+#include "bm_internal.h"
+
+namespace bm {
+namespace {
+
+__device__ const int32_t d_ndtri_q16[4096] = {
+#include "bm_ndtri_q16.inc"
+};
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+__device__ __forceinline__ uint32_t pack_off(int dx, int dy)
+{
+    return (uint32_t)(uint16_t)dx | ((uint32_t)(uint16_t)dy << 16);
+}
+__device__ __forceinline__ int off_dx(uint32_t o) { return (int)(int16_t)(o & 0xffffu); }
+__device__ __forceinline__ int off_dy(uint32_t o) { return (int)(int16_t)(o >> 16); }
+
+// Work-groups are handed to the 8 XCDs round-robin in dispatch order.  Give
+// XCD k the k-th contiguous slice of the tile order instead, so that the tiles
+// sharing one L2 are neighbours in the image.
+__device__ __forceinline__ int xcd_remap(int b, int B)
+{
+    const int q = B >> 3, r = B & 7, k = b & 7;
+    return k * q + min(k, r) + (b >> 3);
+}
+
+__device__ __forceinline__ void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3,
+                                              uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = c0 * 0xD2511F53u, hi0 = __umulhi(c0, 0xD2511F53u);
+        const uint32_t lo1 = c2 * 0xCD9E8D57u, hi1 = __umulhi(c2, 0xCD9E8D57u);
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// round-half-away-from-zero of a Q32 product
+__device__ __forceinline__ int round_q32(int64_t v)
+{
+    const int64_t a = v < 0 ? -v : v;
+    const int64_t r = (a + (int64_t(1) << 31)) >> 32;
+    return (int)(v < 0 ? -r : r);
+}
+
+template <int NSP>
+__device__ __forceinline__ void load_desc(const uint32_t *__restrict__ F, uint32_t idx, uint32_t (&w)[NSP])
+{
+    const uint32_t *p = F + (size_t)idx * NSP;
+    if constexpr (NSP == 1) {
+        w[0] = p[0];
+    } else if constexpr (NSP == 2) {
+        const uint2 v = *reinterpret_cast<const uint2 *>(p);
+        w[0] = v.x;
+        w[1] = v.y;
+    } else {
+#pragma unroll
+        for (int k = 0; k < NSP / 4; ++k) {
+            const uint4 v = reinterpret_cast<const uint4 *>(p)[k];
+            w[4 * k + 0] = v.x;
+            w[4 * k + 1] = v.y;
+            w[4 * k + 2] = v.z;
+            w[4 * k + 3] = v.w;
+        }
+    }
+}
+
+template <int NSP>
+__device__ __forceinline__ void store_desc(uint32_t *__restrict__ F, uint32_t idx, const uint32_t (&w)[NSP])
+{
+    uint32_t *p = F + (size_t)idx * NSP;
+    if constexpr (NSP == 1) {
+        p[0] = w[0];
+    } else if constexpr (NSP == 2) {
+        *reinterpret_cast<uint2 *>(p) = make_uint2(w[0], w[1]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NSP / 4; ++k)
+            reinterpret_cast<uint4 *>(p)[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+    }
+}
+
+template <int NSP>
+__device__ __forceinline__ int hamming(const uint32_t (&a)[NSP], const uint32_t (&b)[NSP])
+{
+    int d = 0;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) d += __popc(a[k] ^ b[k]);
+    return d;
+}
+
+// Fixed-point bilinear sample (Q16 gray levels) at the packed (x0 << 8 | frac)
+// coordinates of one up-sampled column / row.
+__device__ __forceinline__ uint32_t bilinear_q16(const uint8_t *__restrict__ g8, int W, int H, uint32_t ux,
+                                                 uint32_t uy)
+{
+    const int x0 = (int)(ux >> 8), y0 = (int)(uy >> 8);
+    const uint32_t ax = ux & 255u, ay = uy & 255u;
+    const int x1 = min(x0 + 1, W - 1), y1 = min(y0 + 1, H - 1);
+    const uint8_t *r0 = g8 + (size_t)y0 * W;
+    const uint8_t *r1 = g8 + (size_t)y1 * W;
+    const uint32_t top = r0[x0] * (256u - ax) + r0[x1] * ax;
+    const uint32_t bot = r1[x0] * (256u - ax) + r1[x1] * ax;
+    return top * (256u - ay) + bot * ay;
+}
+
+// ---------------------------------------------------------------------------
+// gray: integer BT.601 of interleaved RGB
+
+__global__ void __launch_bounds__(kThreads) k_gray(const uint8_t *__restrict__ rgb, uint8_t *__restrict__ g8,
+                                                   float *__restrict__ g32, int n)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t r = rgb[3 * (size_t)i], g = rgb[3 * (size_t)i + 1], b = rgb[3 * (size_t)i + 2];
+    const uint32_t y = (77u * r + 150u * g + 29u * b + 128u) >> 8;
+    if (g8) g8[i] = (uint8_t)y;
+    if (g32) g32[i] = (float)y;
+}
+
+// ---------------------------------------------------------------------------
+// bilinear up-sampling (stand-alone; the matching path uses the fused BRIEF)
+
+__global__ void __launch_bounds__(kThreads) k_upsample(const uint8_t *__restrict__ g8,
+                                                       const uint32_t *__restrict__ up_x,
+                                                       const uint32_t *__restrict__ up_y, int W, int H,
+                                                       int sx, int sy, float *__restrict__ L)
+{
+    const int gx = (sx + kThreads - 1) / kThreads;
+    const int b = xcd_remap(blockIdx.x, gx * sy);
+    const int x = (b % gx) * kThreads + threadIdx.x, y = b / gx;
+    if (x >= sx) return;
+    L[(size_t)y * sx + x] = (float)bilinear_q16(g8, W, H, up_x[x], up_y[y]) * (1.0f / 65536.0f);
+}
+
+// ---------------------------------------------------------------------------
+// BRIEF.  A 256-thread work-group describes a 64 x 16 tile; thread (lx, ly)
+// owns pixels (lx, ly + 4k), k = 0..3, so each wave covers one image row per k.
+// The tile plus an R-pixel halo sits in LDS (clamped to the image, which is
+// exactly the reference's per-axis offset clamp).  Tests whose four offsets all
+// lie within R read LDS; the others (a wave-uniform branch, because the pattern
+// is shared by all pixels) read the image in global memory.
+
+constexpr int kBriefTW = 64, kBriefTH = 16, kBriefPY = 4;
+constexpr int kBriefLdsBytes = 64 * 1024;
+
+template <int NSP, bool FUSED>
+__global__ void __launch_bounds__(kThreads) k_brief(const float *__restrict__ L, const uint8_t *__restrict__ g8,
+                                                    const uint32_t *__restrict__ up_x,
+                                                    const uint32_t *__restrict__ up_y,
+                                                    const int16_t *__restrict__ pattern, int N, int W, int H,
+                                                    int sx, int sy, int R, uint32_t *__restrict__ F)
+{
+    extern __shared__ float tile[];
+    const int pitch = kBriefTW + 2 * R;
+    const int rows = kBriefTH + 2 * R;
+    const int gxb = (sx + kBriefTW - 1) / kBriefTW, gyb = (sy + kBriefTH - 1) / kBriefTH;
+    const int b = xcd_remap(blockIdx.x, gxb * gyb);
+    const int tx0 = (b % gxb) * kBriefTW, ty0 = (b / gxb) * kBriefTH;
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+
+    for (int r = ly; r < rows; r += 4) {
+        const int gy = clampi(ty0 - R + r, 0, sy - 1);
+        for (int c = lx; c < pitch; c += 64) {
+            const int gx = clampi(tx0 - R + c, 0, sx - 1);
+            float v;
+            if constexpr (FUSED)
+                v = (float)bilinear_q16(g8, W, H, up_x[gx], up_y[gy]) * (1.0f / 65536.0f);
+            else
+                v = L[(size_t)gy * sx + gx];
+            tile[r * pitch + c] = v;
+        }
+    }
+    __syncthreads();
+
+    const int x = tx0 + lx;
+    const int base = (ly + R) * pitch + lx + R;
+    const uint2 *pat = reinterpret_cast<const uint2 *>(pattern);
+    uint32_t words[NSP][kBriefPY];
+#pragma unroll
+    for (int w = 0; w < NSP; ++w)
+#pragma unroll
+        for (int k = 0; k < kBriefPY; ++k) words[w][k] = 0u;
+
+#pragma unroll
+    for (int w = 0; w < NSP; ++w) {
+        const int nb = min(32, N - 32 * w);
+        for (int bit = nb - 1; bit >= 0; --bit) {
+            const uint2 pv = pat[32 * w + bit];
+            const int ox1 = (int16_t)(pv.x & 0xffffu), oy1 = (int16_t)(pv.x >> 16);
+            const int ox2 = (int16_t)(pv.y & 0xffffu), oy2 = (int16_t)(pv.y >> 16);
+            const bool inside = max(abs(ox1), abs(ox2)) <= R && max(abs(oy1), abs(oy2)) <= R;
+            if (inside) {
+                const int o1 = base + oy1 * pitch + ox1, o2 = base + oy2 * pitch + ox2;
+#pragma unroll
+                for (int k = 0; k < kBriefPY; ++k) {
+                    const float a = tile[o1 + 4 * k * pitch], c = tile[o2 + 4 * k * pitch];
+                    words[w][k] = (words[w][k] << 1) | (a > c ? 1u : 0u);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < kBriefPY; ++k) {
+                    const int y = ty0 + ly + 4 * k;
+                    const int xa = clampi(x + ox1, 0, sx - 1), ya = clampi(y + oy1, 0, sy - 1);
+                    const int xb = clampi(x + ox2, 0, sx - 1), yb = clampi(y + oy2, 0, sy - 1);
+                    float a, c;
+                    if constexpr (FUSED) {
+                        a = (float)bilinear_q16(g8, W, H, up_x[xa], up_y[ya]);
+                        c = (float)bilinear_q16(g8, W, H, up_x[xb], up_y[yb]);
+                    } else {
+                        a = L[(size_t)ya * sx + xa];
+                        c = L[(size_t)yb * sx + xb];
+                    }
+                    words[w][k] = (words[w][k] << 1) | (a > c ? 1u : 0u);
+                }
+            }
+        }
+    }
+
+    if (x < sx) {
+#pragma unroll
+        for (int k = 0; k < kBriefPY; ++k) {
+            const int y = ty0 + ly + 4 * k;
+            if (y < sy) {
+                uint32_t out[NSP];
+#pragma unroll
+                for (int w = 0; w < NSP; ++w) out[w] = words[w][k];
+                store_desc<NSP>(F, (uint32_t)(y * sx + x), out);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// field initialisation
+
+template <int NSP>
+__global__ void __launch_bounds__(kThreads) k_init(const uint32_t *__restrict__ F1, const uint32_t *__restrict__ F2,
+                                                   int first, const uint32_t *__restrict__ prev,
+                                                   uint32_t *__restrict__ off, uint16_t *__restrict__ cost,
+                                                   int sx, int sy)
+{
+    const int gx = (sx + kThreads - 1) / kThreads;
+    const int b = xcd_remap(blockIdx.x, gx * sy);
+    const int x = (b % gx) * kThreads + threadIdx.x, y = b / gx;
+    if (x >= sx) return;
+    const int idx = y * sx + x;
+    int nx = 0, ny = 0;
+    if (!first) {
+        const uint32_t o0 = prev[idx];
+        const int dx0 = clampi(off_dx(o0), -x, sx - 1 - x), dy0 = clampi(off_dy(o0), -y, sy - 1 - y);
+        const uint32_t o1 = prev[idx + dy0 * sx + dx0];
+        nx = clampi(off_dx(o1), -x, sx - 1 - x);
+        ny = clampi(off_dy(o1), -y, sy - 1 - y);
+    }
+    uint32_t f1[NSP], f2[NSP];
+    load_desc<NSP>(F1, idx, f1);
+    load_desc<NSP>(F2, idx + ny * sx + nx, f2);
+    off[idx] = pack_off(nx, ny);
+    cost[idx] = (uint16_t)hamming<NSP>(f1, f2);
+}
+
+// ---------------------------------------------------------------------------
+// one jump-flooding level (+ optionally the random search of the iteration)
+
+struct RSArgs {
+    int enabled;
+    uint32_t iteration, pair, key0, key1;
+    int64_t sigma_q16;
+};
+
+template <int NSP>
+__device__ __forceinline__ void random_candidate(const uint32_t (&f1)[NSP], const uint32_t *__restrict__ F2,
+                                                 int idx, int x, int y, int sx, int sy, const RSArgs &rs,
+                                                 uint32_t &best, int &bcost)
+{
+    uint32_t c0 = (uint32_t)idx, c1 = rs.iteration, c2 = rs.pair, c3 = 0u;
+    philox4x32_10(c0, c1, c2, c3, rs.key0, rs.key1);
+    int nx = round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c0 >> 20]);
+    int ny = round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c1 >> 20]);
+    nx = clampi(nx, -x, sx - 1 - x);
+    ny = clampi(ny, -y, sy - 1 - y);
+    uint32_t f2[NSP];
+    load_desc<NSP>(F2, idx + ny * sx + nx, f2);
+    const int c = hamming<NSP>(f1, f2);
+    const uint32_t cand = pack_off(nx, ny);
+    // insert() skips a candidate equal to the current offset (briefmatch_cuda.cu:131-132)
+    if (cand != best && c < bcost) {
+        bcost = c;
+        best = cand;
+    }
+}
+
+template <int NSP, bool RS>
+__global__ void __launch_bounds__(kThreads) k_level(const uint32_t *__restrict__ F1, const uint32_t *__restrict__ F2,
+                                                    const uint32_t *__restrict__ off_in,
+                                                    const uint16_t *__restrict__ cost_in,
+                                                    uint32_t *__restrict__ off_out,
+                                                    uint16_t *__restrict__ cost_out, int sx, int sy, int p,
+                                                    int pd, int double_prop, RSArgs rs)
+{
+    const int gx = (sx + kThreads - 1) / kThreads;
+    const int b = xcd_remap(blockIdx.x, gx * sy);
+    const int x = (b % gx) * kThreads + threadIdx.x, y = b / gx;
+    if (x >= sx) return;
+    const int M = sx * sy;
+    const int idx = y * sx + x;
+
+    uint32_t f1[NSP];
+    load_desc<NSP>(F1, idx, f1);
+    uint32_t best = off_in[idx];
+    int bcost = cost_in[idx];
+
+    // candidate order of the reference: d = -1 (vertical, horizontal), d = +1 (...)
+    const int sv = pd * p * sx, sh = pd * p;
+    const int nidx[4] = {idx - sv, idx - sh, idx + sv, idx + sh};
+    bool valid[4];
+    uint32_t o[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        valid[c] = (c >= 2 || double_prop) && (unsigned)nidx[c] < (unsigned)M;
+        o[c] = valid[c] ? off_in[nidx[c]] : 0u;
+    }
+    int cst[4];
+    uint32_t cand[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int nx = clampi(off_dx(o[c]), -x, sx - 1 - x);
+        const int ny = clampi(off_dy(o[c]), -y, sy - 1 - y);
+        cand[c] = pack_off(nx, ny);
+        cst[c] = 0x7fffffff;
+        if (valid[c]) {
+            uint32_t f2[NSP];
+            load_desc<NSP>(F2, idx + ny * sx + nx, f2);
+            cst[c] = hamming<NSP>(f1, f2);
+        }
+    }
+    // sequential insert() semantics: skip a candidate equal to the running best,
+    // accept a strictly lower cost
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        if (cand[c] != best && cst[c] < bcost) {
+            bcost = cst[c];
+            best = cand[c];
+        }
+    if constexpr (RS) random_candidate<NSP>(f1, F2, idx, x, y, sx, sy, rs, best, bcost);
+    off_out[idx] = best;
+    cost_out[idx] = (uint16_t)bcost;
+}
+
+template <int NSP>
+__global__ void __launch_bounds__(kThreads) k_random_search(const uint32_t *__restrict__ F1,
+                                                            const uint32_t *__restrict__ F2,
+                                                            uint32_t *__restrict__ off,
+                                                            uint16_t *__restrict__ cost, int sx, int sy,
+                                                            RSArgs rs)
+{
+    const int gx = (sx + kThreads - 1) / kThreads;
+    const int b = xcd_remap(blockIdx.x, gx * sy);
+    const int x = (b % gx) * kThreads + threadIdx.x, y = b / gx;
+    if (x >= sx) return;
+    const int idx = y * sx + x;
+    uint32_t f1[NSP];
+    load_desc<NSP>(F1, idx, f1);
+    uint32_t best = off[idx];
+    int bcost = cost[idx];
+    random_candidate<NSP>(f1, F2, idx, x, y, sx, sy, rs, best, bcost);
+    off[idx] = best;
+    cost[idx] = (uint16_t)bcost;
+}
+
+// ---------------------------------------------------------------------------
+// down-sampling: Gaussian (horizontal then vertical, as filterConst twice)
+// evaluated only at the nearest-neighbour sample point, then scaled.
+
+__global__ void __launch_bounds__(kThreads) k_downsample(const uint32_t *__restrict__ off,
+                                                         const uint32_t *__restrict__ nn_x,
+                                                         const uint32_t *__restrict__ nn_y,
+                                                         const float *__restrict__ gauss, int P, int sx, int sy,
+                                                         int W, int H, float scx, float scy, int identity,
+                                                         float *__restrict__ U, float *__restrict__ V)
+{
+    const int gx = (W + kThreads - 1) / kThreads;
+    const int b = xcd_remap(blockIdx.x, gx * H);
+    const int x = (b % gx) * kThreads + threadIdx.x, y = b / gx;
+    if (x >= W) return;
+    const size_t o = (size_t)y * W + x;
+    if (identity) {
+        const uint32_t v = off[(size_t)y * sx + x];
+        U[o] = (float)off_dx(v);
+        V[o] = (float)off_dy(v);
+        return;
+    }
+    const int x1 = (int)nn_x[x], y1 = (int)nn_y[y], P2 = P / 2;
+    float gu = 0.0f, gv = 0.0f;
+    for (int j = 0; j < P; ++j) {
+        const uint32_t *row = off + (size_t)clampi(y1 + j - P2, 0, sy - 1) * sx;
+        float hu = 0.0f, hv = 0.0f;
+        for (int i = 0; i < P; ++i) {
+            const uint32_t v = row[clampi(x1 + i - P2, 0, sx - 1)];
+            hu += gauss[i] * (float)off_dx(v);
+            hv += gauss[i] * (float)off_dy(v);
+        }
+        gu += gauss[j] * hu;
+        gv += gauss[j] * hv;
+    }
+    U[o] = gu * scx;
+    V[o] = gv * scy;
+}
+
+// ---------------------------------------------------------------------------
+// 1-D median: the fsize window (clamped) in registers, partially sorted with
+// min/max exchanges exactly as far as the middle element.
+
+template <int FS>
+__global__ void __launch_bounds__(kThreads) k_median(const float *__restrict__ in0, const float *__restrict__ in1,
+                                                     float *__restrict__ out0, float *__restrict__ out1, int W,
+                                                     int H, int dir)
+{
+    constexpr int KS = FS / 2;
+    const int gx = (W + kThreads - 1) / kThreads;
+    const int b = xcd_remap(blockIdx.x, gx * H);
+    const int x = (b % gx) * kThreads + threadIdx.x, y = b / gx;
+    if (x >= W) return;
+    const float *in = blockIdx.y ? in1 : in0;
+    float *out = blockIdx.y ? out1 : out0;
+    float v[FS];
+    if (dir) {
+        const float *row = in + (size_t)y * W;
+#pragma unroll
+        for (int i = 0; i < FS; ++i) v[i] = row[clampi(x - KS + i, 0, W - 1)];
+    } else {
+#pragma unroll
+        for (int i = 0; i < FS; ++i) v[i] = in[(size_t)clampi(y - KS + i, 0, H - 1) * W + x];
+    }
+#pragma unroll
+    for (int i = 0; i <= KS; ++i)
+#pragma unroll
+        for (int j = i + 1; j < FS; ++j) {
+            const float lo = fminf(v[i], v[j]), hi = fmaxf(v[i], v[j]);
+            v[i] = lo;
+            v[j] = hi;
+        }
+    out[(size_t)y * W + x] = v[KS];
+}
+
+// ---------------------------------------------------------------------------
+// Cross-trilateral filter.  Work-group: 64 x 16 output tile, thread (lx, ly)
+// owns the four consecutive rows 4 ly + k.  Source rows stream through LDS in
+// bands of BR rows as float4 (a u, a v, b I, 0), where a^2 = log2(e) / (2
+// sigmaEPE) and b^2 = log2(e) / (2 sigmaI 255^2), so one tap is
+//   t = (du')^2 + (dv')^2 + (dI')^2 + log2(e) i^2 / (2 sigmaS),  w = exp2(-t),
+// the product of the reference's three exponentials.  The j (row) part of the
+// spatial term is applied once per source row.
+
+constexpr int kTriTW = 64, kTriTH = 16, kTriPY = 4;
+constexpr int kTriLdsBytes = 64 * 1024;
+
+__global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict__ Ui, const float *__restrict__ Vi,
+                                                         const float *__restrict__ Um, const float *__restrict__ Vm,
+                                                         const float *__restrict__ I, float *__restrict__ Uo,
+                                                         float *__restrict__ Vo, int W, int H, int s, int BR,
+                                                         float a, float bI, const float *__restrict__ spt)
+{
+    extern __shared__ float4 band[];
+    const int pitch = kTriTW + 2 * s;
+    const int gxb = (W + kTriTW - 1) / kTriTW, gyb = (H + kTriTH - 1) / kTriTH;
+    const int blk = xcd_remap(blockIdx.x, gxb * gyb);
+    const int tx0 = (blk % gxb) * kTriTW, ty0 = (blk / gxb) * kTriTH;
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    const int x = tx0 + lx, y0 = ty0 + 4 * ly;
+    const int xc = min(x, W - 1);
+
+    float um[kTriPY], vm[kTriPY], g0[kTriPY], au[kTriPY], av[kTriPY], aw[kTriPY];
+#pragma unroll
+    for (int k = 0; k < kTriPY; ++k) {
+        const size_t o = (size_t)min(y0 + k, H - 1) * W + xc;
+        um[k] = Um[o] * a;
+        vm[k] = Vm[o] * a;
+        g0[k] = I[o] * bI;
+        au[k] = av[k] = aw[k] = 0.0f;
+    }
+
+    const int src0 = ty0 - s, src1 = ty0 + kTriTH + s;  // source rows of the tile
+    const int my0 = y0 - s, my1 = y0 + kTriPY - 1 + s;  // source rows this wave needs
+    for (int band0 = src0; band0 < src1; band0 += BR) {
+        __syncthreads();
+        for (int r = ly; r < BR; r += 4) {
+            const int gy = clampi(band0 + r, 0, H - 1);
+            const float *ru = Ui + (size_t)gy * W, *rv = Vi + (size_t)gy * W, *ri = I + (size_t)gy * W;
+            for (int c = lx; c < pitch; c += 64) {
+                const int gx = clampi(tx0 - s + c, 0, W - 1);
+                band[r * pitch + c] = make_float4(ru[gx] * a, rv[gx] * a, ri[gx] * bI, 0.0f);
+            }
+        }
+        __syncthreads();
+        const int rend = min(BR, src1 - band0);
+        for (int r = 0; r < rend; ++r) {
+            const int ys = band0 + r;
+            if (ys < my0 || ys >= my1) continue;  // wave-uniform
+            float ru[kTriPY], rv[kTriPY], rw[kTriPY];
+#pragma unroll
+            for (int k = 0; k < kTriPY; ++k) ru[k] = rv[k] = rw[k] = 0.0f;
+            const float4 *src = band + r * pitch + lx;
+            for (int i = 0; i < 2 * s; ++i) {
+                const float4 q = src[i];
+                const float sp = spt[i];
+#pragma unroll
+                for (int k = 0; k < kTriPY; ++k) {
+                    const float du = q.x - um[k], dv = q.y - vm[k], dg = q.z - g0[k];
+                    float t = fmaf(du, du, sp);
+                    t = fmaf(dv, dv, t);
+                    t = fmaf(dg, dg, t);
+                    const float w = __builtin_amdgcn_exp2f(-t);
+                    rw[k] += w;
+                    ru[k] = fmaf(w, q.x, ru[k]);
+                    rv[k] = fmaf(w, q.y, rv[k]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kTriPY; ++k) {
+                const int j = ys - (y0 + k);
+                const float ej = __builtin_amdgcn_exp2f(-spt[clampi(j + s, 0, 2 * s - 1)]);
+                const float e = (j >= -s && j < s) ? ej : 0.0f;
+                aw[k] = fmaf(e, rw[k], aw[k]);
+                au[k] = fmaf(e, ru[k], au[k]);
+                av[k] = fmaf(e, rv[k], av[k]);
+            }
+        }
+    }
+    if (x < W) {
+        const float ia = 1.0f / a;
+#pragma unroll
+        for (int k = 0; k < kTriPY; ++k) {
+            const int y = y0 + k;
+            if (y < H) {
+                const float den = fmaxf(1e-5f, aw[k]);
+                Uo[(size_t)y * W + x] = (au[k] * ia) / den;
+                Vo[(size_t)y * W + x] = (av[k] * ia) / den;
+            }
+        }
+    }
+}
+
+// plain 16-byte-per-lane copy, grid-stride
+__global__ void __launch_bounds__(kThreads) k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
+        dst[i] = src[i];
+}
+
+inline unsigned blocks_rows(int width, int rows) { return (unsigned)(((width + kThreads - 1) / kThreads) * rows); }
+
+}  // namespace
+
+int brief_max_radius()
+{
+    int R = 0;
+    while ((kBriefTW + 2 * (R + 1)) * (kBriefTH + 2 * (R + 1)) * 4 <= kBriefLdsBytes) ++R;
+    return R;
+}
+
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
+{
+    const size_t n = bytes / 16;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 256 * 32);
+    hipLaunchKernelGGL(k_copy, dim3(std::max(blocks, 1u)), dim3(kThreads), 0, st, (const uint4 *)src, (uint4 *)dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gray(const Geometry &g, const uint8_t *rgb, uint8_t *g8, float *g32, hipStream_t st)
+{
+    const int n = g.W * g.H;
+    hipLaunchKernelGGL(k_gray, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, st, rgb, g8, g32, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_upsample(const Geometry &g, const uint8_t *g8, float *L, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_upsample, dim3(blocks_rows(g.sx, g.sy)), dim3(kThreads), 0, st, g8, g.up_x, g.up_y, g.W,
+                       g.H, g.sx, g.sy, L);
+    return hipGetLastError();
+}
+
+template <int NSP, bool FUSED>
+static hipError_t launch_brief_t(const Geometry &g, const float *L, const uint8_t *g8, uint32_t *F, hipStream_t st)
+{
+    const int R = g.brief_radius;
+    const size_t lds = (size_t)(kBriefTW + 2 * R) * (kBriefTH + 2 * R) * sizeof(float);
+    const int gxb = (g.sx + kBriefTW - 1) / kBriefTW, gyb = (g.sy + kBriefTH - 1) / kBriefTH;
+    hipLaunchKernelGGL((k_brief<NSP, FUSED>), dim3(gxb * gyb), dim3(kThreads), lds, st, L, g8, g.up_x, g.up_y,
+                       g.pattern, g.N, g.W, g.H, g.sx, g.sy, R, F);
+    return hipGetLastError();
+}
+
+hipError_t launch_brief(const Geometry &g, const float *L, const uint8_t *g8, uint32_t *F, hipStream_t st)
+{
+    const bool fused = (L == nullptr);
+    switch (g.NSP) {
+    case 1: return fused ? launch_brief_t<1, true>(g, L, g8, F, st) : launch_brief_t<1, false>(g, L, g8, F, st);
+    case 2: return fused ? launch_brief_t<2, true>(g, L, g8, F, st) : launch_brief_t<2, false>(g, L, g8, F, st);
+    case 4: return fused ? launch_brief_t<4, true>(g, L, g8, F, st) : launch_brief_t<4, false>(g, L, g8, F, st);
+    case 8: return fused ? launch_brief_t<8, true>(g, L, g8, F, st) : launch_brief_t<8, false>(g, L, g8, F, st);
+    case 16: return fused ? launch_brief_t<16, true>(g, L, g8, F, st) : launch_brief_t<16, false>(g, L, g8, F, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+#define BM_NSP_DISPATCH(NSPV, EXPR)                                                                             \
+    switch (NSPV) {                                                                                             \
+    case 1: { constexpr int NSP = 1; EXPR; } break;                                                             \
+    case 2: { constexpr int NSP = 2; EXPR; } break;                                                             \
+    case 4: { constexpr int NSP = 4; EXPR; } break;                                                             \
+    case 8: { constexpr int NSP = 8; EXPR; } break;                                                             \
+    case 16: { constexpr int NSP = 16; EXPR; } break;                                                           \
+    default: return hipErrorInvalidValue;                                                                       \
+    }
+
+hipError_t launch_init(const Geometry &g, const uint32_t *F1, const uint32_t *F2, int first, const uint32_t *prev,
+                       uint32_t *off, uint16_t *cost, hipStream_t st)
+{
+    const dim3 grid(blocks_rows(g.sx, g.sy));
+    BM_NSP_DISPATCH(g.NSP, hipLaunchKernelGGL((k_init<NSP>), grid, dim3(kThreads), 0, st, F1, F2, first, prev, off,
+                                              cost, g.sx, g.sy));
+    return hipGetLastError();
+}
+
+static RSArgs make_rs(const Geometry &g, int enabled, uint32_t iteration, uint32_t pair)
+{
+    RSArgs rs;
+    rs.enabled = enabled;
+    rs.iteration = iteration;
+    rs.pair = pair;
+    rs.key0 = (uint32_t)g.seed;
+    rs.key1 = (uint32_t)(g.seed >> 32);
+    rs.sigma_q16 = g.rs_sigma_q16;
+    return rs;
+}
+
+hipError_t launch_level(const Geometry &g, const uint32_t *F1, const uint32_t *F2, const uint32_t *off_in,
+                        const uint16_t *cost_in, uint32_t *off_out, uint16_t *cost_out, int p, int pd,
+                        int rs_iteration, uint32_t pair, hipStream_t st)
+{
+    const dim3 grid(blocks_rows(g.sx, g.sy));
+    const RSArgs rs = make_rs(g, rs_iteration >= 0, (uint32_t)(rs_iteration < 0 ? 0 : rs_iteration), pair);
+    if (rs.enabled) {
+        BM_NSP_DISPATCH(g.NSP, hipLaunchKernelGGL((k_level<NSP, true>), grid, dim3(kThreads), 0, st, F1, F2, off_in,
+                                                  cost_in, off_out, cost_out, g.sx, g.sy, p, pd, g.doubleProp, rs));
+    } else {
+        BM_NSP_DISPATCH(g.NSP, hipLaunchKernelGGL((k_level<NSP, false>), grid, dim3(kThreads), 0, st, F1, F2,
+                                                  off_in, cost_in, off_out, cost_out, g.sx, g.sy, p, pd,
+                                                  g.doubleProp, rs));
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_random_search(const Geometry &g, const uint32_t *F1, const uint32_t *F2, uint32_t *off,
+                                uint16_t *cost, uint32_t iteration, uint32_t pair, hipStream_t st)
+{
+    const dim3 grid(blocks_rows(g.sx, g.sy));
+    const RSArgs rs = make_rs(g, 1, iteration, pair);
+    BM_NSP_DISPATCH(g.NSP, hipLaunchKernelGGL((k_random_search<NSP>), grid, dim3(kThreads), 0, st, F1, F2, off,
+                                              cost, g.sx, g.sy, rs));
+    return hipGetLastError();
+}
+
+hipError_t launch_downsample(const Geometry &g, const uint32_t *off, float *U, float *V, hipStream_t st)
+{
+    const int identity = (g.sx == g.W && g.sy == g.H) ? 1 : 0;
+    const float scx = (float)g.W / (float)g.sx, scy = (float)g.H / (float)g.sy;
+    hipLaunchKernelGGL(k_downsample, dim3(blocks_rows(g.W, g.H)), dim3(kThreads), 0, st, off, g.nn_x, g.nn_y,
+                       g.gauss, g.P, g.sx, g.sy, g.W, g.H, scx, scy, identity, U, V);
+    return hipGetLastError();
+}
+
+template <int FS>
+static hipError_t launch_median_t(const Geometry &g, const float *in0, const float *in1, float *out0, float *out1,
+                                  int dir, hipStream_t st)
+{
+    const dim3 grid(blocks_rows(g.W, g.H), in1 ? 2 : 1);
+    hipLaunchKernelGGL((k_median<FS>), grid, dim3(kThreads), 0, st, in0, in1, out0, out1, g.W, g.H, dir);
+    return hipGetLastError();
+}
+
+hipError_t launch_median(const Geometry &g, const float *in0, const float *in1, float *out0, float *out1, int dir,
+                         hipStream_t st)
+{
+    switch (g.fsize) {
+#define BM_MED(FSV) case FSV: return launch_median_t<FSV>(g, in0, in1, out0, out1, dir, st);
+    BM_MED(1) BM_MED(3) BM_MED(5) BM_MED(7) BM_MED(9) BM_MED(11) BM_MED(13) BM_MED(15) BM_MED(17) BM_MED(19)
+    BM_MED(21) BM_MED(23) BM_MED(25) BM_MED(27) BM_MED(29) BM_MED(31) BM_MED(33) BM_MED(35) BM_MED(37)
+    BM_MED(39) BM_MED(41) BM_MED(43) BM_MED(45) BM_MED(47)
+#undef BM_MED
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_trilateral(const Geometry &g, const float *Ui, const float *Vi, const float *Um, const float *Vm,
+                             const float *I, float *Uo, float *Vo, hipStream_t st)
+{
+    const int s = g.tri_s;
+    const int pitch = kTriTW + 2 * s;
+    int BR = kTriLdsBytes / (pitch * (int)sizeof(float4));
+    if (BR < 1) return hipErrorInvalidValue;
+    if (BR > 16) BR = 16;
+    const size_t lds = (size_t)BR * pitch * sizeof(float4);
+    const int gxb = (g.W + kTriTW - 1) / kTriTW, gyb = (g.H + kTriTH - 1) / kTriTH;
+    hipLaunchKernelGGL(k_trilateral, dim3(gxb * gyb), dim3(kThreads), lds, st, Ui, Vi, Um, Vm, I, Uo, Vo, g.W, g.H,
+                       s, BR, g.tri_a, g.tri_b, g.tri_spatial);
+    return hipGetLastError();
+}
+
+}  // namespace bm

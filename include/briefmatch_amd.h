@@ -1,0 +1,222 @@
+/*
+ * briefmatch_amd.h -- C ABI of the MI355X (gfx950) BriefMatch flow engine.
+ *
+ * The drop-in boundary for the reference's extern "C" launch layer
+ * (kernels/briefmatch_cuda.h:64-93, kernels/util_cuda.h:114-129).  Differences
+ * from that layer, all deliberate:
+ *   - every entry point returns a status (BM_OK = 0) instead of throwing a C++
+ *     exception across the C boundary (util_cuda.h:102-112); bm_last_error()
+ *     gives the message;
+ *   - sizes and configuration live in an opaque bm_ctx (one per BMParams, like
+ *     the reference's BriefMatch object) instead of module-global __constant__
+ *     tables, so several configurations can coexist in one process;
+ *   - sizes are size_t; device work is asynchronous on the context's HIP stream
+ *     and bm_sync() waits for it (the reference synchronises after every launch);
+ *   - the match field is (int16 dx, int16 dy) packed in one uint32 per pixel
+ *     plus a uint16 Hamming cost, instead of float D / sint NX / sint NY.
+ * All pointers named *_dev are device pointers; *_host are host pointers.
+ */
+#ifndef BRIEFMATCH_AMD_H
+#define BRIEFMATCH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define BM_OK 0
+#define BM_E_PARAM 1   /* parameter outside the supported range */
+#define BM_E_ALLOC 2   /* device or host allocation failed */
+#define BM_E_LAUNCH 3  /* HIP runtime / kernel launch error */
+#define BM_E_IO 4      /* file read / write error */
+#define BM_E_STATE 5   /* call out of order (e.g. no previous frame) */
+
+/* Copy directions for bm_memcpy (util_cuda.h:74-77). */
+#define BM_HOST_TO_DEVICE 0
+#define BM_DEVICE_TO_HOST 1
+#define BM_DEVICE_TO_DEVICE 2
+#define BM_HOST_TO_HOST 3
+
+/* Limits of the reference: PATCH_MAX/4 feature bits (briefmatch_cuda.h:62),
+ * MAX_MEDIAN (util_cuda.h:56), CONST_KERNEL_SIZE (util_cuda.h:58). */
+#define BM_MAX_FEATURE_BITS 512
+#define BM_MAX_MEDIAN 47
+#define BM_MAX_GAUSS_TAPS 100
+
+/* Matching parameters: the fields and meaning of BMParams (briefmatch.h:65-89). */
+typedef struct bm_params {
+    unsigned sx, sy;          /* matched (up-sampled) resolution */
+    unsigned sx_in, sy_in;    /* input resolution */
+    unsigned N;               /* BRIEF length in bits, 2..512 */
+    unsigned iterations;      /* propagate + random-search rounds */
+    unsigned jfMax;           /* largest jump-flooding step */
+    unsigned P;               /* Gaussian taps (derived from sigma) */
+    unsigned fsize;           /* median window, odd, <= 47 */
+    float patchArea;          /* BRIEF patch radius in matched pixels */
+    float sigma;              /* Gaussian down-sampling sigma (matched pixels) */
+    float sEPE, sI, sS;       /* trilateral sigmas: flow, image (I/255), spatial */
+    float maxmot;             /* colour-coding max motion */
+    int doubleProp;           /* 4 propagation candidates (else 2) */
+    int filter;               /* median + trilateral refinement */
+    int tInit;                /* temporal initialisation in sequences */
+} bm_params;
+
+typedef struct bm_ctx bm_ctx;
+
+/* ---- parameters ------------------------------------------------------- */
+
+/* BMParams::BMParams() defaults (briefmatch.h:67-84). */
+void bm_params_default(bm_params *out);
+
+/* The CLI derivation of OFPipeline::setParams (of_pipeline.cpp:178-190) and the
+ * Gaussian tap count of BriefMatch::setup (briefmatch.cpp:153-154): up-sampling
+ * factors ux/uy (0 selects the reference defaults 2.740/3.052), patch radius,
+ * down-sampling sigma, median size and spatial sigma in percent of the diagonal. */
+int bm_params_derive(unsigned W, unsigned H, float ux, float uy, float patch_radius,
+                     float sigma_ds, float median_size, float range_spatial, bm_params *out);
+
+/* ---- context (BriefMatch::setup / clean, briefmatch.cpp:72-238) -------- */
+
+/* Validates the limits of briefmatch.cpp:78-96, builds the BRIEF pattern from
+ * `seed` (replacing the time-seeded Boost pattern, briefmatch.cpp:124-148), the
+ * Gaussian kernel (:153-172) and the device buffers, on a new HIP stream. */
+int bm_ctx_create(const bm_params *params, uint64_t seed, bm_ctx **out);
+void bm_ctx_destroy(bm_ctx *ctx);
+int bm_ctx_get_params(const bm_ctx *ctx, bm_params *out);
+/* The context's hipStream_t, as void*. */
+void *bm_ctx_stream(bm_ctx *ctx);
+/* Last error message of this context (ctx == NULL: of the calling thread). */
+const char *bm_last_error(const bm_ctx *ctx);
+int bm_sync(bm_ctx *ctx);
+/* Host Philox4x32-10 (the counter-based stream of the BRIEF pattern and of the
+ * random search; replaces cuRAND, briefmatch_cuda.cu:57,237-245). */
+int bm_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* Descriptor stride in uint32 words per pixel: ceil(N/32) rounded up to a power
+ * of two (words beyond ceil(N/32) are zero). */
+unsigned bm_descriptor_stride(const bm_ctx *ctx);
+/* The BRIEF pattern: 4N int16 (ox1, oy1, ox2, oy2) per test. */
+int bm_get_pattern(const bm_ctx *ctx, int16_t *pattern_host);
+/* Replace the pattern (CUDA_copyPatchToConst, briefmatch_cuda.cu:376-379). */
+int bm_set_pattern(bm_ctx *ctx, const int16_t *pattern_host);
+/* The normalised Gaussian kernel (CUDA_copyToConst, util_cuda.cu:277-280). */
+int bm_get_gauss_kernel(const bm_ctx *ctx, float *kernel_host, unsigned *P);
+/* Device pointers of the context's last match field (sx*sy entries). */
+int bm_ctx_field(bm_ctx *ctx, uint32_t **off_dev, uint16_t **cost_dev);
+
+/* ---- device memory (CUDA_constructor / destructor / copy, util_cuda.cu:206-239) */
+
+int bm_malloc(void **ptr_dev, size_t bytes);
+/* Waits for all queued device work before releasing the buffer. */
+int bm_free(void *ptr_dev);
+int bm_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int bm_memset(void *ptr_dev, int value, size_t bytes);
+
+/* ---- stages (asynchronous on the context stream) ------------------------ */
+
+/* Interleaved RGB uint8 (W*H*3) -> gray uint8 and/or FP32 (either may be NULL). */
+int bm_gray(bm_ctx *ctx, const uint8_t *rgb_dev, uint8_t *gray8_dev, float *gray32_dev);
+/* Up-sampling sx_in x sy_in gray8 -> sx x sy FP32, bilinear in fixed point
+ * (replaces CUDA_resizeLanczos, util_cuda.cu:250-257). */
+int bm_upsample(bm_ctx *ctx, const uint8_t *gray8_dev, float *L_dev);
+/* BRIEF descriptors of an up-sampled FP32 frame (CUDA_binaryFeatures,
+ * briefmatch_cuda.cu:311-319).  F has sx*sy*bm_descriptor_stride() words. */
+int bm_brief(bm_ctx *ctx, const float *L_dev, uint32_t *F_dev);
+/* Up-sampling fused into BRIEF: the same descriptors as bm_upsample followed by
+ * bm_brief, without writing the up-sampled frame. */
+int bm_brief_gray(bm_ctx *ctx, const uint8_t *gray8_dev, uint32_t *F_dev);
+/* Field initialisation (CUDA_initialize, briefmatch_cuda.cu:321-332): first != 0
+ * gives offset (0,0); otherwise temporal, following prev_off (a different buffer
+ * from off) once. */
+int bm_init(bm_ctx *ctx, const uint32_t *F1_dev, const uint32_t *F2_dev, int first,
+            const uint32_t *prev_off_dev, uint32_t *off_dev, uint16_t *cost_dev);
+/* One level of jump flooding from a snapshot (off_in/cost_in) into a different
+ * buffer (off_out/cost_out). */
+int bm_propagate_level(bm_ctx *ctx, const uint32_t *F1_dev, const uint32_t *F2_dev,
+                       const uint32_t *off_in_dev, const uint16_t *cost_in_dev,
+                       uint32_t *off_out_dev, uint16_t *cost_out_dev, int p, int pd);
+/* All levels jfMax..1 of one propagation (CUDA_propagate, briefmatch_cuda.cu:334-347),
+ * updating off/cost; pd = -1 for even iterations, +1 for odd (briefmatch.cpp:270,289). */
+int bm_propagate(bm_ctx *ctx, const uint32_t *F1_dev, const uint32_t *F2_dev,
+                 uint32_t *off_dev, uint16_t *cost_dev, int pd);
+/* Random search (CUDA_randomSearch, briefmatch_cuda.cu:349-360) with the stateless
+ * Philox stream of (pixel, iteration, pair, seed). */
+int bm_random_search(bm_ctx *ctx, const uint32_t *F1_dev, const uint32_t *F2_dev,
+                     uint32_t *off_dev, uint16_t *cost_dev, uint32_t iteration, uint32_t pair);
+/* BriefMatch::match after the descriptors (briefmatch.cpp:265-290): init, then
+ * `iterations` x {propagate, random search}.  Random search runs fused into the
+ * last propagation level; results equal the separate calls. */
+int bm_match(bm_ctx *ctx, const uint32_t *F1_dev, const uint32_t *F2_dev, int first,
+             const uint32_t *prev_off_dev, uint32_t pair, uint32_t *off_dev, uint16_t *cost_dev);
+/* BriefMatch::downsample (briefmatch.cpp:296-324): field -> FP32 flow at input
+ * resolution (Gaussian, nearest-neighbour decimation, scaling), fused. */
+int bm_downsample(bm_ctx *ctx, const uint32_t *off_dev, float *U_dev, float *V_dev);
+/* One 1-D median pass (CUDA_medianFilter1D, util_cuda.cu:197-204), dir 1 = x. */
+int bm_median1d(bm_ctx *ctx, const float *in_dev, float *out_dev, int dir);
+/* Cross-trilateral filter (CUDA_cbFilter, briefmatch_cuda.cu:362-374); sigmas are
+ * taken from the context (sEPE^2, sI^2, sS^2 as BriefMatch::process passes them). */
+int bm_trilateral(bm_ctx *ctx, const float *Ui_dev, const float *Vi_dev, const float *Um_dev,
+                  const float *Vm_dev, const float *I_dev, float *Uo_dev, float *Vo_dev);
+/* BriefMatch::process (briefmatch.cpp:327-352): medians x then y of U and V, then
+ * the trilateral filter guided by I (the first frame at input resolution, FP32). */
+int bm_refine(bm_ctx *ctx, const float *U_dev, const float *V_dev, const float *I_dev,
+              float *Uo_dev, float *Vo_dev);
+
+/* ---- whole path ------------------------------------------------------------ */
+
+/* One isolated pair: device-resident interleaved RGB uint8 frames (sx_in x sy_in)
+ * -> FP32 flow U, V (frame 1 -> frame 2, on the frame-1 grid).  `pair` selects the
+ * random-search stream.  The match field stays in the context (bm_ctx_field). */
+int bm_flow_pair(bm_ctx *ctx, const uint8_t *rgb1_dev, const uint8_t *rgb2_dev, uint32_t pair,
+                 float *U_dev, float *V_dev);
+/* Sequence mode (OFPipeline::processFrame, of_pipeline.cpp:241-387): describe one
+ * new frame, match it against the previous one with temporal initialisation,
+ * refine, and keep its descriptors for the next call.  *has_flow is 0 for the
+ * first frame of a sequence (no previous frame). */
+int bm_stream_frame(bm_ctx *ctx, const uint8_t *rgb_dev, float *U_dev, float *V_dev, int *has_flow);
+/* Forget the previous frame (the next bm_stream_frame starts a new sequence). */
+int bm_stream_reset(bm_ctx *ctx);
+
+/* ---- timing (CudaTimer, util_cuda.h:131-141) ----------------------------- */
+
+int bm_timer_start(bm_ctx *ctx);
+/* Milliseconds since bm_timer_start on the context stream (waits for it). */
+int bm_timer_stop(bm_ctx *ctx, float *ms);
+
+/* Per-stage device time.  While enabled, every kernel the context launches is
+ * bracketed by HIP events on its stream (the reference times whole phases with
+ * CudaTimer, of_pipeline.cpp:267-325).  bm_profile_read waits for the stream and
+ * returns, per stage id, the summed milliseconds and the launch count since the
+ * last read, then resets them. */
+#define BM_STAGE_GRAY 0
+#define BM_STAGE_UPSAMPLE 1
+#define BM_STAGE_BRIEF 2
+#define BM_STAGE_INIT 3
+#define BM_STAGE_LEVEL 4          /* jump-flooding level */
+#define BM_STAGE_LEVEL_RS 5       /* last level with the fused random search */
+#define BM_STAGE_RANDOM_SEARCH 6
+#define BM_STAGE_DOWNSAMPLE 7
+#define BM_STAGE_MEDIAN 8
+#define BM_STAGE_TRILATERAL 9
+#define BM_STAGE_COPY 10
+#define BM_STAGE_COUNT 11
+int bm_profile_enable(bm_ctx *ctx, int on);
+int bm_profile_read(bm_ctx *ctx, double *ms, unsigned *launches);
+
+/* Plain device-to-device copy kernel on the context stream (bytes % 16 == 0);
+ * the reference point for HBM traffic counters. */
+int bm_device_copy(bm_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes);
+
+/* ---- metrics ------------------------------------------------------------- */
+
+/* End-point error against ground truth (BriefMatch::error, briefmatch.cpp:468-514),
+ * host arrays; occluded may be NULL.  epe_noc skips occluded pixels. */
+int bm_epe(const float *U_host, const float *V_host, const float *gtU_host, const float *gtV_host,
+           const uint8_t *occluded_host, size_t n, double *epe_all, double *epe_noc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

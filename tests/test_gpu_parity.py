@@ -1,0 +1,311 @@
+"""GPU parity: every stage of the HIP path against the CPU oracle, through the C ABI.
+
+Integer stages (gray, up-sampling, BRIEF, init, propagation, random search, the
+whole match field) must agree bit for bit; FP32 stages (down-sampling, median,
+trilateral, final flow) within 1e-3 px max-abs.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import briefmatch_amd as bm
+from briefmatch_amd import synth
+from conftest import say
+
+pytestmark = pytest.mark.gpu
+
+FLOW_TOL = 1e-3  # px, max-abs of the FP32 flow against the oracle
+
+
+def _pair(W, H, seed, **gen):
+    gen.setdefault("motion", "rects")
+    gen.setdefault("max_motion", 6.0)
+    return synth.make_pair(W, H, seed=seed, **gen)
+
+
+def _oracle_params(p):
+    op = oracle.Params()
+    for n, _ in op._fields_:
+        setattr(op, n, getattr(p, n))
+    return op
+
+
+def _descs(ctx, frame, fused=True):
+    """Device descriptors of one RGB frame (returns the buffer, gray8 and L)."""
+    rgb = bm.DeviceBuffer(array=frame)
+    g8 = bm.DeviceBuffer(ctx.W * ctx.H)
+    ctx.gray(rgb, g8=g8)
+    F = ctx.desc_buffer()
+    L = None
+    if fused:
+        ctx.brief_gray(g8, F)
+    else:
+        L = bm.DeviceBuffer(ctx.sx * ctx.sy * 4)
+        ctx.upsample(g8, L)
+        ctx.brief(L, F)
+    ctx.sync()
+    return F, g8, L
+
+
+def _oracle_descs(frame, p, pattern):
+    g = oracle.gray(frame)
+    L = oracle.upsample(g, p.sx, p.sy)
+    return oracle.brief(L, pattern, p.N), g, L
+
+
+def _pad(F, NSP):
+    sy, sx, NS = F.shape
+    out = np.zeros((sy, sx, NSP), np.uint32)
+    out[..., :NS] = F
+    return out
+
+
+@pytest.mark.parametrize("W,H,up,N", [(320, 240, 1.5, 32), (48, 32, 2.0, 512), (61, 37, 2.3, 72),
+                                      (96, 64, 1.0, 128), (80, 50, 3.1, 100)])
+def test_gray_upsample_brief(W, H, up, N):
+    say("gray/upsample/brief", W, H, up, N)
+    pr = _pair(W, H, seed=W + N)
+    p = bm.derive_params(W, H, up, up, N=N)
+    ctx = bm.Context(p, seed=11)
+    pat = ctx.pattern()
+    assert np.array_equal(pat, oracle.make_pattern(11, N, p.patchArea))
+    rgb = bm.DeviceBuffer(array=pr.frame1)
+    g8, g32 = bm.DeviceBuffer(W * H), bm.DeviceBuffer(W * H * 4)
+    ctx.gray(rgb, g8=g8, g32=g32)
+    L = bm.DeviceBuffer(p.sx * p.sy * 4)
+    ctx.upsample(g8, L)
+    F, Ff = ctx.desc_buffer(), ctx.desc_buffer()
+    ctx.brief(L, F)
+    ctx.brief_gray(g8, Ff)
+    ctx.sync()
+    og = oracle.gray(pr.frame1)
+    assert np.array_equal(g8.download((H, W), np.uint8), og)
+    assert np.array_equal(g32.download((H, W), np.float32), og.astype(np.float32))
+    oL = oracle.upsample(og, p.sx, p.sy)
+    assert np.array_equal(L.download((p.sy, p.sx), np.float32), oL)
+    oF = _pad(oracle.brief(oL, pat, N), ctx.NSP)
+    assert np.array_equal(ctx.download_desc(F), oF)
+    assert np.array_equal(ctx.download_desc(Ff), oF)
+
+
+@pytest.mark.parametrize("N,double_prop", [(32, True), (128, True), (128, False), (512, True)])
+def test_levels_random_search_init(N, double_prop):
+    """Level / random-search / init parity from a random (non-trivial) snapshot."""
+    say("levels", N, double_prop)
+    W, H, up = 72, 40, 2.0
+    pr = _pair(W, H, seed=3 + N)
+    p = bm.derive_params(W, H, up, up, N=N, doubleProp=int(double_prop))
+    ctx = bm.Context(p, seed=5)
+    F1, _, _ = _descs(ctx, pr.frame1)
+    F2, _, _ = _descs(ctx, pr.frame2)
+    oF1, _, _ = _oracle_descs(pr.frame1, p, ctx.pattern())
+    oF2, _, _ = _oracle_descs(pr.frame2, p, ctx.pattern())
+    rng = np.random.default_rng(N)
+    sx, sy = p.sx, p.sy
+    xx, yy = np.meshgrid(np.arange(sx), np.arange(sy))
+    dx = np.clip(rng.integers(-25, 26, (sy, sx)), -xx, sx - 1 - xx).astype(np.int16)
+    dy = np.clip(rng.integers(-25, 26, (sy, sx)), -yy, sy - 1 - yy).astype(np.int16)
+    off0 = (dx.view(np.uint16).astype(np.uint32) | (dy.view(np.uint16).astype(np.uint32) << 16))
+    # costs of that field from the oracle's temporal init would differ; use init(first) costs + random field
+    _, c_first = oracle.init(oF1, oF2, first=True)
+    cost0 = np.minimum(c_first, np.uint16(N)).astype(np.uint16)
+    oi, ci = bm.DeviceBuffer(array=off0), bm.DeviceBuffer(array=cost0)
+    oo, co = ctx.field_buffers()
+    for p_lvl in (256, 31, 8, 1):
+        for pd in (-1, 1):
+            ctx.propagate_level(F1, F2, oi, ci, oo, co, p_lvl, pd)
+            ctx.sync()
+            eo, ec = oracle.propagate_level(oF1, oF2, p_lvl, pd, double_prop, off0, cost0)
+            assert np.array_equal(oo.download((sy, sx), np.uint32), eo), (p_lvl, pd)
+            assert np.array_equal(co.download((sy, sx), np.uint16), ec), (p_lvl, pd)
+    # random search, in place
+    ctx.random_search(F1, F2, oi, ci, iteration=3, pair=7)
+    ctx.sync()
+    eo, ec = oracle.random_search(oF1, oF2, 3, 7, 5, off0, cost0)
+    assert np.array_equal(oi.download((sy, sx), np.uint32), eo)
+    assert np.array_equal(ci.download((sy, sx), np.uint16), ec)
+    # temporal init from the random field
+    prev = bm.DeviceBuffer(array=off0)
+    ctx.init(F1, F2, oo, co, first=False, prev_off=prev)
+    ctx.sync()
+    eo, ec = oracle.init(oF1, oF2, first=False, prev_off=off0)
+    assert np.array_equal(oo.download((sy, sx), np.uint32), eo)
+    assert np.array_equal(co.download((sy, sx), np.uint16), ec)
+
+
+@pytest.mark.parametrize("W,H,up,N,iters,jf", [(320, 240, 1.5, 32, 2, 256), (100, 70, 2.9, 128, 3, 64),
+                                               (64, 48, 4.9, 512, 2, 256), (90, 60, 2.0, 64, 1, 100)])
+def test_match_field(W, H, up, N, iters, jf):
+    say("match", W, H, up, N, iters, jf)
+    pr = _pair(W, H, seed=N + iters)
+    p = bm.derive_params(W, H, up, up, N=N, iterations=iters, jfMax=jf)
+    ctx = bm.Context(p, seed=9)
+    F1, _, _ = _descs(ctx, pr.frame1)
+    F2, _, _ = _descs(ctx, pr.frame2)
+    off, cost = ctx.field_buffers()
+    ctx.match(F1, F2, off, cost, first=True, pair=4)
+    ctx.sync()
+    oF1, _, _ = _oracle_descs(pr.frame1, p, ctx.pattern())
+    oF2, _, _ = _oracle_descs(pr.frame2, p, ctx.pattern())
+    eo, ec = oracle.match(oF1, oF2, iters, jf, True, True, None, 4, 9)
+    assert np.array_equal(off.download((p.sy, p.sx), np.uint32), eo)
+    assert np.array_equal(cost.download((p.sy, p.sx), np.uint16), ec)
+    # propagate + separate random search equals the fused match
+    off2, cost2 = ctx.field_buffers()
+    ctx.init(F1, F2, off2, cost2, first=True)
+    pd = -1
+    for i in range(iters):
+        ctx.propagate(F1, F2, off2, cost2, pd)
+        ctx.random_search(F1, F2, off2, cost2, iteration=i, pair=4)
+        pd = -pd
+    ctx.sync()
+    assert np.array_equal(off2.download((p.sy, p.sx), np.uint32), eo)
+    assert np.array_equal(cost2.download((p.sy, p.sx), np.uint16), ec)
+
+
+def test_refinement_stages():
+    """Down-sampling, 1-D medians and the trilateral filter on the oracle's field."""
+    say("refine stages")
+    W, H, up = 160, 120, 2.9
+    pr = _pair(W, H, seed=21, max_motion=10.0)
+    p = bm.derive_params(W, H, up, up, N=64)
+    ctx = bm.Context(p, seed=2)
+    oF1, og, _ = _oracle_descs(pr.frame1, p, ctx.pattern())
+    oF2, _, _ = _oracle_descs(pr.frame2, p, ctx.pattern())
+    eo, _ = oracle.match(oF1, oF2, 2, 256, True, True, None, 0, 2)
+    U, V = ctx.image_buffer(), ctx.image_buffer()
+    d_eo = bm.DeviceBuffer(array=eo)
+    ctx.downsample(d_eo, U, V)
+    ctx.sync()
+    oU, oV = oracle.downsample(eo, W, H, ctx.gauss_kernel())
+    gU, gV = U.download((H, W), np.float32), V.download((H, W), np.float32)
+    assert np.abs(gU - oU).max() <= 1e-4 and np.abs(gV - oV).max() <= 1e-4
+    # medians on identical inputs are exact order statistics
+    for d in (0, 1):
+        out, src = ctx.image_buffer(), bm.DeviceBuffer(array=oU)
+        ctx.median1d(src, out, d)
+        ctx.sync()
+        assert np.array_equal(out.download((H, W), np.float32), oracle.median1d(oU, d, p.fsize))
+    oUm = oracle.median1d(oracle.median1d(oU, 1, p.fsize), 0, p.fsize)
+    oVm = oracle.median1d(oracle.median1d(oV, 1, p.fsize), 0, p.fsize)
+    guide = og.astype(np.float32)
+    Uo, Vo = ctx.image_buffer(), ctx.image_buffer()
+    ins = [bm.DeviceBuffer(array=a) for a in (oU, oV, oUm, oVm, guide)]
+    ctx.trilateral(*ins, Uo, Vo)
+    ctx.sync()
+    tU, tV = oracle.trilateral(oU, oV, oUm, oVm, guide, p.sEPE ** 2, p.sI ** 2, p.sS ** 2)
+    assert np.abs(Uo.download((H, W), np.float32) - tU).max() <= FLOW_TOL
+    assert np.abs(Vo.download((H, W), np.float32) - tV).max() <= FLOW_TOL
+    # the composed refinement entry point
+    ins = [bm.DeviceBuffer(array=a) for a in (oU, oV, guide)]
+    ctx.refine(*ins, Uo, Vo)
+    ctx.sync()
+    assert np.abs(Uo.download((H, W), np.float32) - tU).max() <= FLOW_TOL
+
+
+@pytest.mark.parametrize("filt", [1, 0])
+def test_flow_pair_c1(filt):
+    """The 320x240 / u1.5 / 32-bit workload, whole path, against the oracle."""
+    say("flow_pair c1 filter", filt)
+    pr = synth.workload_pair("c1_320x240_u1.5_n32", seed=1)
+    p = bm.derive_params(320, 240, 1.5, 1.5, N=32, filter=filt)
+    ctx = bm.Context(p, seed=7)
+    U, V = ctx.flow_numpy(pr.frame1, pr.frame2, pair=3)
+    off, cost = ctx.field()
+    eo, ec, oU, oV = oracle.flow_pair(_oracle_params(p), pr.frame1, pr.frame2, seed=7, pair=3)
+    assert np.array_equal(off, eo)
+    assert np.array_equal(cost, ec)
+    assert np.abs(U - oU).max() <= FLOW_TOL and np.abs(V - oV).max() <= FLOW_TOL
+    e_all, _ = bm.epe(U, V, pr.gt, pr.occ)
+    z_all, _ = bm.epe(np.zeros_like(U), np.zeros_like(V), pr.gt, pr.occ)
+    # the refined flow is far better than zero flow; the raw decimated field is noisier
+    assert e_all < (0.5 if filt else 0.95) * z_all
+
+
+def test_sequence_temporal_init():
+    """Sequence mode: three frames, temporal init from the previous field."""
+    say("stream 3 frames")
+    W, H, up, N = 120, 90, 2.0, 64
+    base = _pair(W, H, seed=31, max_motion=5.0)
+    f3 = _pair(W, H, seed=32, max_motion=5.0).frame2
+    frames = [base.frame1, base.frame2, f3]
+    p = bm.derive_params(W, H, up, up, N=N)
+    ctx = bm.Context(p, seed=13)
+    U, V = ctx.image_buffer(), ctx.image_buffer()
+    outs = []
+    keep = []
+    for k, fr in enumerate(frames):
+        keep.append(bm.DeviceBuffer(array=fr))
+        has = ctx.stream_frame(keep[-1], U, V)
+        assert has == (k > 0)
+        if has:
+            ctx.sync()
+            outs.append((ctx.field(), U.download((H, W), np.float32), V.download((H, W), np.float32)))
+    pat = ctx.pattern()
+    oF = [_oracle_descs(f, p, pat) for f in frames]
+    kern = ctx.gauss_kernel()
+    prev = None
+    for k in range(2):
+        first = prev is None
+        eo, ec = oracle.match(oF[k][0], oF[k + 1][0], 2, 256, True, first, prev, k, 13)
+        (go, gc), gU, gV = outs[k]
+        assert np.array_equal(go, eo), k
+        assert np.array_equal(gc, ec), k
+        oU, oV = oracle.downsample(eo, W, H, kern)
+        oUm = oracle.median1d(oracle.median1d(oU, 1, p.fsize), 0, p.fsize)
+        oVm = oracle.median1d(oracle.median1d(oV, 1, p.fsize), 0, p.fsize)
+        tU, tV = oracle.trilateral(oU, oV, oUm, oVm, oF[k][1].astype(np.float32), p.sEPE ** 2, p.sI ** 2,
+                                   p.sS ** 2)
+        assert np.abs(gU - tU).max() <= FLOW_TOL and np.abs(gV - tV).max() <= FLOW_TOL
+        prev = eo
+
+
+def _field_invariants(off, cost, F1, F2, NS, row0=0, row_step=1):
+    """Size-independent checks of a final field on the rows row0::row_step:
+    in-image targets, and the stored cost is the Hamming distance between the
+    pixel's descriptor and the descriptor at the stored offset."""
+    sy, sx = off.shape
+    rows = np.arange(row0, sy, row_step)
+    o = off[rows]
+    dx = (o & 0xFFFF).astype(np.uint16).view(np.int16).astype(np.int64)
+    dy = (o >> 16).astype(np.uint16).view(np.int16).astype(np.int64)
+    xx = np.arange(sx)[None, :]
+    tx, ty = xx + dx, rows[:, None] + dy
+    assert (tx >= 0).all() and (tx < sx).all() and (ty >= 0).all() and (ty < sy).all()
+    x = np.bitwise_xor(F1[rows][..., :NS], F2[ty, tx][..., :NS])
+    pc = np.unpackbits(np.ascontiguousarray(x).view(np.uint8), axis=-1).sum(-1)
+    assert np.array_equal(pc, cost[rows].astype(np.int64))
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("name,seed", [("c2_1920x1080_u2.9_n128", 0), ("c3_1280x720_u4.9_n512", 0)])
+def test_full_size_match_field(name, seed):
+    """The 1080p/128-bit and 720p/512-bit workloads: bit-exact match field against
+    the oracle, the field invariants, and the final flow on bands of rows."""
+    say("full size", name)
+    wl = synth.WORKLOADS[name]
+    pr = synth.workload_pair(name, seed=seed)
+    W, H = wl["W"], wl["H"]
+    p = bm.derive_params(W, H, wl["up"], wl["up"], N=wl["N"])
+    ctx = bm.Context(p, seed=1)
+    U, V = ctx.flow_numpy(pr.frame1, pr.frame2, pair=0)
+    off, cost = ctx.field()
+    pat = ctx.pattern()
+    oF1, og, _ = _oracle_descs(pr.frame1, p, pat)
+    oF2, _, _ = _oracle_descs(pr.frame2, p, pat)
+    eo, ec = oracle.match(oF1, oF2, 2, 256, True, True, None, 0, 1)
+    assert np.array_equal(off, eo)
+    assert np.array_equal(cost, ec)
+    _field_invariants(off, cost, oF1, oF2, (p.N + 31) // 32, row0=3, row_step=61)
+    del oF1, oF2
+    oU, oV = oracle.downsample(eo, W, H, ctx.gauss_kernel())
+    oUm = oracle.median1d(oracle.median1d(oU, 1, p.fsize), 0, p.fsize)
+    oVm = oracle.median1d(oracle.median1d(oV, 1, p.fsize), 0, p.fsize)
+    guide = og.astype(np.float32)
+    for r0, r1 in ((0, 24), (H // 2 - 12, H // 2 + 12), (H - 24, H)):
+        tU, tV = oracle.trilateral(oU, oV, oUm, oVm, guide, p.sEPE ** 2, p.sI ** 2, p.sS ** 2, r0, r1)
+        assert np.abs(U[r0:r1] - tU[r0:r1]).max() <= FLOW_TOL, (r0, r1)
+        assert np.abs(V[r0:r1] - tV[r0:r1]).max() <= FLOW_TOL, (r0, r1)
+    e_all, e_noc = bm.epe(U, V, pr.gt, pr.occ)
+    print("[epe]", name, e_all, e_noc, flush=True)
+
