@@ -52,6 +52,10 @@ struct bm_ctx {
     std::vector<float> gauss;
     // tables
     int16_t *d_pattern = nullptr;
+    uint32_t *d_plan = nullptr;  // fused BRIEF test records
+    int *d_plan_count = nullptr;
+    std::vector<uint32_t> plan;
+    std::vector<int> plan_count;
     uint32_t *d_up_x = nullptr, *d_up_y = nullptr, *d_nn_x = nullptr, *d_nn_y = nullptr;
     float *d_gauss = nullptr, *d_spt = nullptr;
     // work buffers
@@ -193,13 +197,94 @@ void make_pattern(uint64_t seed, unsigned N, float patchArea, std::vector<int16_
     }
 }
 
+// Fused BRIEF tiling: the far halo is the largest pattern offset, the near halo
+// the smallest of 8/16/24/32 covering min(far, 32); the gray tile bound is the
+// widest input-resolution extent of any tile.  The kernel is used when one
+// work-group's LDS fits in 64 KiB.
+void plan_brief(bm_ctx *ctx, int rmax)
+{
+    bm::Geometry &g = ctx->g;
+    const int want = std::min(rmax, 32);
+    const int rn = want <= 8 ? 8 : want <= 16 ? 16 : want <= 24 ? 24 : 32;
+    rmax = std::max(rmax, rn);  // the coordinate tables also cover the near tile
+    g.brief_far = rmax;
+    g.brief_near = 0;
+    std::vector<uint32_t> upx(g.sx), upy(g.sy);
+    for (int x = 0; x < g.sx; ++x) upx[x] = (uint32_t)((uint64_t)x * g.W * 256u / g.sx);
+    for (int y = 0; y < g.sy; ++y) upy[y] = (uint32_t)((uint64_t)y * g.H * 256u / g.sy);
+    auto extent = [&](const std::vector<uint32_t> &up, int n, int in, int t0, int tw) {
+        const int c0 = std::max(0, std::min(n - 1, t0 - rmax));
+        const int c1 = std::max(0, std::min(n - 1, t0 + tw - 1 + rmax));
+        const int a = (int)(up[c0] >> 8), b = std::min((int)(up[c1] >> 8) + 1, in - 1);
+        return b - a + 1;
+    };
+    int gw = 0;
+    for (int t0 = 0; t0 < g.sx; t0 += 64) gw = std::max(gw, extent(upx, g.sx, g.W, t0, 64));
+    const int TH = bm::brief_tile_rows(g.NSP), which = TH == 32 ? 1 : 0;
+    int gh = 0;
+    for (int t0 = 0; t0 < g.sy; t0 += TH) gh = std::max(gh, extent(upy, g.sy, g.H, t0, TH));
+    g.brief_gw[which] = gw;
+    g.brief_gh[which] = gh;
+    g.brief_gw[1 - which] = g.brief_gh[1 - which] = 0;
+    if (gw >= 4096 || (size_t)gh * gw >= 32768) return;
+    if (bm::brief_lds_bytes(g, rn, TH) > 64 * 1024) return;
+    g.brief_near = rn;
+}
+
+// Test records of the fused BRIEF: per descriptor word, the near tests (all
+// offsets within the near halo; record = LDS offsets of both samples) then the
+// far tests (record = packed int16 offsets), each with its bit position.
+void build_brief_plan(bm_ctx *ctx)
+{
+    const bm::Geometry &g = ctx->g;
+    const int NSP = g.NSP, RN = g.brief_near, pitch = 64 + 2 * RN;
+    ctx->plan.assign((size_t)NSP * 32 * 4, 0u);
+    ctx->plan_count.assign((size_t)NSP * 2, 0);
+    for (int w = 0; w < g.NS; ++w) {
+        std::vector<int> near_, far_;
+        for (int b = 0; b < 32 && 32 * w + b < g.N; ++b) {
+            const int16_t *o = &ctx->pattern[4 * (32 * w + b)];
+            const int m = std::max(std::max(std::abs((int)o[0]), std::abs((int)o[1])),
+                                   std::max(std::abs((int)o[2]), std::abs((int)o[3])));
+            (m <= RN ? near_ : far_).push_back(b);
+        }
+        int r = 0;
+        for (int b : near_) {
+            const int16_t *o = &ctx->pattern[4 * (32 * w + b)];
+            uint32_t *rec = &ctx->plan[((size_t)32 * w + r++) * 4];
+            rec[0] = (uint32_t)(o[1] * pitch + o[0]);
+            rec[1] = (uint32_t)(o[3] * pitch + o[2]);
+            rec[2] = (uint32_t)b;
+        }
+        for (int b : far_) {
+            const int16_t *o = &ctx->pattern[4 * (32 * w + b)];
+            uint32_t *rec = &ctx->plan[((size_t)32 * w + r++) * 4];
+            rec[0] = (uint32_t)(uint16_t)o[0] | ((uint32_t)(uint16_t)o[1] << 16);
+            rec[1] = (uint32_t)(uint16_t)o[2] | ((uint32_t)(uint16_t)o[3] << 16);
+            rec[2] = (uint32_t)b;
+        }
+        ctx->plan_count[2 * w] = (int)near_.size();
+        ctx->plan_count[2 * w + 1] = (int)far_.size();
+    }
+}
+
+int upload_pattern(bm_ctx *ctx)
+{
+    int r = upload(ctx, ctx->d_pattern, ctx->pattern);
+    if (r == BM_OK) r = upload(ctx, ctx->d_plan, ctx->plan);
+    if (r == BM_OK) r = upload(ctx, ctx->d_plan_count, ctx->plan_count);
+    return r;
+}
+
 int set_pattern_internal(bm_ctx *ctx, const std::vector<int16_t> &pat)
 {
     ctx->pattern = pat;
     int rmax = 0;
     for (int16_t v : pat) rmax = std::max(rmax, std::abs((int)v));
     ctx->g.brief_radius = std::min(rmax, bm::brief_max_radius());
-    if (ctx->d_pattern) return upload(ctx, ctx->d_pattern, pat);
+    plan_brief(ctx, rmax);
+    build_brief_plan(ctx);
+    if (ctx->d_pattern) return upload_pattern(ctx);
     return BM_OK;
 }
 
@@ -430,6 +515,8 @@ int bm_ctx_create(const bm_params *params, uint64_t seed, bm_ctx **out)
 #define BM_ALLOC(ptr, n) \
     if (r == BM_OK) r = dev_alloc(ctx, &(ptr), (n))
     BM_ALLOC(ctx->d_pattern, pat.size());
+    BM_ALLOC(ctx->d_plan, ctx->plan.size());
+    BM_ALLOC(ctx->d_plan_count, ctx->plan_count.size());
     BM_ALLOC(ctx->d_up_x, upx.size());
     BM_ALLOC(ctx->d_up_y, upy.size());
     BM_ALLOC(ctx->d_nn_x, nnx.size());
@@ -450,7 +537,7 @@ int bm_ctx_create(const bm_params *params, uint64_t seed, bm_ctx **out)
     BM_ALLOC(ctx->d_Um, Win);
     BM_ALLOC(ctx->d_Vm, Win);
 #undef BM_ALLOC
-    if (r == BM_OK) r = upload(ctx, ctx->d_pattern, pat);
+    if (r == BM_OK) r = upload_pattern(ctx);
     if (r == BM_OK) r = upload(ctx, ctx->d_up_x, upx);
     if (r == BM_OK) r = upload(ctx, ctx->d_up_y, upy);
     if (r == BM_OK) r = upload(ctx, ctx->d_nn_x, nnx);
@@ -463,6 +550,8 @@ int bm_ctx_create(const bm_params *params, uint64_t seed, bm_ctx **out)
         return fail(nullptr, r, msg);
     }
     g.pattern = ctx->d_pattern;
+    g.brief_plan = ctx->d_plan;
+    g.brief_plan_count = ctx->d_plan_count;
     g.up_x = ctx->d_up_x;
     g.up_y = ctx->d_up_y;
     g.nn_x = ctx->d_nn_x;

@@ -23,7 +23,12 @@ struct Geometry {
     int P;           // Gaussian taps
     int fsize;       // median window
     int jfMax, iterations, doubleProp;
-    int brief_radius;             // LDS halo of the BRIEF tile (pixels)
+    int brief_radius;             // LDS halo of the stand-alone BRIEF tile (pixels)
+    int brief_far;                // largest |pattern offset| (fused BRIEF far halo)
+    int brief_near;               // fused BRIEF near halo: 8, 16, 24 or 32; 0 = kernel unavailable
+    int brief_gw[2], brief_gh[2]; // largest gray tile of the fused BRIEF, [0]: 16-row tiles, [1]: 32-row tiles
+    const uint32_t *brief_plan;   // fused BRIEF test records (4 words each, 32 per descriptor word)
+    const int *brief_plan_count;  // per descriptor word: near and far record counts
     uint64_t seed;                // pattern + random-search key
     int64_t rs_sigma_q16;         // random-search sigma in Q16
     float tri_a, tri_b;           // trilateral range scales (see bm_kernels.hip)
@@ -61,6 +66,10 @@ hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
 
 // LDS budget of the BRIEF tile: the largest halo radius that fits.
 int brief_max_radius();
+// Fused BRIEF: LDS bytes of one work-group for near halo RN and tile height TH,
+// and the tile height used for a descriptor stride.
+size_t brief_lds_bytes(const Geometry &g, int RN, int TH);
+int brief_tile_rows(int NSP);
 
 }  // namespace bm
 

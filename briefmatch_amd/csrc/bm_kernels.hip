@@ -45,6 +45,22 @@ __device__ __forceinline__ int xcd_remap(int b, int B)
     return k * q + min(k, r) + (b >> 3);
 }
 
+// Row-segment blocks (kThreads pixels of one row) visited strip by strip: within
+// a strip of kStrip rows the order runs down a column of blocks first, so the
+// blocks resident on one XCD at any time cover a compact 2-D patch and the
+// data-dependent descriptor gathers around it stay in that XCD's L2.
+constexpr int kStrip = 256;
+
+__device__ __forceinline__ void strip_block(int b, int gx, int rows_total, int &bx, int &by)
+{
+    const int per = gx * kStrip;
+    const int strip = b / per;
+    const int rows = min(kStrip, rows_total - strip * kStrip);
+    const int rem = b - strip * per;
+    bx = rem / rows;
+    by = strip * kStrip + (rem - bx * rows);
+}
+
 __device__ __forceinline__ void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3,
                                               uint32_t k0, uint32_t k1)
 {
@@ -259,6 +275,157 @@ __global__ void __launch_bounds__(kThreads) k_brief(const float *__restrict__ L,
 }
 
 // ---------------------------------------------------------------------------
+// BRIEF with the up-sampling fused, everything in LDS.  Per work-group (64 x TH
+// matched pixels, TH = 4 PY, thread (lx, ly) owns rows ly + 4k):
+//   gt  the input-resolution gray tile covering the tile plus the FAR halo Rf
+//       (the largest pattern offset), bytes;
+//   xt, yt  per matched column / row of that halo: the bilinear source index
+//       (relative to gt), its clamped neighbour step and the 8-bit weight;
+//   lt  the up-sampled tile plus a NEAR halo RN <= Rf, FP32, computed from gt.
+// Tests whose offsets lie within RN read lt (two LDS loads per pixel, the
+// per-pixel offsets are immediates because the pitch is a template constant);
+// the rest evaluate the bilinear samples from gt.  No test reads global memory.
+
+struct BriefArgs {
+    const uint8_t *g8;
+    const uint32_t *up_x, *up_y;
+    const uint4 *plan;      // 32 records per descriptor word, near tests first
+    const int *plan_count;  // per word: near count, far count
+    int N, W, H, sx, sy, Rf, gw, gh;
+    uint32_t *F;
+};
+
+__device__ __forceinline__ uint32_t bil_lds(const uint8_t *gt, uint32_t xv, uint32_t yv, int gw)
+{
+    const int x0 = (int)(xv & 0xfffu), dx1 = (int)((xv >> 12) & 1u);
+    const uint32_t ax = xv >> 16;
+    const int r0 = (int)(yv & 0x7fffu), dr1 = (int)((yv >> 15) & 1u) * gw;
+    const uint32_t ay = yv >> 16;
+    const uint8_t *p0 = gt + r0 + x0, *p1 = p0 + dr1;
+    const uint32_t top = __umul24(p0[0], 256u - ax) + __umul24(p0[dx1], ax);
+    const uint32_t bot = __umul24(p1[0], 256u - ax) + __umul24(p1[dx1], ax);
+    return __umul24(top, 256u - ay) + __umul24(bot, ay);
+}
+
+// Shift one comparison bit into a descriptor word: a > b for non-negative
+// values compared as unsigned integers (FP32 bit patterns of non-negative
+// floats order like the floats), bit = sign of (b - a), funnel-shifted in.
+__device__ __forceinline__ uint32_t push_bit(uint32_t word, uint32_t a, uint32_t b)
+{
+    return __builtin_amdgcn_alignbit(word, b - a, 31);
+}
+
+template <int NSP, int RN, int PY>
+__global__ void __launch_bounds__(kThreads) k_brief_lds(BriefArgs a)
+{
+    constexpr int TH = 4 * PY, PITCH = kBriefTW + 2 * RN, ROWS = TH + 2 * RN;
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *lt = reinterpret_cast<float *>(smem);
+    const int Rf = a.Rf, XT = kBriefTW + 2 * Rf, YT = TH + 2 * Rf;
+    uint32_t *xt = reinterpret_cast<uint32_t *>(lt + PITCH * ROWS);
+    uint32_t *yt = xt + XT;
+    uint8_t *gt = reinterpret_cast<uint8_t *>(yt + YT);
+    const int sx = a.sx, sy = a.sy, W = a.W, H = a.H;
+
+    const int gxb = (sx + kBriefTW - 1) / kBriefTW, gyb = (sy + TH - 1) / TH;
+    const int b = xcd_remap(blockIdx.x, gxb * gyb);
+    const int tx0 = (b % gxb) * kBriefTW, ty0 = (b / gxb) * TH;
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+
+    const int gx0 = (int)(a.up_x[clampi(tx0 - Rf, 0, sx - 1)] >> 8);
+    const int gy0 = (int)(a.up_y[clampi(ty0 - Rf, 0, sy - 1)] >> 8);
+    const int gx1 = min((int)(a.up_x[clampi(tx0 + kBriefTW - 1 + Rf, 0, sx - 1)] >> 8) + 1, W - 1);
+    const int gy1 = min((int)(a.up_y[clampi(ty0 + TH - 1 + Rf, 0, sy - 1)] >> 8) + 1, H - 1);
+    const int gw = gx1 - gx0 + 1, gh = gy1 - gy0 + 1;
+
+    for (int i = threadIdx.x; i < XT; i += kThreads) {
+        const uint32_t u = a.up_x[clampi(tx0 - Rf + i, 0, sx - 1)];
+        const int x0 = (int)(u >> 8), x1 = min(x0 + 1, W - 1);
+        xt[i] = (uint32_t)(x0 - gx0) | ((uint32_t)(x1 - x0) << 12) | ((u & 255u) << 16);
+    }
+    for (int i = threadIdx.x; i < YT; i += kThreads) {
+        const uint32_t u = a.up_y[clampi(ty0 - Rf + i, 0, sy - 1)];
+        const int y0 = (int)(u >> 8), y1 = min(y0 + 1, H - 1);
+        yt[i] = (uint32_t)((y0 - gy0) * gw) | ((uint32_t)(y1 - y0) << 15) | ((u & 255u) << 16);
+    }
+    for (int r = ly; r < gh; r += 4) {
+        const uint8_t *src = a.g8 + (size_t)(gy0 + r) * W + gx0;
+        for (int c = lx; c < gw; c += 64) gt[r * gw + c] = src[c];
+    }
+    __syncthreads();
+    for (int r = ly; r < ROWS; r += 4) {
+        const uint32_t yv = yt[r - RN + Rf];
+        for (int c = lx; c < PITCH; c += 64)
+            lt[r * PITCH + c] = (float)bil_lds(gt, xt[c - RN + Rf], yv, gw) * (1.0f / 65536.0f);
+    }
+    __syncthreads();
+
+    const float *lbase = lt + (ly + RN) * PITCH + lx + RN;
+    const uint32_t *xrow = xt + lx + Rf;
+    const uint32_t *ycol = yt + ly + Rf;
+    uint32_t words[NSP][PY];
+#pragma unroll
+    for (int w = 0; w < NSP; ++w)
+#pragma unroll
+        for (int k = 0; k < PY; ++k) words[w][k] = 0u;
+
+    // The plan lists each word's tests near-first: record {o1, o2, bit, 0} with
+    // o1/o2 the LDS offsets (row * PITCH + col) of a near test, or the packed
+    // (ox, oy) int16 pairs of a far one.  Bits are placed explicitly, so the
+    // order of evaluation does not matter.
+#pragma unroll
+    for (int w = 0; w < NSP; ++w) {
+        const uint4 *rec = a.plan + 32 * w;
+        const int nnear = a.plan_count[2 * w], nfar = a.plan_count[2 * w + 1];
+        int t = 0;
+        for (; t + 2 <= nnear; t += 2) {
+            const uint4 r0 = rec[t], r1 = rec[t + 1];
+            const uint32_t *p1 = reinterpret_cast<const uint32_t *>(lbase + (int)r0.x);
+            const uint32_t *p2 = reinterpret_cast<const uint32_t *>(lbase + (int)r0.y);
+            const uint32_t *p3 = reinterpret_cast<const uint32_t *>(lbase + (int)r1.x);
+            const uint32_t *p4 = reinterpret_cast<const uint32_t *>(lbase + (int)r1.y);
+#pragma unroll
+            for (int k = 0; k < PY; ++k) {
+                words[w][k] |= ((p2[4 * k * PITCH] - p1[4 * k * PITCH]) >> 31) << r0.z;
+                words[w][k] |= ((p4[4 * k * PITCH] - p3[4 * k * PITCH]) >> 31) << r1.z;
+            }
+        }
+        if (t < nnear) {
+            const uint4 r0 = rec[t];
+            const uint32_t *p1 = reinterpret_cast<const uint32_t *>(lbase + (int)r0.x);
+            const uint32_t *p2 = reinterpret_cast<const uint32_t *>(lbase + (int)r0.y);
+#pragma unroll
+            for (int k = 0; k < PY; ++k) words[w][k] |= ((p2[4 * k * PITCH] - p1[4 * k * PITCH]) >> 31) << r0.z;
+        }
+        for (int f = 0; f < nfar; ++f) {
+            const uint4 r0 = rec[nnear + f];
+            const int ox1 = (int16_t)(r0.x & 0xffffu), oy1 = (int16_t)(r0.x >> 16);
+            const int ox2 = (int16_t)(r0.y & 0xffffu), oy2 = (int16_t)(r0.y >> 16);
+            const uint32_t xa = xrow[ox1], xb = xrow[ox2];
+#pragma unroll
+            for (int k = 0; k < PY; ++k) {
+                const uint32_t va = bil_lds(gt, xa, ycol[4 * k + oy1], gw);
+                const uint32_t vb = bil_lds(gt, xb, ycol[4 * k + oy2], gw);
+                words[w][k] |= ((vb - va) >> 31) << r0.z;
+            }
+        }
+    }
+    const int x = tx0 + lx;
+    if (x < sx) {
+#pragma unroll
+        for (int k = 0; k < PY; ++k) {
+            const int y = ty0 + ly + 4 * k;
+            if (y < sy) {
+                uint32_t out[NSP];
+#pragma unroll
+                for (int w = 0; w < NSP; ++w) out[w] = words[w][k];
+                store_desc<NSP>(a.F, (uint32_t)(y * sx + x), out);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // field initialisation
 
 template <int NSP>
@@ -268,8 +435,9 @@ __global__ void __launch_bounds__(kThreads) k_init(const uint32_t *__restrict__ 
                                                    int sx, int sy)
 {
     const int gx = (sx + kThreads - 1) / kThreads;
-    const int b = xcd_remap(blockIdx.x, gx * sy);
-    const int x = (b % gx) * kThreads + threadIdx.x, y = b / gx;
+    int bx, y;
+    strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, bx, y);
+    const int x = bx * kThreads + threadIdx.x;
     if (x >= sx) return;
     const int idx = y * sx + x;
     int nx = 0, ny = 0;
@@ -307,14 +475,16 @@ __device__ __forceinline__ void random_candidate(const uint32_t (&f1)[NSP], cons
     int ny = round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c1 >> 20]);
     nx = clampi(nx, -x, sx - 1 - x);
     ny = clampi(ny, -y, sy - 1 - y);
-    uint32_t f2[NSP];
-    load_desc<NSP>(F2, idx + ny * sx + nx, f2);
-    const int c = hamming<NSP>(f1, f2);
     const uint32_t cand = pack_off(nx, ny);
     // insert() skips a candidate equal to the current offset (briefmatch_cuda.cu:131-132)
-    if (cand != best && c < bcost) {
-        bcost = c;
-        best = cand;
+    if (cand != best) {
+        uint32_t f2[NSP];
+        load_desc<NSP>(F2, idx + ny * sx + nx, f2);
+        const int c = hamming<NSP>(f1, f2);
+        if (c < bcost) {
+            bcost = c;
+            best = cand;
+        }
     }
 }
 
@@ -327,16 +497,15 @@ __global__ void __launch_bounds__(kThreads) k_level(const uint32_t *__restrict__
                                                     int pd, int double_prop, RSArgs rs)
 {
     const int gx = (sx + kThreads - 1) / kThreads;
-    const int b = xcd_remap(blockIdx.x, gx * sy);
-    const int x = (b % gx) * kThreads + threadIdx.x, y = b / gx;
+    int bx, y;
+    strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, bx, y);
+    const int x = bx * kThreads + threadIdx.x;
     if (x >= sx) return;
     const int M = sx * sy;
     const int idx = y * sx + x;
 
-    uint32_t f1[NSP];
-    load_desc<NSP>(F1, idx, f1);
-    uint32_t best = off_in[idx];
-    int bcost = cost_in[idx];
+    const uint32_t own = off_in[idx];
+    const int own_cost = cost_in[idx];
 
     // candidate order of the reference: d = -1 (vertical, horizontal), d = +1 (...)
     const int sv = pd * p * sx, sh = pd * p;
@@ -348,25 +517,48 @@ __global__ void __launch_bounds__(kThreads) k_level(const uint32_t *__restrict__
         valid[c] = (c >= 2 || double_prop) && (unsigned)nidx[c] < (unsigned)M;
         o[c] = valid[c] ? off_in[nidx[c]] : 0u;
     }
-    int cst[4];
     uint32_t cand[4];
+    int tgt[4];
+    bool need[4];
+    bool any = false;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const int nx = clampi(off_dx(o[c]), -x, sx - 1 - x);
         const int ny = clampi(off_dy(o[c]), -y, sy - 1 - y);
         cand[c] = pack_off(nx, ny);
+        tgt[c] = idx + ny * sx + nx;
+        // a candidate equal to the pixel's own offset or to an earlier candidate
+        // has a known cost: no gather (insert() likewise skips equal offsets)
+        bool fresh = valid[c] && cand[c] != own;
+#pragma unroll
+        for (int e = 0; e < c; ++e) fresh = fresh && !(valid[e] && cand[e] == cand[c]);
+        need[c] = fresh;
+        any = any || fresh;
+    }
+    uint32_t f1[NSP];
+    if (any || RS) load_desc<NSP>(F1, idx, f1);
+    int cst[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
         cst[c] = 0x7fffffff;
-        if (valid[c]) {
+        if (need[c]) {
             uint32_t f2[NSP];
-            load_desc<NSP>(F2, idx + ny * sx + nx, f2);
+            load_desc<NSP>(F2, tgt[c], f2);
             cst[c] = hamming<NSP>(f1, f2);
+        } else if (valid[c]) {
+            cst[c] = own_cost;
+#pragma unroll
+            for (int e = c - 1; e >= 0; --e)
+                if (valid[e] && cand[e] == cand[c]) cst[c] = cst[e];
         }
     }
     // sequential insert() semantics: skip a candidate equal to the running best,
     // accept a strictly lower cost
+    uint32_t best = own;
+    int bcost = own_cost;
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-        if (cand[c] != best && cst[c] < bcost) {
+        if (valid[c] && cand[c] != best && cst[c] < bcost) {
             bcost = cst[c];
             best = cand[c];
         }
@@ -383,8 +575,9 @@ __global__ void __launch_bounds__(kThreads) k_random_search(const uint32_t *__re
                                                             RSArgs rs)
 {
     const int gx = (sx + kThreads - 1) / kThreads;
-    const int b = xcd_remap(blockIdx.x, gx * sy);
-    const int x = (b % gx) * kThreads + threadIdx.x, y = b / gx;
+    int bx, y;
+    strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, bx, y);
+    const int x = bx * kThreads + threadIdx.x;
     if (x >= sx) return;
     const int idx = y * sx + x;
     uint32_t f1[NSP];
@@ -587,6 +780,16 @@ int brief_max_radius()
     return R;
 }
 
+size_t brief_lds_bytes(const Geometry &g, int RN, int TH)
+{
+    const size_t lt = (size_t)(kBriefTW + 2 * RN) * (TH + 2 * RN) * sizeof(float);
+    const size_t tabs = (size_t)(kBriefTW + 2 * g.brief_far + TH + 2 * g.brief_far) * sizeof(uint32_t);
+    const int which = TH == 32 ? 1 : 0;
+    return lt + tabs + (size_t)g.brief_gw[which] * g.brief_gh[which];
+}
+
+int brief_tile_rows(int NSP) { return NSP <= 4 ? 32 : 16; }
+
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
 {
     const size_t n = bytes / 16;
@@ -620,9 +823,44 @@ static hipError_t launch_brief_t(const Geometry &g, const float *L, const uint8_
     return hipGetLastError();
 }
 
+template <int NSP, int RN>
+static hipError_t launch_brief_lds_t(const Geometry &g, const uint8_t *g8, uint32_t *F, hipStream_t st)
+{
+    constexpr int PY = NSP <= 4 ? 8 : 4;
+    constexpr int TH = 4 * PY;
+    const size_t lds = brief_lds_bytes(g, RN, TH);
+    BriefArgs a{g8, g.up_x, g.up_y, reinterpret_cast<const uint4 *>(g.brief_plan), g.brief_plan_count, g.N, g.W, g.H, g.sx, g.sy, g.brief_far, g.brief_gw[PY == 8 ? 1 : 0],
+                g.brief_gh[PY == 8 ? 1 : 0], F};
+    const int gxb = (g.sx + kBriefTW - 1) / kBriefTW, gyb = (g.sy + TH - 1) / TH;
+    hipLaunchKernelGGL((k_brief_lds<NSP, RN, PY>), dim3(gxb * gyb), dim3(kThreads), lds, st, a);
+    return hipGetLastError();
+}
+
+template <int NSP>
+static hipError_t launch_brief_lds_n(const Geometry &g, const uint8_t *g8, uint32_t *F, hipStream_t st)
+{
+    switch (g.brief_near) {
+    case 8: return launch_brief_lds_t<NSP, 8>(g, g8, F, st);
+    case 16: return launch_brief_lds_t<NSP, 16>(g, g8, F, st);
+    case 24: return launch_brief_lds_t<NSP, 24>(g, g8, F, st);
+    case 32: return launch_brief_lds_t<NSP, 32>(g, g8, F, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_brief(const Geometry &g, const float *L, const uint8_t *g8, uint32_t *F, hipStream_t st)
 {
     const bool fused = (L == nullptr);
+    if (fused && g.brief_near > 0) {
+        switch (g.NSP) {
+        case 1: return launch_brief_lds_n<1>(g, g8, F, st);
+        case 2: return launch_brief_lds_n<2>(g, g8, F, st);
+        case 4: return launch_brief_lds_n<4>(g, g8, F, st);
+        case 8: return launch_brief_lds_n<8>(g, g8, F, st);
+        case 16: return launch_brief_lds_n<16>(g, g8, F, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (g.NSP) {
     case 1: return fused ? launch_brief_t<1, true>(g, L, g8, F, st) : launch_brief_t<1, false>(g, L, g8, F, st);
     case 2: return fused ? launch_brief_t<2, true>(g, L, g8, F, st) : launch_brief_t<2, false>(g, L, g8, F, st);

@@ -133,7 +133,9 @@ int bm_brief_gray(bm_ctx *ctx, const uint8_t *gray8_dev, uint32_t *F_dev);
 int bm_init(bm_ctx *ctx, const uint32_t *F1_dev, const uint32_t *F2_dev, int first,
             const uint32_t *prev_off_dev, uint32_t *off_dev, uint16_t *cost_dev);
 /* One level of jump flooding from a snapshot (off_in/cost_in) into a different
- * buffer (off_out/cost_out). */
+ * buffer (off_out/cost_out).  cost_in[i] must be the Hamming cost of off_in[i],
+ * as in every field this library produces (and in the reference's D/NX/NY):
+ * candidates equal to a known offset reuse its cost instead of a new gather. */
 int bm_propagate_level(bm_ctx *ctx, const uint32_t *F1_dev, const uint32_t *F2_dev,
                        const uint32_t *off_in_dev, const uint16_t *cost_in_dev,
                        uint32_t *off_out_dev, uint16_t *cost_out_dev, int p, int pd);

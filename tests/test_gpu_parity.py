@@ -106,9 +106,11 @@ def test_levels_random_search_init(N, double_prop):
     dx = np.clip(rng.integers(-25, 26, (sy, sx)), -xx, sx - 1 - xx).astype(np.int16)
     dy = np.clip(rng.integers(-25, 26, (sy, sx)), -yy, sy - 1 - yy).astype(np.int16)
     off0 = (dx.view(np.uint16).astype(np.uint32) | (dy.view(np.uint16).astype(np.uint32) << 16))
-    # costs of that field from the oracle's temporal init would differ; use init(first) costs + random field
-    _, c_first = oracle.init(oF1, oF2, first=True)
-    cost0 = np.minimum(c_first, np.uint16(N)).astype(np.uint16)
+    # the stored cost of every entry is the Hamming cost of its offset, as in any
+    # field the library produces (the precondition of bm_propagate_level)
+    tx, ty = xx + dx, yy + dy
+    x = np.bitwise_xor(oF1, oF2[ty, tx])
+    cost0 = np.unpackbits(np.ascontiguousarray(x).view(np.uint8), axis=-1).sum(-1).astype(np.uint16)
     oi, ci = bm.DeviceBuffer(array=off0), bm.DeviceBuffer(array=cost0)
     oo, co = ctx.field_buffers()
     for p_lvl in (256, 31, 8, 1):
