@@ -62,15 +62,15 @@ def build_cpp(verbose=False):
     if not all(os.path.exists(s) for s in srcs):
         return None
     rpath = "-Wl,-rpath,$ORIGIN"
-    flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", "-I" + INCLUDE, "-I/opt/rocm/include",
-             "-D__HIP_PLATFORM_AMD__"]
+    flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-ffp-contract=off", "-I" + INCLUDE]
     tmp = CPPLIB + ".tmp"
-    _run(["g++"] + flags + ["-shared", "-o", tmp] + srcs + ["-L" + PKG, "-lbriefmatch_amd", rpath], verbose)
+    _run(["g++"] + flags + ["-shared", "-o", tmp] + srcs + ["-L" + PKG, "-lbriefmatch_amd", "-lz", rpath], verbose)
     os.replace(tmp, CPPLIB)
     main = os.path.join(CSRC, "main.cpp")
     if os.path.exists(main):
         tmp = CLI + ".tmp"
-        _run(["g++"] + flags + ["-o", tmp, main, "-L" + PKG, "-lbriefmatch_cpp", "-lbriefmatch_amd", rpath], verbose)
+        _run(["g++"] + flags + ["-o", tmp, main, "-L" + PKG, "-lbriefmatch_cpp", "-lbriefmatch_amd", "-lz", rpath],
+             verbose)
         os.replace(tmp, CLI)
     return CPPLIB
 

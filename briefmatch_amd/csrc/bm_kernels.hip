@@ -488,6 +488,11 @@ __device__ __forceinline__ void random_candidate(const uint32_t (&f1)[NSP], cons
     }
 }
 
+// Pixels per thread of the level kernels: two (256 apart, so every access of a
+// wave stays one coalesced row segment) for short descriptors, one for long ones.
+template <int NSP>
+constexpr int level_px() { return NSP <= 4 ? 2 : 1; }
+
 template <int NSP, bool RS>
 __global__ void __launch_bounds__(kThreads) k_level(const uint32_t *__restrict__ F1, const uint32_t *__restrict__ F2,
                                                     const uint32_t *__restrict__ off_in,
@@ -496,75 +501,122 @@ __global__ void __launch_bounds__(kThreads) k_level(const uint32_t *__restrict__
                                                     uint16_t *__restrict__ cost_out, int sx, int sy, int p,
                                                     int pd, int double_prop, RSArgs rs)
 {
-    const int gx = (sx + kThreads - 1) / kThreads;
+    constexpr int PX = level_px<NSP>();
+    constexpr int span = kThreads * PX;
+    const int gx = (sx + span - 1) / span;
     int bx, y;
     strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, bx, y);
-    const int x = bx * kThreads + threadIdx.x;
-    if (x >= sx) return;
     const int M = sx * sy;
-    const int idx = y * sx + x;
-
-    const uint32_t own = off_in[idx];
-    const int own_cost = cost_in[idx];
-
-    // candidate order of the reference: d = -1 (vertical, horizontal), d = +1 (...)
     const int sv = pd * p * sx, sh = pd * p;
-    const int nidx[4] = {idx - sv, idx - sh, idx + sv, idx + sh};
-    bool valid[4];
-    uint32_t o[4];
+
+    int xq[PX], idx[PX];
+    bool live[PX];
+    uint32_t own[PX];
+    int own_cost[PX];
+    uint32_t o[PX][4];
+    bool valid[PX][4];
+    // round 1: own entry and the four neighbours' offsets of the snapshot
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        valid[c] = (c >= 2 || double_prop) && (unsigned)nidx[c] < (unsigned)M;
-        o[c] = valid[c] ? off_in[nidx[c]] : 0u;
-    }
-    uint32_t cand[4];
-    int tgt[4];
-    bool need[4];
-    bool any = false;
+    for (int q = 0; q < PX; ++q) {
+        xq[q] = bx * span + q * kThreads + threadIdx.x;
+        live[q] = xq[q] < sx;
+        idx[q] = y * sx + min(xq[q], sx - 1);
+        own[q] = live[q] ? off_in[idx[q]] : 0u;
+        own_cost[q] = live[q] ? (int)cost_in[idx[q]] : 0;
+        // candidate order of the reference: d = -1 (vertical, horizontal), d = +1 (...)
+        const int nidx[4] = {idx[q] - sv, idx[q] - sh, idx[q] + sv, idx[q] + sh};
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int nx = clampi(off_dx(o[c]), -x, sx - 1 - x);
-        const int ny = clampi(off_dy(o[c]), -y, sy - 1 - y);
-        cand[c] = pack_off(nx, ny);
-        tgt[c] = idx + ny * sx + nx;
-        // a candidate equal to the pixel's own offset or to an earlier candidate
-        // has a known cost: no gather (insert() likewise skips equal offsets)
-        bool fresh = valid[c] && cand[c] != own;
-#pragma unroll
-        for (int e = 0; e < c; ++e) fresh = fresh && !(valid[e] && cand[e] == cand[c]);
-        need[c] = fresh;
-        any = any || fresh;
-    }
-    uint32_t f1[NSP];
-    if (any || RS) load_desc<NSP>(F1, idx, f1);
-    int cst[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        cst[c] = 0x7fffffff;
-        if (need[c]) {
-            uint32_t f2[NSP];
-            load_desc<NSP>(F2, tgt[c], f2);
-            cst[c] = hamming<NSP>(f1, f2);
-        } else if (valid[c]) {
-            cst[c] = own_cost;
-#pragma unroll
-            for (int e = c - 1; e >= 0; --e)
-                if (valid[e] && cand[e] == cand[c]) cst[c] = cst[e];
+        for (int c = 0; c < 4; ++c) {
+            valid[q][c] = live[q] && (c >= 2 || double_prop) && (unsigned)nidx[c] < (unsigned)M;
+            o[q][c] = valid[q][c] ? off_in[nidx[c]] : 0u;
         }
     }
-    // sequential insert() semantics: skip a candidate equal to the running best,
-    // accept a strictly lower cost
-    uint32_t best = own;
-    int bcost = own_cost;
+    // the random-search candidate depends only on (pixel, iteration, pair)
+    uint32_t rcand[PX];
+    int rtgt[PX];
+    if constexpr (RS) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-        if (valid[c] && cand[c] != best && cst[c] < bcost) {
-            bcost = cst[c];
-            best = cand[c];
+        for (int q = 0; q < PX; ++q) {
+            uint32_t c0 = (uint32_t)idx[q], c1 = rs.iteration, c2 = rs.pair, c3 = 0u;
+            philox4x32_10(c0, c1, c2, c3, rs.key0, rs.key1);
+            const int x = min(xq[q], sx - 1);
+            const int nx = clampi(round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c0 >> 20]), -x, sx - 1 - x);
+            const int ny = clampi(round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c1 >> 20]), -y, sy - 1 - y);
+            rcand[q] = pack_off(nx, ny);
+            rtgt[q] = idx[q] + ny * sx + nx;
         }
-    if constexpr (RS) random_candidate<NSP>(f1, F2, idx, x, y, sx, sy, rs, best, bcost);
-    off_out[idx] = best;
-    cost_out[idx] = (uint16_t)bcost;
+    }
+    uint32_t cand[PX][4];
+    int tgt[PX][4];
+    bool need[PX][4], any[PX];
+#pragma unroll
+    for (int q = 0; q < PX; ++q) {
+        const int x = min(xq[q], sx - 1);
+        any[q] = false;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int nx = clampi(off_dx(o[q][c]), -x, sx - 1 - x);
+            const int ny = clampi(off_dy(o[q][c]), -y, sy - 1 - y);
+            cand[q][c] = pack_off(nx, ny);
+            tgt[q][c] = idx[q] + ny * sx + nx;
+            // a candidate equal to the pixel's own offset or to an earlier candidate
+            // has a known cost: no gather (insert() likewise skips equal offsets)
+            bool fresh = valid[q][c] && cand[q][c] != own[q];
+#pragma unroll
+            for (int e = 0; e < c; ++e) fresh = fresh && !(valid[q][e] && cand[q][e] == cand[q][c]);
+            need[q][c] = fresh;
+            any[q] = any[q] || fresh;
+        }
+    }
+    // round 2: own descriptor, candidate gathers and the random-search gather,
+    // all issued before any cost is formed
+    uint32_t f1[PX][NSP], f2[PX][4][NSP], fr[PX][NSP];
+    bool rs_need[PX];
+#pragma unroll
+    for (int q = 0; q < PX; ++q) {
+        rs_need[q] = RS && live[q] && rcand[q] != own[q];
+        if (any[q] || rs_need[q]) load_desc<NSP>(F1, (uint32_t)idx[q], f1[q]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (need[q][c]) load_desc<NSP>(F2, (uint32_t)tgt[q][c], f2[q][c]);
+        if (rs_need[q]) load_desc<NSP>(F2, (uint32_t)rtgt[q], fr[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < PX; ++q) {
+        int cst[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            cst[c] = 0x7fffffff;
+            if (need[q][c]) {
+                cst[c] = hamming<NSP>(f1[q], f2[q][c]);
+            } else if (valid[q][c]) {
+                cst[c] = own_cost[q];
+#pragma unroll
+                for (int e = c - 1; e >= 0; --e)
+                    if (valid[q][e] && cand[q][e] == cand[q][c]) cst[c] = cst[e];
+            }
+        }
+        const int rcost = rs_need[q] ? hamming<NSP>(f1[q], fr[q]) : 0x7fffffff;
+        // sequential insert() semantics: skip a candidate equal to the running
+        // best, accept a strictly lower cost; the random candidate comes last
+        uint32_t best = own[q];
+        int bcost = own_cost[q];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (valid[q][c] && cand[q][c] != best && cst[c] < bcost) {
+                bcost = cst[c];
+                best = cand[q][c];
+            }
+        // equal to own: skipped when own is still the best, not cheaper otherwise
+        if (rs_need[q] && rcand[q] != best && rcost < bcost) {
+            bcost = rcost;
+            best = rcand[q];
+        }
+        if (live[q]) {
+            off_out[idx[q]] = best;
+            cost_out[idx[q]] = (uint16_t)bcost;
+        }
+    }
 }
 
 template <int NSP>
@@ -906,7 +958,8 @@ hipError_t launch_level(const Geometry &g, const uint32_t *F1, const uint32_t *F
                         const uint16_t *cost_in, uint32_t *off_out, uint16_t *cost_out, int p, int pd,
                         int rs_iteration, uint32_t pair, hipStream_t st)
 {
-    const dim3 grid(blocks_rows(g.sx, g.sy));
+    const int span = kThreads * (g.NSP <= 4 ? 2 : 1);
+    const dim3 grid((unsigned)(((g.sx + span - 1) / span) * g.sy));
     const RSArgs rs = make_rs(g, rs_iteration >= 0, (uint32_t)(rs_iteration < 0 ? 0 : rs_iteration), pair);
     if (rs.enabled) {
         BM_NSP_DISPATCH(g.NSP, hipLaunchKernelGGL((k_level<NSP, true>), grid, dim3(kThreads), 0, st, F1, F2, off_in,

@@ -1,0 +1,112 @@
+"""Turns rocprofv3 CSV output into the short text summaries kept under profiles/.
+
+    python -m briefmatch_amd.rocprof_summary stats  <kernel_stats.csv> <out.txt> --cmd "<command>"
+    python -m briefmatch_amd.rocprof_summary traffic <fetch_counter_collection.csv> <write_counter_collection.csv>
+           <copy_fetch.csv> <copy_write.csv> <copy_bytes> <out.txt> <traffic.json> --cmd "<commands>"
+
+`traffic` scales each kernel's mean FETCH_SIZE and WRITE_SIZE (KiB per dispatch)
+by what the same counter reads for the plain copy kernel (k_copy) of known
+size, and writes the HBM bytes per launch per kernel to traffic.json, which
+bench.py reports as roofline.traffic.
+"""
+import argparse
+import csv
+import json
+import re
+from collections import OrderedDict, defaultdict
+
+
+def short_name(name):
+    n = name.replace("bm::(anonymous namespace)::", "").replace("void ", "")
+    m = re.match(r"([A-Za-z_0-9]+(?:<[^>]*>)?)", n)
+    return m.group(1) if m else n[:40]
+
+
+STAGE_OF = OrderedDict([("k_gray", "gray"), ("k_upsample", "upsample"), ("k_brief", "brief"),
+                        ("k_init", "init"), ("k_level<", "level"), ("k_random_search", "random_search"),
+                        ("k_downsample", "downsample"), ("k_median", "median"), ("k_trilateral", "trilateral"),
+                        ("k_copy", "copy")])
+
+
+def stage_of(short):
+    for k, v in STAGE_OF.items():
+        if short.startswith(k):
+            if v == "level" and short.endswith("true>"):
+                return "level_rs"
+            return v
+    return short
+
+
+def stats(path, out, cmd):
+    rows = list(csv.DictReader(open(path)))
+    lines = ["# rocprofv3 kernel statistics", "# command: " + cmd, "",
+             "%-34s %7s %12s %12s %12s %7s" % ("kernel", "calls", "total_ms", "avg_us", "min_us", "pct")]
+    for r in rows:
+        lines.append("%-34s %7d %12.3f %12.2f %12.2f %7.2f" % (
+            short_name(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"]) / 1e6,
+            float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3, float(r["Percentage"])))
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def _mean_by_kernel(path):
+    acc = defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        acc[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def traffic(fetch, write, cfetch, cwrite, copy_bytes, out, out_json, cmd):
+    f = _mean_by_kernel(fetch)
+    w = _mean_by_kernel(write)
+    cf = _mean_by_kernel(cfetch)
+    cw = _mean_by_kernel(cwrite)
+    kib = copy_bytes / 1024.0
+    sf = kib / cf["k_copy"]
+    sw = kib / cw["k_copy"]
+    lines = ["# HBM traffic per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch)",
+             "# command(s): " + cmd,
+             "# calibration: k_copy of %d bytes reads FETCH_SIZE %.1f KiB, WRITE_SIZE %.1f KiB;" % (
+                 copy_bytes, cf["k_copy"], cw["k_copy"]),
+             "#   scale factors fetch x%.4f, write x%.4f" % (sf, sw), "",
+             "%-34s %14s %14s %14s" % ("kernel", "fetch_MB", "write_MB", "total_MB")]
+    tj = {"_calibration": {"copy_bytes": copy_bytes, "fetch_scale": sf, "write_scale": sw}}
+    per_stage = defaultdict(list)
+    for k in f:
+        if k == "k_copy" or k.startswith("__amd"):
+            continue
+        fb, wb = f[k] * sf * 1024, w.get(k, 0.0) * sw * 1024
+        lines.append("%-34s %14.1f %14.1f %14.1f" % (k, fb / 1e6, wb / 1e6, (fb + wb) / 1e6))
+        per_stage[stage_of(k)].append(fb + wb)
+    for st, v in per_stage.items():
+        tj[st] = sum(v) / len(v)
+    with open(out, "w") as fo:
+        fo.write("\n".join(lines) + "\n")
+    with open(out_json, "w") as fo:
+        json.dump(tj, fo, indent=1)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="mode", required=True)
+    a = sub.add_parser("stats")
+    a.add_argument("csv")
+    a.add_argument("out")
+    a.add_argument("--cmd", required=True)
+    b = sub.add_parser("traffic")
+    for n in ("fetch", "write", "copy_fetch", "copy_write"):
+        b.add_argument(n)
+    b.add_argument("copy_bytes", type=int)
+    b.add_argument("out")
+    b.add_argument("out_json")
+    b.add_argument("--cmd", required=True)
+    args = ap.parse_args()
+    if args.mode == "stats":
+        stats(args.csv, args.out, args.cmd)
+    else:
+        traffic(args.fetch, args.write, args.copy_fetch, args.copy_write, args.copy_bytes, args.out, args.out_json,
+                args.cmd)
+
+
+if __name__ == "__main__":
+    main()

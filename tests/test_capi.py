@@ -28,6 +28,13 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_io_library_exports_every_declared_symbol():
+    from briefmatch_amd import io as bio
+    L = bio.lib()
+    missing = [n for n in _declared("bm_io.h") if not hasattr(L, n)]
+    assert not missing, missing
+
+
 def test_cpp_api_library_exports():
     so = os.path.join(ROOT, "briefmatch_amd", "libbriefmatch_cpp.so")
     if not os.path.exists(so):
