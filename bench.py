@@ -126,6 +126,64 @@ def cpu_baseline(pr, p, pattern, seed, gpu_off, gpu_cost, gpu_U, gpu_V, kernel, 
     return cb, parity
 
 
+def bench_batch(a):
+    """Uneven pairs (sizes uniform in [320x240, 1920x1080], random aspect,
+    32/128-bit, up-sampling 1.5-2.9): per-pair launches on one stream against
+    one hipGraph launch of all pairs, results compared bit for bit."""
+    import briefmatch_amd as bm
+    from briefmatch_amd import synth
+    rng = np.random.default_rng(a.seed)
+    params, frames = [], []
+    t0 = time.perf_counter()
+    for i in range(a.pairs):
+        W, H = int(rng.integers(320, 1921)), int(rng.integers(240, 1081))
+        N = int(rng.choice([32, 128]))
+        u = float(rng.uniform(1.5, 2.9))
+        mm = 8.0 + 32.0 * (max(W, H) - 320) / 1600.0
+        pr = synth.make_pair(W, H, motion="rects", n_rects=3 + (i % 6), max_motion=mm, seed=1000 + i)
+        params.append(bm.derive_params(W, H, u, u, N=N))
+        frames.append(pr)
+    gen_s = time.perf_counter() - t0
+    batch = bm.Batch(params, seed=1, streams=8)
+    r1 = [bm.DeviceBuffer(array=f.frame1) for f in frames]
+    r2 = [bm.DeviceBuffer(array=f.frame2) for f in frames]
+    Us = [bm.DeviceBuffer(p.sx_in * p.sy_in * 4) for p in params]
+    Vs = [bm.DeviceBuffer(p.sx_in * p.sy_in * 4) for p in params]
+    tuned = 0
+    if not a.no_autotune:
+        for i, c in enumerate(batch.ctx):
+            c.autotune(r1[i], r2[i])
+            tuned += 1
+    pix = sum(p.sx_in * p.sy_in for p in params)
+    out = {}
+    fields = {}
+    for name, mode in (("per_pair", bm.BATCH_PER_PAIR), ("graph", bm.BATCH_GRAPH)):
+        for _ in range(max(1, a.warmup)):
+            batch.flow(r1, r2, Us, Vs, mode=mode)
+        batch.sync()
+        batch.timer_start()
+        for _ in range(a.steps):
+            batch.flow(r1, r2, Us, Vs, mode=mode)
+        ms = batch.timer_stop() / a.steps
+        out[name] = {"ms_per_batch": round(ms, 3), "value": round(pix / (ms * 1e3), 2), "unit": "Mpix/s"}
+        fields[name] = [(c.field(), U.download((p.sy_in, p.sx_in), np.float32)) for c, U, p in
+                        zip(batch.ctx, Us, params)]
+    exact = all(np.array_equal(x[0][0], y[0][0]) and np.array_equal(x[0][1], y[0][1]) and np.array_equal(x[1], y[1])
+                for x, y in zip(fields["per_pair"], fields["graph"]))
+    res = {"metric": "batched flow Mpix/s (%d uneven pairs)" % a.pairs, "value": out["graph"]["value"],
+           "unit": "Mpix/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": out["graph"]["ms_per_batch"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u32+f32", "data": "synthetic",
+           "config": {"workload": "batch%d" % a.pairs, "pairs": a.pairs, "input_mpix": round(pix / 1e6, 3),
+                      "matched_mpix": round(sum(p.sx * p.sy for p in params) / 1e6, 3),
+                      "feature_bits": sorted(set(int(p.N) for p in params)), "autotuned_pairs": tuned,
+                      "streams": 8, "generator_seconds": round(gen_s, 1)},
+           "per_pair": out["per_pair"], "graph": out["graph"],
+           "graph_over_per_pair": round(out["graph"]["value"] / out["per_pair"]["value"], 3),
+           "bit_exact_graph_vs_per_pair": bool(exact)}
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +194,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and parity leg")
     ap.add_argument("--band-rows", type=int, default=135, help="trilateral rows of the CPU sample")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--pairs", type=int, default=64, help="pairs of the batch workload")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (--gpus %d requested); the flow path shards by "
@@ -161,6 +221,9 @@ def main():
                           "GBps_read_plus_write": 2 * n / (ms / cnt / 1e3) / 1e9}))
         return
 
+    if a.workload.startswith("batch"):
+        return bench_batch(a)
+
     wl = synth.WORKLOADS[a.workload]
     pr = synth.workload_pair(a.workload, seed=a.seed)
     W, H = wl["W"], wl["H"]
@@ -170,6 +233,7 @@ def main():
     rgb1, rgb2 = bm.DeviceBuffer(array=pr.frame1), bm.DeviceBuffer(array=pr.frame2)
     U, V = ctx.image_buffer(), ctx.image_buffer()
 
+    tuning = ctx.tuning().as_dict() if a.no_autotune else ctx.autotune(rgb1, rgb2).as_dict()
     for _ in range(a.warmup):
         ctx.flow_pair(rgb1, rgb2, U, V)
     ctx.sync()
@@ -229,7 +293,7 @@ def main():
         "config": {"workload": a.workload, "input": "%dx%d RGB uint8" % (W, H),
                    "matched": "%dx%d" % (p.sx, p.sy), "feature_bits": p.N, "iterations": p.iterations,
                    "jfMax": p.jfMax, "median": p.fsize, "trilateral_window": 2 * s_tri,
-                   "mode": "isolated pair, inputs resident in HBM"},
+                   "mode": "isolated pair, inputs resident in HBM", "tuning": tuning},
         "roofline": roof,
         "stages": stages,
         "streaming": {"ms_per_frame": round(st_ms, 4), "value": round(W * H / (st_ms * 1e3), 3),

@@ -35,6 +35,14 @@ class BriefMatchError(RuntimeError):
         self.code = code
 
 
+class Tuning(C.Structure):
+    """bm_tuning: launch-shape choices; results do not depend on them."""
+    _fields_ = [(n, C.c_int) for n in ("level_px", "strip_rows", "brief_near", "tri_band")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class Params(C.Structure):
     """bm_params: the fields of BMParams (briefmatch.h:65-89)."""
     _fields_ = [(n, C.c_uint) for n in ("sx", "sy", "sx_in", "sy_in", "N", "iterations", "jfMax", "P",
@@ -105,6 +113,17 @@ def lib():
         "bm_profile_enable": (I, [V, I]),
         "bm_profile_read": (I, [V, V, V]),
         "bm_device_copy": (I, [V, V, V, C.c_size_t]),
+        "bm_get_tuning": (I, [V, C.POINTER(Tuning)]),
+        "bm_set_tuning": (I, [V, C.POINTER(Tuning)]),
+        "bm_autotune": (I, [V, V, V, C.c_char_p, C.POINTER(Tuning)]),
+        "bm_batch_create": (I, [I, C.POINTER(Params), U64, I, C.POINTER(V)]),
+        "bm_batch_destroy": (None, [V]),
+        "bm_batch_ctx": (I, [V, I, C.POINTER(V)]),
+        "bm_batch_flow": (I, [V, V, V, V, V, I]),
+        "bm_batch_sync": (I, [V]),
+        "bm_batch_timer_start": (I, [V]),
+        "bm_batch_timer_stop": (I, [V, C.POINTER(F)]),
+        "bm_batch_last_error": (C.c_char_p, [V]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -359,6 +378,24 @@ class Context:
     def device_copy(self, dst, src, nbytes):
         self._c(lib().bm_device_copy(self.h, _ptr(dst), _ptr(src), nbytes))
 
+    def tuning(self):
+        t = Tuning()
+        self._c(lib().bm_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, **kw):
+        t = self.tuning()
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._c(lib().bm_set_tuning(self.h, C.byref(t)))
+        return t
+
+    def autotune(self, rgb1, rgb2, cache_path=None):
+        t = Tuning()
+        self._c(lib().bm_autotune(self.h, _ptr(rgb1), _ptr(rgb2), cache_path.encode() if cache_path else None,
+                                  C.byref(t)))
+        return t
+
     def timer_start(self):
         self._c(lib().bm_timer_start(self.h))
 
@@ -376,3 +413,74 @@ class Context:
         self.flow_pair(r1, r2, U, V, pair=pair)
         self.sync()
         return U.download((self.H, self.W), np.float32), V.download((self.H, self.W), np.float32)
+
+
+class _BorrowedContext(Context):
+    """A context owned by a Batch (not destroyed on its own)."""
+
+    def __init__(self, handle):
+        self.h = handle
+        got = Params()
+        _check(lib().bm_ctx_get_params(self.h, C.byref(got)), self.h)
+        self.params = got
+        p = got
+        self.W, self.H, self.sx, self.sy = p.sx_in, p.sy_in, p.sx, p.sy
+        self.N = p.N
+        self.NS = (p.N + 31) // 32
+        self.NSP = lib().bm_descriptor_stride(self.h)
+
+    def close(self):
+        self.h = None
+
+
+BATCH_PER_PAIR, BATCH_GRAPH = 0, 1
+
+
+class Batch:
+    """Pairs of uneven size / up-sampling / feature length run together
+    (bm_batch_*): per-pair launches or one captured hipGraph."""
+
+    def __init__(self, params_list, seed=0, streams=8):
+        arr = (Params * len(params_list))(*[p.copy() for p in params_list])
+        h = C.c_void_p()
+        _check(lib().bm_batch_create(len(params_list), arr, seed, streams, C.byref(h)))
+        self.h = h
+        self.n = len(params_list)
+        self.ctx = []
+        for i in range(self.n):
+            c = C.c_void_p()
+            _check(lib().bm_batch_ctx(self.h, i, C.byref(c)))
+            self.ctx.append(_BorrowedContext(c))
+
+    def _c(self, status):
+        if status != BM_OK:
+            raise BriefMatchError(status, (lib().bm_batch_last_error(self.h) or b"").decode(errors="replace"))
+
+    def flow(self, rgb1, rgb2, U, V, mode=BATCH_GRAPH):
+        n = self.n
+        arrs = [(C.c_void_p * n)(*[_ptr(b) for b in lst]) for lst in (rgb1, rgb2, U, V)]
+        self._c(lib().bm_batch_flow(self.h, *arrs, mode))
+
+    def sync(self):
+        self._c(lib().bm_batch_sync(self.h))
+
+    def timer_start(self):
+        self._c(lib().bm_batch_timer_start(self.h))
+
+    def timer_stop(self):
+        ms = C.c_float()
+        self._c(lib().bm_batch_timer_stop(self.h, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            for c in self.ctx:
+                c.h = None
+            lib().bm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

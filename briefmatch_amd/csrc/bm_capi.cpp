@@ -7,7 +7,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <new>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -201,11 +203,11 @@ void make_pattern(uint64_t seed, unsigned N, float patchArea, std::vector<int16_
 // the smallest of 8/16/24/32 covering min(far, 32); the gray tile bound is the
 // widest input-resolution extent of any tile.  The kernel is used when one
 // work-group's LDS fits in 64 KiB.
-void plan_brief(bm_ctx *ctx, int rmax)
+bool plan_brief(bm_ctx *ctx, int rmax, int rn_req)
 {
     bm::Geometry &g = ctx->g;
     const int want = std::min(rmax, 32);
-    const int rn = want <= 8 ? 8 : want <= 16 ? 16 : want <= 24 ? 24 : 32;
+    const int rn = rn_req > 0 ? rn_req : (want <= 8 ? 8 : want <= 16 ? 16 : want <= 24 ? 24 : 32);
     rmax = std::max(rmax, rn);  // the coordinate tables also cover the near tile
     g.brief_far = rmax;
     g.brief_near = 0;
@@ -226,9 +228,11 @@ void plan_brief(bm_ctx *ctx, int rmax)
     g.brief_gw[which] = gw;
     g.brief_gh[which] = gh;
     g.brief_gw[1 - which] = g.brief_gh[1 - which] = 0;
-    if (gw >= 4096 || (size_t)gh * gw >= 32768) return;
-    if (bm::brief_lds_bytes(g, rn, TH) > 64 * 1024) return;
+    if (rn_req == 0) return true;  // stand-alone kernel requested
+    if (gw >= 4096 || (size_t)gh * gw >= 32768) return false;
+    if (bm::brief_lds_bytes(g, rn, TH) > 64 * 1024) return false;
     g.brief_near = rn;
+    return true;
 }
 
 // Test records of the fused BRIEF: per descriptor word, the near tests (all
@@ -276,13 +280,20 @@ int upload_pattern(bm_ctx *ctx)
     return r;
 }
 
-int set_pattern_internal(bm_ctx *ctx, const std::vector<int16_t> &pat)
+int pattern_rmax(const bm_ctx *ctx)
+{
+    int rmax = 0;
+    for (int16_t v : ctx->pattern) rmax = std::max(rmax, std::abs((int)v));
+    return rmax;
+}
+
+// rn_req: -1 default near halo, 0 stand-alone kernel, else the requested halo.
+int set_pattern_internal(bm_ctx *ctx, const std::vector<int16_t> &pat, int rn_req = -1)
 {
     ctx->pattern = pat;
-    int rmax = 0;
-    for (int16_t v : pat) rmax = std::max(rmax, std::abs((int)v));
+    const int rmax = pattern_rmax(ctx);
     ctx->g.brief_radius = std::min(rmax, bm::brief_max_radius());
-    plan_brief(ctx, rmax);
+    if (!plan_brief(ctx, rmax, rn_req)) return fail(ctx, BM_E_PARAM, "BRIEF near halo does not fit in LDS");
     build_brief_plan(ctx);
     if (ctx->d_pattern) return upload_pattern(ctx);
     return BM_OK;
@@ -463,6 +474,8 @@ int bm_ctx_create(const bm_params *params, uint64_t seed, bm_ctx **out)
     g.iterations = (int)p.iterations;
     g.doubleProp = p.doubleProp ? 1 : 0;
     g.seed = seed;
+    g.level_px = NSP <= 4 ? 2 : 1;
+    g.strip_rows = 256;
     const float rs_sigma = std::sqrt(std::ceil((float)p.sx * 0.5f / 3.0f));
     g.rs_sigma_q16 = (int64_t)std::llround((double)rs_sigma * 65536.0);
 
@@ -486,6 +499,7 @@ int bm_ctx_create(const bm_params *params, uint64_t seed, bm_ctx **out)
         delete ctx;
         return fail(nullptr, BM_E_PARAM, "trilateral window outside [1, 2000]");
     }
+    g.tri_band = p.filter ? bm::tri_max_band(g.tri_s) : 1;
     std::vector<float> spt(std::max(2 * g.tri_s, 1));
     for (int m_ = 0; m_ < 2 * g.tri_s; ++m_) {
         const double i = m_ - g.tri_s;
@@ -608,7 +622,10 @@ int bm_set_pattern(bm_ctx *ctx, const int16_t *pat)
     if (!ctx || !pat) return fail(nullptr, BM_E_PARAM, "null argument");
     std::vector<int16_t> v(pat, pat + 4 * (size_t)ctx->p.N);
     BM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return set_pattern_internal(ctx, v);
+    const int keep = ctx->g.brief_near > 0 ? ctx->g.brief_near : -1;
+    int r = set_pattern_internal(ctx, v, keep);
+    if (r != BM_OK && keep > 0) r = set_pattern_internal(ctx, v, -1);
+    return r;
 }
 
 int bm_get_gauss_kernel(const bm_ctx *ctx, float *out, unsigned *P)
@@ -973,6 +990,369 @@ int bm_epe(const float *U, const float *V, const float *gu, const float *gv, con
     }
     *epe_all = na ? sa / (double)na : 0.0;
     *epe_noc = nn ? sn / (double)nn : 0.0;
+    return BM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// tuning, autotuning, batches
+
+struct bm_batch {
+    std::vector<bm_ctx *> ctx;
+    std::vector<hipStream_t> streams;
+    hipStream_t origin = nullptr;
+    hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> joins;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<const void *> key;
+    std::string err;
+};
+
+namespace {
+
+int tuning_valid(const bm_ctx *ctx, const bm_tuning &t, std::string &why)
+{
+    if (t.level_px != 1 && t.level_px != 2) return why = "level_px must be 1 or 2", 0;
+    if (t.level_px == 2 && ctx->g.NSP > 8) return why = "level_px 2 needs at most 256 feature bits", 0;
+    if (t.strip_rows < 16 || t.strip_rows > 4096) return why = "strip_rows outside [16, 4096]", 0;
+    if (!(t.brief_near == 0 || t.brief_near == 8 || t.brief_near == 16 || t.brief_near == 24 || t.brief_near == 32))
+        return why = "brief_near must be 0, 8, 16, 24 or 32", 0;
+    const int maxb = ctx->p.filter ? bm::tri_max_band(ctx->g.tri_s) : 16;
+    if (t.tri_band < 1 || t.tri_band > std::max(1, maxb)) return why = "tri_band outside the LDS limit", 0;
+    return 1;
+}
+
+int apply_tuning(bm_ctx *ctx, const bm_tuning &t)
+{
+    std::string why;
+    if (!tuning_valid(ctx, t, why)) return fail(ctx, BM_E_PARAM, why);
+    BM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (t.brief_near != ctx->g.brief_near) {
+        const std::vector<int16_t> pat = ctx->pattern;
+        const int r = set_pattern_internal(ctx, pat, t.brief_near);
+        if (r != BM_OK) return r;
+    }
+    ctx->g.level_px = t.level_px;
+    ctx->g.strip_rows = t.strip_rows;
+    ctx->g.tri_band = t.tri_band;
+    return BM_OK;
+}
+
+bm_tuning current_tuning(const bm_ctx *ctx)
+{
+    bm_tuning t;
+    t.level_px = ctx->g.level_px;
+    t.strip_rows = ctx->g.strip_rows;
+    t.brief_near = ctx->g.brief_near;
+    t.tri_band = ctx->g.tri_band;
+    return t;
+}
+
+std::string tuning_key(const bm_ctx *ctx)
+{
+    hipDeviceProp_t prop;
+    int dev = 0;
+    std::string arch = "unknown";
+    int cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        arch = prop.gcnArchName;
+        cus = prop.multiProcessorCount;
+    }
+    for (char &c : arch)
+        if (c == ' ' || c == ':') c = '_';
+    std::ostringstream k;
+    k << arch << "/" << cus << "cu/" << ctx->g.W << "x" << ctx->g.H << "/" << ctx->g.sx << "x" << ctx->g.sy << "/n"
+      << ctx->g.N << "/m" << ctx->g.fsize << "/s" << ctx->g.tri_s << "/r" << pattern_rmax(ctx) << "/f"
+      << ctx->p.filter;
+    return k.str();
+}
+
+}  // namespace
+
+extern "C" {
+
+int bm_get_tuning(const bm_ctx *ctx, bm_tuning *out)
+{
+    if (!ctx || !out) return fail(nullptr, BM_E_PARAM, "null argument");
+    *out = current_tuning(ctx);
+    return BM_OK;
+}
+
+int bm_set_tuning(bm_ctx *ctx, const bm_tuning *t)
+{
+    if (!ctx || !t) return fail(nullptr, BM_E_PARAM, "null argument");
+    return apply_tuning(ctx, *t);
+}
+
+int bm_autotune(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, const char *cache_path, bm_tuning *out)
+{
+    int r = check(ctx, rgb1, "rgb1");
+    if (!r) r = check(ctx, rgb2, "rgb2");
+    if (r) return r;
+    const std::string key = tuning_key(ctx);
+    if (cache_path) {
+        std::ifstream in(cache_path);
+        std::string line;
+        while (std::getline(in, line)) {
+            std::istringstream ls(line);
+            std::string k;
+            bm_tuning t;
+            if (ls >> k >> t.level_px >> t.strip_rows >> t.brief_near >> t.tri_band && k == key) {
+                std::string why;
+                if (tuning_valid(ctx, t, why) && apply_tuning(ctx, t) == BM_OK) {
+                    if (out) *out = t;
+                    return BM_OK;
+                }
+            }
+        }
+    }
+    const size_t Win = (size_t)ctx->g.W * ctx->g.H;
+    float *U = nullptr, *V = nullptr;
+    BM_HIP(ctx, hipMalloc(&U, Win * sizeof(float)));
+    if (hipMalloc(&V, Win * sizeof(float)) != hipSuccess) {
+        (void)hipFree(U);
+        return fail(ctx, BM_E_ALLOC, "autotune scratch allocation failed");
+    }
+    const bool prof = ctx->prof;
+    ctx->prof = true;
+    // device time of the stages a knob affects, second of two runs
+    auto measure = [&](const bm_tuning &t, std::initializer_list<int> stages, double &ms) -> int {
+        int e = apply_tuning(ctx, t);
+        if (e != BM_OK) return e;
+        double tot[BM_STAGE_COUNT];
+        unsigned n[BM_STAGE_COUNT];
+        for (int rep = 0; rep < 2; ++rep) {
+            e = bm_profile_read(ctx, nullptr, nullptr);
+            if (e == BM_OK) e = bm_flow_pair(ctx, rgb1, rgb2, 0, U, V);
+            if (e == BM_OK) e = bm_profile_read(ctx, tot, n);
+            if (e != BM_OK) return e;
+        }
+        ms = 0.0;
+        for (int st : stages) ms += tot[st];
+        return BM_OK;
+    };
+    bm_tuning best = current_tuning(ctx);
+    struct Knob {
+        int bm_tuning::*field;
+        std::vector<int> values;
+        std::initializer_list<int> stages;
+    };
+    const std::initializer_list<int> lv = {BM_STAGE_INIT, BM_STAGE_LEVEL, BM_STAGE_LEVEL_RS};
+    const std::initializer_list<int> br = {BM_STAGE_BRIEF};
+    const std::initializer_list<int> tr = {BM_STAGE_TRILATERAL};
+    std::vector<int> bands;
+    const int maxb = ctx->p.filter ? bm::tri_max_band(ctx->g.tri_s) : 0;
+    for (int v : {4, 8, 12, 16})
+        if (v <= maxb) bands.push_back(v);
+    const Knob knobs[] = {{&bm_tuning::brief_near, {16, 24, 32}, br},
+                          {&bm_tuning::level_px, {1, 2}, lv},
+                          {&bm_tuning::strip_rows, {64, 128, 256, 512, 1024}, lv},
+                          {&bm_tuning::tri_band, bands, tr}};
+    int e = BM_OK;
+    for (const Knob &kn : knobs) {
+        double best_ms = -1.0;
+        bm_tuning winner = best;
+        for (int v : kn.values) {
+            bm_tuning t = best;
+            t.*(kn.field) = v;
+            std::string why;
+            if (!tuning_valid(ctx, t, why)) continue;
+            if (t.brief_near > 0) {  // check that the near halo fits before timing it
+                const int save = ctx->g.brief_near;
+                const bool fits = plan_brief(ctx, pattern_rmax(ctx), t.brief_near);
+                plan_brief(ctx, pattern_rmax(ctx), save);
+                if (!fits) continue;
+            }
+            double ms = 0.0;
+            e = measure(t, kn.stages, ms);
+            if (e != BM_OK) break;
+            if (best_ms < 0 || ms < best_ms) {
+                best_ms = ms;
+                winner = t;
+            }
+        }
+        if (e != BM_OK) break;
+        best = winner;
+    }
+    ctx->prof = prof;
+    (void)hipFree(U);
+    (void)hipFree(V);
+    if (e != BM_OK) return e;
+    e = apply_tuning(ctx, best);
+    if (e != BM_OK) return e;
+    if (out) *out = best;
+    if (cache_path) {
+        std::ofstream o(cache_path, std::ios::app);
+        o << key << " " << best.level_px << " " << best.strip_rows << " " << best.brief_near << " " << best.tri_band
+          << "\n";
+    }
+    return BM_OK;
+}
+
+const char *bm_batch_last_error(const bm_batch *b) { return b ? b->err.c_str() : g_thread_error.c_str(); }
+
+void bm_batch_destroy(bm_batch *b)
+{
+    if (!b) return;
+    if (b->origin) (void)hipStreamSynchronize(b->origin);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (bm_ctx *c : b->ctx) bm_ctx_destroy(c);
+    for (hipEvent_t e : b->joins) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {b->fork, b->ev0, b->ev1})
+        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
+    if (b->origin) (void)hipStreamDestroy(b->origin);
+    delete b;
+}
+
+int bm_batch_create(int n, const bm_params *params, uint64_t seed, int streams, bm_batch **out)
+{
+    if (!params || !out || n <= 0) return fail(nullptr, BM_E_PARAM, "bad batch arguments");
+    *out = nullptr;
+    bm_batch *b = new (std::nothrow) bm_batch;
+    if (!b) return fail(nullptr, BM_E_ALLOC, "out of host memory");
+    for (int i = 0; i < n; ++i) {
+        bm_ctx *c = nullptr;
+        const int r = bm_ctx_create(&params[i], seed, &c);
+        if (r != BM_OK) {
+            std::string m = "pair " + std::to_string(i) + ": " + g_thread_error;
+            bm_batch_destroy(b);
+            return fail(nullptr, r, m);
+        }
+        b->ctx.push_back(c);
+    }
+    streams = std::max(1, std::min(streams, n));
+    hipError_t e = hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking);
+    for (int k = 0; k < streams && e == hipSuccess; ++k) {
+        hipStream_t s = nullptr;
+        e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e == hipSuccess) b->streams.push_back(s);
+        hipEvent_t j = nullptr;
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&j, hipEventDisableTiming);
+        if (e == hipSuccess) b->joins.push_back(j);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e != hipSuccess) {
+        std::string m = std::string("HIP stream/event creation failed: ") + hipGetErrorString(e);
+        bm_batch_destroy(b);
+        return fail(nullptr, BM_E_LAUNCH, m);
+    }
+    *out = b;
+    return BM_OK;
+}
+
+int bm_batch_ctx(bm_batch *b, int i, bm_ctx **ctx)
+{
+    if (!b || !ctx || i < 0 || i >= (int)b->ctx.size()) return fail(nullptr, BM_E_PARAM, "bad batch index");
+    *ctx = b->ctx[i];
+    return BM_OK;
+}
+
+static int batch_fail(bm_batch *b, int code, const std::string &m)
+{
+    b->err = m;
+    return code;
+}
+
+// Enqueue every pair of the batch; pair i runs on streams[i % S] (or all on one stream).
+static int batch_enqueue(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r2, float *const *U,
+                         float *const *V, bool one_stream)
+{
+    const int n = (int)b->ctx.size();
+    for (int i = 0; i < n; ++i) {
+        bm_ctx *c = b->ctx[i];
+        hipStream_t own = c->stream;
+        const bool prof = c->prof;
+        c->stream = one_stream ? b->origin : b->streams[i % b->streams.size()];
+        c->prof = false;
+        const int r = bm_flow_pair(c, r1[i], r2[i], (uint32_t)i, U[i], V[i]);
+        c->stream = own;
+        c->prof = prof;
+        if (r != BM_OK) return batch_fail(b, r, "pair " + std::to_string(i) + ": " + c->err);
+    }
+    return BM_OK;
+}
+
+int bm_batch_flow(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r2, float *const *U, float *const *V,
+                  int mode)
+{
+    if (!b || !r1 || !r2 || !U || !V) return fail(nullptr, BM_E_PARAM, "null argument");
+    const int n = (int)b->ctx.size();
+    for (int i = 0; i < n; ++i)
+        if (!r1[i] || !r2[i] || !U[i] || !V[i]) return batch_fail(b, BM_E_PARAM, "null pointer in batch");
+    for (bm_ctx *c : b->ctx)  // work already queued on a pair's own stream comes first
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return batch_fail(b, BM_E_LAUNCH, "stream sync failed");
+    if (mode == BM_BATCH_PER_PAIR) return batch_enqueue(b, r1, r2, U, V, true);
+    if (mode != BM_BATCH_GRAPH) return batch_fail(b, BM_E_PARAM, "unknown batch mode");
+    std::vector<const void *> key;
+    for (int i = 0; i < n; ++i) {
+        key.push_back(r1[i]);
+        key.push_back(r2[i]);
+        key.push_back(U[i]);
+        key.push_back(V[i]);
+    }
+    if (!b->exec || key != b->key) {
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) e = hipEventRecord(b->fork, b->origin);
+        for (hipStream_t s : b->streams)
+            if (e == hipSuccess) e = hipStreamWaitEvent(s, b->fork, 0);
+        int r = BM_OK;
+        if (e == hipSuccess) r = batch_enqueue(b, r1, r2, U, V, false);
+        for (size_t k = 0; k < b->streams.size() && e == hipSuccess; ++k) {
+            e = hipEventRecord(b->joins[k], b->streams[k]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(b->origin, b->joins[k], 0);
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(b->origin, &g);
+        if (e == hipSuccess) e = e2;
+        if (r != BM_OK || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            return r != BM_OK ? r : batch_fail(b, BM_E_LAUNCH, std::string("graph capture: ") + hipGetErrorString(e));
+        }
+        e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return batch_fail(b, BM_E_LAUNCH, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        b->graph = g;
+        b->key = key;
+    }
+    const hipError_t e = hipGraphLaunch(b->exec, b->origin);
+    if (e != hipSuccess) return batch_fail(b, BM_E_LAUNCH, std::string("graph launch: ") + hipGetErrorString(e));
+    return BM_OK;
+}
+
+int bm_batch_sync(bm_batch *b)
+{
+    if (!b) return fail(nullptr, BM_E_PARAM, "null batch");
+    const hipError_t e = hipStreamSynchronize(b->origin);
+    if (e != hipSuccess) return batch_fail(b, BM_E_LAUNCH, std::string("sync: ") + hipGetErrorString(e));
+    return BM_OK;
+}
+
+int bm_batch_timer_start(bm_batch *b)
+{
+    if (!b) return fail(nullptr, BM_E_PARAM, "null batch");
+    if (hipEventRecord(b->ev0, b->origin) != hipSuccess) return batch_fail(b, BM_E_LAUNCH, "event record failed");
+    return BM_OK;
+}
+
+int bm_batch_timer_stop(bm_batch *b, float *ms)
+{
+    if (!b || !ms) return fail(nullptr, BM_E_PARAM, "null argument");
+    if (hipEventRecord(b->ev1, b->origin) != hipSuccess || hipEventSynchronize(b->ev1) != hipSuccess ||
+        hipEventElapsedTime(ms, b->ev0, b->ev1) != hipSuccess)
+        return batch_fail(b, BM_E_LAUNCH, "batch timer failed");
     return BM_OK;
 }
 

@@ -40,7 +40,14 @@ struct Geometry {
     const uint32_t *nn_x, *nn_y;  // nearest-neighbour source column / row per input pixel
     const float *gauss;           // P taps
     const float *tri_spatial;     // 2s entries: log2(e) i^2 / (2 sigmaS^2), i = -s..s-1
+    // tuning (results do not depend on it)
+    int level_px;                 // pixels per thread of the level kernels: 1 or 2
+    int strip_rows;               // strip height of the matching kernels' block order
+    int tri_band;                 // max source rows per LDS band of the trilateral filter
 };
+
+// Largest trilateral band (rows) that fits the LDS budget for half window s.
+int tri_max_band(int s);
 
 hipError_t launch_gray(const Geometry &g, const uint8_t *rgb, uint8_t *g8, float *g32, hipStream_t st);
 hipError_t launch_upsample(const Geometry &g, const uint8_t *g8, float *L, hipStream_t st);

@@ -46,19 +46,17 @@ __device__ __forceinline__ int xcd_remap(int b, int B)
 }
 
 // Row-segment blocks (kThreads pixels of one row) visited strip by strip: within
-// a strip of kStrip rows the order runs down a column of blocks first, so the
+// a strip of strip_rows rows the order runs down a column of blocks first, so the
 // blocks resident on one XCD at any time cover a compact 2-D patch and the
 // data-dependent descriptor gathers around it stay in that XCD's L2.
-constexpr int kStrip = 256;
-
-__device__ __forceinline__ void strip_block(int b, int gx, int rows_total, int &bx, int &by)
+__device__ __forceinline__ void strip_block(int b, int gx, int rows_total, int strip_rows, int &bx, int &by)
 {
-    const int per = gx * kStrip;
+    const int per = gx * strip_rows;
     const int strip = b / per;
-    const int rows = min(kStrip, rows_total - strip * kStrip);
+    const int rows = min(strip_rows, rows_total - strip * strip_rows);
     const int rem = b - strip * per;
     bx = rem / rows;
-    by = strip * kStrip + (rem - bx * rows);
+    by = strip * strip_rows + (rem - bx * rows);
 }
 
 __device__ __forceinline__ void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3,
@@ -432,11 +430,11 @@ template <int NSP>
 __global__ void __launch_bounds__(kThreads) k_init(const uint32_t *__restrict__ F1, const uint32_t *__restrict__ F2,
                                                    int first, const uint32_t *__restrict__ prev,
                                                    uint32_t *__restrict__ off, uint16_t *__restrict__ cost,
-                                                   int sx, int sy)
+                                                   int sx, int sy, int strip_rows)
 {
     const int gx = (sx + kThreads - 1) / kThreads;
     int bx, y;
-    strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, bx, y);
+    strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, strip_rows, bx, y);
     const int x = bx * kThreads + threadIdx.x;
     if (x >= sx) return;
     const int idx = y * sx + x;
@@ -488,24 +486,20 @@ __device__ __forceinline__ void random_candidate(const uint32_t (&f1)[NSP], cons
     }
 }
 
-// Pixels per thread of the level kernels: two (256 apart, so every access of a
-// wave stays one coalesced row segment) for short descriptors, one for long ones.
-template <int NSP>
-constexpr int level_px() { return NSP <= 4 ? 2 : 1; }
-
-template <int NSP, bool RS>
+// PX pixels per thread (256 apart, so every access of a wave stays one
+// coalesced row segment); a tuning choice, results do not depend on it.
+template <int NSP, bool RS, int PX>
 __global__ void __launch_bounds__(kThreads) k_level(const uint32_t *__restrict__ F1, const uint32_t *__restrict__ F2,
                                                     const uint32_t *__restrict__ off_in,
                                                     const uint16_t *__restrict__ cost_in,
                                                     uint32_t *__restrict__ off_out,
                                                     uint16_t *__restrict__ cost_out, int sx, int sy, int p,
-                                                    int pd, int double_prop, RSArgs rs)
+                                                    int pd, int double_prop, int strip_rows, RSArgs rs)
 {
-    constexpr int PX = level_px<NSP>();
     constexpr int span = kThreads * PX;
     const int gx = (sx + span - 1) / span;
     int bx, y;
-    strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, bx, y);
+    strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, strip_rows, bx, y);
     const int M = sx * sy;
     const int sv = pd * p * sx, sh = pd * p;
 
@@ -624,11 +618,11 @@ __global__ void __launch_bounds__(kThreads) k_random_search(const uint32_t *__re
                                                             const uint32_t *__restrict__ F2,
                                                             uint32_t *__restrict__ off,
                                                             uint16_t *__restrict__ cost, int sx, int sy,
-                                                            RSArgs rs)
+                                                            int strip_rows, RSArgs rs)
 {
     const int gx = (sx + kThreads - 1) / kThreads;
     int bx, y;
-    strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, bx, y);
+    strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, strip_rows, bx, y);
     const int x = bx * kThreads + threadIdx.x;
     if (x >= sx) return;
     const int idx = y * sx + x;
@@ -842,6 +836,12 @@ size_t brief_lds_bytes(const Geometry &g, int RN, int TH)
 
 int brief_tile_rows(int NSP) { return NSP <= 4 ? 32 : 16; }
 
+int tri_max_band(int s)
+{
+    const int BR = kTriLdsBytes / ((kTriTW + 2 * s) * (int)sizeof(float4));
+    return BR < 16 ? BR : 16;
+}
+
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
 {
     const size_t n = bytes / 16;
@@ -938,7 +938,7 @@ hipError_t launch_init(const Geometry &g, const uint32_t *F1, const uint32_t *F2
 {
     const dim3 grid(blocks_rows(g.sx, g.sy));
     BM_NSP_DISPATCH(g.NSP, hipLaunchKernelGGL((k_init<NSP>), grid, dim3(kThreads), 0, st, F1, F2, first, prev, off,
-                                              cost, g.sx, g.sy));
+                                              cost, g.sx, g.sy, g.strip_rows));
     return hipGetLastError();
 }
 
@@ -954,21 +954,32 @@ static RSArgs make_rs(const Geometry &g, int enabled, uint32_t iteration, uint32
     return rs;
 }
 
+template <int NSP, bool RS, int PX>
+static void launch_level_t(const Geometry &g, const uint32_t *F1, const uint32_t *F2, const uint32_t *off_in,
+                           const uint16_t *cost_in, uint32_t *off_out, uint16_t *cost_out, int p, int pd,
+                           const RSArgs &rs, hipStream_t st)
+{
+    const int span = kThreads * PX;
+    const dim3 grid((unsigned)(((g.sx + span - 1) / span) * g.sy));
+    hipLaunchKernelGGL((k_level<NSP, RS, PX>), grid, dim3(kThreads), 0, st, F1, F2, off_in, cost_in, off_out,
+                       cost_out, g.sx, g.sy, p, pd, g.doubleProp, g.strip_rows, rs);
+}
+
 hipError_t launch_level(const Geometry &g, const uint32_t *F1, const uint32_t *F2, const uint32_t *off_in,
                         const uint16_t *cost_in, uint32_t *off_out, uint16_t *cost_out, int p, int pd,
                         int rs_iteration, uint32_t pair, hipStream_t st)
 {
-    const int span = kThreads * (g.NSP <= 4 ? 2 : 1);
-    const dim3 grid((unsigned)(((g.sx + span - 1) / span) * g.sy));
     const RSArgs rs = make_rs(g, rs_iteration >= 0, (uint32_t)(rs_iteration < 0 ? 0 : rs_iteration), pair);
+    const bool two = g.level_px == 2;
+#define BM_LEVEL_CALL(RSV)                                                                                   \
+    (two ? launch_level_t<NSP, RSV, 2>(g, F1, F2, off_in, cost_in, off_out, cost_out, p, pd, rs, st)         \
+         : launch_level_t<NSP, RSV, 1>(g, F1, F2, off_in, cost_in, off_out, cost_out, p, pd, rs, st))
     if (rs.enabled) {
-        BM_NSP_DISPATCH(g.NSP, hipLaunchKernelGGL((k_level<NSP, true>), grid, dim3(kThreads), 0, st, F1, F2, off_in,
-                                                  cost_in, off_out, cost_out, g.sx, g.sy, p, pd, g.doubleProp, rs));
+        BM_NSP_DISPATCH(g.NSP, BM_LEVEL_CALL(true));
     } else {
-        BM_NSP_DISPATCH(g.NSP, hipLaunchKernelGGL((k_level<NSP, false>), grid, dim3(kThreads), 0, st, F1, F2,
-                                                  off_in, cost_in, off_out, cost_out, g.sx, g.sy, p, pd,
-                                                  g.doubleProp, rs));
+        BM_NSP_DISPATCH(g.NSP, BM_LEVEL_CALL(false));
     }
+#undef BM_LEVEL_CALL
     return hipGetLastError();
 }
 
@@ -978,7 +989,7 @@ hipError_t launch_random_search(const Geometry &g, const uint32_t *F1, const uin
     const dim3 grid(blocks_rows(g.sx, g.sy));
     const RSArgs rs = make_rs(g, 1, iteration, pair);
     BM_NSP_DISPATCH(g.NSP, hipLaunchKernelGGL((k_random_search<NSP>), grid, dim3(kThreads), 0, st, F1, F2, off,
-                                              cost, g.sx, g.sy, rs));
+                                              cost, g.sx, g.sy, g.strip_rows, rs));
     return hipGetLastError();
 }
 
@@ -1020,7 +1031,7 @@ hipError_t launch_trilateral(const Geometry &g, const float *Ui, const float *Vi
     const int pitch = kTriTW + 2 * s;
     int BR = kTriLdsBytes / (pitch * (int)sizeof(float4));
     if (BR < 1) return hipErrorInvalidValue;
-    if (BR > 16) BR = 16;
+    if (BR > g.tri_band) BR = g.tri_band;
     const size_t lds = (size_t)BR * pitch * sizeof(float4);
     const int gxb = (g.W + kTriTW - 1) / kTriTW, gyb = (g.H + kTriTH - 1) / kTriTH;
     hipLaunchKernelGGL(k_trilateral, dim3(gxb * gyb), dim3(kThreads), lds, st, Ui, Vi, Um, Vm, I, Uo, Vo, g.W, g.H,

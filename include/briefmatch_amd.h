@@ -181,6 +181,45 @@ int bm_stream_frame(bm_ctx *ctx, const uint8_t *rgb_dev, float *U_dev, float *V_
 /* Forget the previous frame (the next bm_stream_frame starts a new sequence). */
 int bm_stream_reset(bm_ctx *ctx);
 
+/* ---- tuning ------------------------------------------------------------------ */
+
+/* Launch-shape choices of one context.  Every valid tuning computes identical
+ * results; only the speed changes. */
+typedef struct bm_tuning {
+    int level_px;     /* pixels per thread of the jump-flooding kernels: 1 or 2 (2 needs N <= 256) */
+    int strip_rows;   /* rows per strip of the matching kernels' work-group order: 16..4096 */
+    int brief_near;   /* LDS near halo of the fused BRIEF: 8, 16, 24, 32; 0 = kernel with global far taps */
+    int tri_band;     /* source rows per LDS band of the trilateral filter: 1..(LDS limit, <= 16) */
+} bm_tuning;
+
+int bm_get_tuning(const bm_ctx *ctx, bm_tuning *out);
+int bm_set_tuning(bm_ctx *ctx, const bm_tuning *t);
+/* Times the pair (device RGB frames) under candidate tunings, one knob at a
+ * time, keeps the fastest and returns it in *out (may be NULL).  With a
+ * cache_path, a tuning stored for the same device and geometry is reused and a
+ * new one is appended. */
+int bm_autotune(bm_ctx *ctx, const uint8_t *rgb1_dev, const uint8_t *rgb2_dev, const char *cache_path,
+                bm_tuning *out);
+
+/* ---- batches of uneven pairs ------------------------------------------------ */
+
+/* n pairs, each with its own parameters (resolution, up-sampling, feature
+ * length may all differ) and context; `streams` parallel branches for the graph
+ * mode.  Pair i uses random-search stream index i, so a batched result equals
+ * bm_flow_pair(ctx_i, ..., pair = i). */
+typedef struct bm_batch bm_batch;
+int bm_batch_create(int n, const bm_params *params, uint64_t seed, int streams, bm_batch **out);
+void bm_batch_destroy(bm_batch *b);
+int bm_batch_ctx(bm_batch *b, int i, bm_ctx **ctx);
+#define BM_BATCH_PER_PAIR 0  /* the pairs' launches one pair after the other on one stream */
+#define BM_BATCH_GRAPH 1     /* all pairs as one hipGraph launch (captured on first use) */
+int bm_batch_flow(bm_batch *b, const uint8_t *const *rgb1_dev, const uint8_t *const *rgb2_dev, float *const *U_dev,
+                  float *const *V_dev, int mode);
+int bm_batch_sync(bm_batch *b);
+int bm_batch_timer_start(bm_batch *b);
+int bm_batch_timer_stop(bm_batch *b, float *ms);
+const char *bm_batch_last_error(const bm_batch *b);
+
 /* ---- timing (CudaTimer, util_cuda.h:131-141) ----------------------------- */
 
 int bm_timer_start(bm_ctx *ctx);

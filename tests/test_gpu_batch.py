@@ -1,0 +1,113 @@
+"""Tuning invariance, the autotuner and batches of uneven pairs on the GPU.
+
+Every valid tuning must give bit-identical fields and flows; a batch (per-pair
+launches or one hipGraph) must give, for pair i, exactly bm_flow_pair(ctx_i,
+pair = i), whose field in turn equals the CPU oracle."""
+import itertools
+
+import numpy as np
+import pytest
+
+import briefmatch_amd as bm
+import oracle
+from briefmatch_amd import synth
+from conftest import say
+
+pytestmark = pytest.mark.gpu
+
+
+def _flow(ctx, pr, pair=0):
+    U, V = ctx.flow_numpy(pr.frame1, pr.frame2, pair=pair)
+    off, cost = ctx.field()
+    return off, cost, U, V
+
+
+@pytest.mark.parametrize("N", [32, 128, 512])
+def test_tuning_does_not_change_results(N):
+    say("tuning invariance", N)
+    pr = synth.make_pair(150, 100, motion="rects", max_motion=8.0, seed=N)
+    p = bm.derive_params(150, 100, 2.9, 2.9, N=N)
+    ctx = bm.Context(p, seed=3)
+    ref = _flow(ctx, pr)
+    t0 = ctx.tuning()
+    px = (1, 2) if N <= 256 else (1,)
+    for lp, strip, near, band in itertools.product(px, (16, 256), (0, 8, 32), (1, t0.tri_band)):
+        ctx.set_tuning(level_px=lp, strip_rows=strip, brief_near=near, tri_band=band)
+        got = _flow(ctx, pr)
+        for a, b in zip(ref, got):
+            assert np.array_equal(a, b), (lp, strip, near, band)
+
+
+def test_tuning_rejects_invalid():
+    p = bm.derive_params(64, 48, 2.0, 2.0, N=512)
+    ctx = bm.Context(p)
+    for kw in ({"level_px": 3}, {"level_px": 2}, {"strip_rows": 8}, {"brief_near": 12}, {"tri_band": 0}):
+        with pytest.raises(bm.BriefMatchError):
+            ctx.set_tuning(**kw)
+
+
+def test_autotune_and_cache(tmp_path):
+    say("autotune")
+    pr = synth.make_pair(320, 240, motion="rects", seed=5)
+    p = bm.derive_params(320, 240, 2.9, 2.9, N=128)
+    ctx = bm.Context(p, seed=1)
+    ref = _flow(ctx, pr)
+    r1, r2 = bm.DeviceBuffer(array=pr.frame1), bm.DeviceBuffer(array=pr.frame2)
+    cache = str(tmp_path / "tune.txt")
+    t = ctx.autotune(r1, r2, cache)
+    assert t.level_px in (1, 2) and 16 <= t.strip_rows <= 4096
+    assert len(open(cache).read().splitlines()) == 1
+    ctx2 = bm.Context(p, seed=1)
+    t2 = ctx2.autotune(r1, r2, cache)                  # served from the cache
+    assert t2.as_dict() == t.as_dict()
+    assert len(open(cache).read().splitlines()) == 1
+    for a, b in zip(ref, _flow(ctx, pr)):
+        assert np.array_equal(a, b)
+
+
+def _batch_workload(n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        W, H = int(rng.integers(64, 200)), int(rng.integers(48, 150))
+        N = int(rng.choice([32, 128]))
+        u = float(rng.uniform(1.5, 2.9))
+        pr = synth.make_pair(W, H, motion="rects", max_motion=6.0, seed=seed * 100 + i)
+        out.append((bm.derive_params(W, H, u, u, N=N), pr))
+    return out
+
+
+def test_batch_equals_per_pair_and_oracle():
+    say("batch")
+    wl = _batch_workload(6, 7)
+    batch = bm.Batch([p for p, _ in wl], seed=11, streams=3)
+    r1 = [bm.DeviceBuffer(array=pr.frame1) for _, pr in wl]
+    r2 = [bm.DeviceBuffer(array=pr.frame2) for _, pr in wl]
+    Us = [bm.DeviceBuffer(p.sx_in * p.sy_in * 4) for p, _ in wl]
+    Vs = [bm.DeviceBuffer(p.sx_in * p.sy_in * 4) for p, _ in wl]
+    results = {}
+    for mode in (bm.BATCH_PER_PAIR, bm.BATCH_GRAPH, bm.BATCH_GRAPH):  # graph captured, then replayed
+        batch.flow(r1, r2, Us, Vs, mode=mode)
+        batch.sync()
+        res = []
+        for i, (p, _) in enumerate(wl):
+            off, cost = batch.ctx[i].field()
+            res.append((off, cost, Us[i].download((p.sy_in, p.sx_in), np.float32),
+                        Vs[i].download((p.sy_in, p.sx_in), np.float32)))
+        results.setdefault(mode, []).append(res)
+    base = results[bm.BATCH_PER_PAIR][0]
+    for res in results[bm.BATCH_GRAPH]:
+        for a, b in zip(base, res):
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y)
+    for i, (p, pr) in enumerate(wl):
+        ctx = bm.Context(p, seed=11)
+        single = _flow(ctx, pr, pair=i)
+        for x, y in zip(base[i], single):
+            assert np.array_equal(x, y), i
+        op = oracle.Params()
+        for n_, _ in op._fields_:
+            setattr(op, n_, getattr(ctx.params, n_))
+        eo, ec, oU, oV = oracle.flow_pair(op, pr.frame1, pr.frame2, seed=11, pair=i)
+        assert np.array_equal(base[i][0], eo) and np.array_equal(base[i][1], ec), i
+        assert np.abs(base[i][2] - oU).max() <= 1e-3 and np.abs(base[i][3] - oV).max() <= 1e-3
