@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(_HERE, "libbriefmatch_amd.so")
 BM_OK, BM_E_PARAM, BM_E_ALLOC, BM_E_LAUNCH, BM_E_IO, BM_E_STATE = range(6)
 # stage ids of bm_profile_read (BM_STAGE_* in briefmatch_amd.h)
 STAGES = ("gray", "upsample", "brief", "init", "level", "level_rs", "random_search", "downsample", "median",
-          "trilateral", "copy")
+          "trilateral", "copy", "proposal")
 H2D, D2H, D2D, H2H = range(4)
 
 
@@ -124,6 +124,25 @@ def lib():
         "bm_batch_timer_start": (I, [V]),
         "bm_batch_timer_stop": (I, [V, C.POINTER(F)]),
         "bm_batch_last_error": (C.c_char_p, [V]),
+        "bm_net_create": (I, [C.POINTER(V)]),
+        "bm_net_destroy": (None, [V]),
+        "bm_net_last_error": (C.c_char_p, [V]),
+        "bm_net_set_weights": (I, [V, V, C.c_size_t]),
+        "bm_net_get_weights": (I, [V, V, C.c_size_t]),
+        "bm_net_load": (I, [V, C.c_char_p]),
+        "bm_net_save": (I, [V, C.c_char_p]),
+        "bm_net_set_optimizer": (I, [V, F, F, F]),
+        "bm_net_forward": (I, [V, V, C.c_uint, C.c_uint, C.c_uint, V]),
+        "bm_net_gradients": (I, [V, V, V, C.c_uint, C.c_uint, C.c_uint, C.POINTER(F), V]),
+        "bm_net_train_step": (I, [V, V, V, C.c_uint, V, C.c_uint, C.c_uint, C.c_uint, F, C.POINTER(F)]),
+        "bm_net_sync": (I, [V]),
+        "bm_proposal_dims": (I, [V, C.POINTER(C.c_uint), C.POINTER(C.c_uint)]),
+        "bm_ctx_frames": (I, [V, V, V]),
+        "bm_proposal_features": (I, [V, V, V, V]),
+        "bm_coarse_flow": (I, [V, V, V, V]),
+        "bm_proposal_offsets": (I, [V, V, V]),
+        "bm_init_proposal": (I, [V, V, V, I, V, V, V, V]),
+        "bm_flow_pair_learned": (I, [V, V, V, V, U32, V, V]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -404,6 +423,36 @@ class Context:
         self._c(lib().bm_timer_stop(self.h, C.byref(ms)))
         return ms.value
 
+    # -- learned proposal ---------------------------------------------------
+    def proposal_dims(self):
+        """(cw, ch): the coarse grid, W / 8 x H / 8."""
+        cw, ch = C.c_uint(), C.c_uint()
+        self._c(lib().bm_proposal_dims(self.h, C.byref(cw), C.byref(ch)))
+        return cw.value, ch.value
+
+    def frames(self):
+        """Device pointers (g8[2], F[2]) of the context's own frame buffers."""
+        g8 = (C.c_void_p * 2)()
+        F = (C.c_void_p * 2)()
+        self._c(lib().bm_ctx_frames(self.h, g8, F))
+        return [C.c_void_p(g8[0]), C.c_void_p(g8[1])], [C.c_void_p(F[0]), C.c_void_p(F[1])]
+
+    def proposal_features(self, g1, g2, feat):
+        self._c(lib().bm_proposal_features(self.h, _ptr(g1), _ptr(g2), _ptr(feat)))
+
+    def coarse_flow(self, U, V, label):
+        self._c(lib().bm_coarse_flow(self.h, _ptr(U), _ptr(V), _ptr(label)))
+
+    def proposal_offsets(self, net_out, coff):
+        self._c(lib().bm_proposal_offsets(self.h, _ptr(net_out), _ptr(coff)))
+
+    def init_proposal(self, F1, F2, coff, off, cost, first=True, prev_off=None):
+        self._c(lib().bm_init_proposal(self.h, _ptr(F1), _ptr(F2), 1 if first else 0, _ptr(prev_off), _ptr(coff),
+                                       _ptr(off), _ptr(cost)))
+
+    def flow_pair_learned(self, net, rgb1, rgb2, U, V, pair=0):
+        self._c(lib().bm_flow_pair_learned(self.h, net.h, _ptr(rgb1), _ptr(rgb2), pair, _ptr(U), _ptr(V)))
+
     # -- convenience ---------------------------------------------------------
     def flow_numpy(self, frame1, frame2, pair=0):
         """Host RGB uint8 frames -> host (U, V) FP32 via bm_flow_pair."""
@@ -411,6 +460,15 @@ class Context:
         r2 = DeviceBuffer(array=np.ascontiguousarray(frame2, np.uint8))
         U, V = self.image_buffer(), self.image_buffer()
         self.flow_pair(r1, r2, U, V, pair=pair)
+        self.sync()
+        return U.download((self.H, self.W), np.float32), V.download((self.H, self.W), np.float32)
+
+    def flow_numpy_learned(self, net, frame1, frame2, pair=0):
+        """Host RGB uint8 frames -> host (U, V) FP32 via bm_flow_pair_learned."""
+        r1 = DeviceBuffer(array=np.ascontiguousarray(frame1, np.uint8))
+        r2 = DeviceBuffer(array=np.ascontiguousarray(frame2, np.uint8))
+        U, V = self.image_buffer(), self.image_buffer()
+        self.flow_pair_learned(net, r1, r2, U, V, pair=pair)
         self.sync()
         return U.download((self.H, self.W), np.float32), V.download((self.H, self.W), np.float32)
 
@@ -477,6 +535,78 @@ class Batch:
             for c in self.ctx:
                 c.h = None
             lib().bm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+NET_FEATURES, NET_OUTPUTS, NET_PARAMS = 128, 16, 193808
+
+
+class Net:
+    """The learned-proposal network (bm_net_*): BF16-on-MFMA conv net trained
+    with Adam on device mini-batches."""
+
+    def __init__(self, weights=None, path=None):
+        h = C.c_void_p()
+        st = lib().bm_net_create(C.byref(h))
+        if st != BM_OK:
+            raise BriefMatchError(st, (lib().bm_net_last_error(None) or b"").decode(errors="replace"))
+        self.h = h
+        if path is not None:
+            self.load(path)
+        elif weights is not None:
+            self.set_weights(weights)
+
+    def _c(self, status):
+        if status != BM_OK:
+            raise BriefMatchError(status, (lib().bm_net_last_error(self.h) or b"").decode(errors="replace"))
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._c(lib().bm_net_set_weights(self.h, _ptr(w), w.size))
+
+    def weights(self):
+        w = np.empty(NET_PARAMS, np.float32)
+        self._c(lib().bm_net_get_weights(self.h, _ptr(w), w.size))
+        return w
+
+    def load(self, path):
+        self._c(lib().bm_net_load(self.h, os.fsencode(path)))
+
+    def save(self, path):
+        self._c(lib().bm_net_save(self.h, os.fsencode(path)))
+
+    def set_optimizer(self, beta1=0.9, beta2=0.999, eps=1e-8):
+        self._c(lib().bm_net_set_optimizer(self.h, beta1, beta2, eps))
+
+    def forward(self, feat, n, ch, cw, out):
+        self._c(lib().bm_net_forward(self.h, _ptr(feat), n, ch, cw, _ptr(out)))
+
+    def gradients(self, feat, label, n, ch, cw):
+        """(loss, gradient vector) of the mean absolute error; no update."""
+        loss = C.c_float()
+        g = np.empty(NET_PARAMS, np.float32)
+        self._c(lib().bm_net_gradients(self.h, _ptr(feat), _ptr(label), n, ch, cw, C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def train_step(self, feat_set, label_set, rows, idx, ch, cw, lr, want_loss=False):
+        idx = np.ascontiguousarray(idx, np.int32)
+        loss = C.c_float()
+        self._c(lib().bm_net_train_step(self.h, _ptr(feat_set), _ptr(label_set), rows, _ptr(idx), idx.size, ch, cw,
+                                        lr, C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def sync(self):
+        self._c(lib().bm_net_sync(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bm_net_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -45,7 +45,7 @@ def _run(cmd, verbose):
 def build_lib(verbose=False):
     objs = []
     os.makedirs(os.path.join(PKG, "_build"), exist_ok=True)
-    for src in ("bm_kernels.hip", "bm_capi.cpp"):
+    for src in ("bm_kernels.hip", "bm_net.hip", "bm_capi.cpp", "bm_net_capi.cpp"):
         obj = os.path.join(PKG, "_build", src + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
         _run([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)

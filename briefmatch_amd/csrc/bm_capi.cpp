@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "bm_internal.h"
+#include "bm_net.h"
 
 namespace {
 
@@ -68,6 +69,12 @@ struct bm_ctx {
     uint16_t *d_cost[2] = {nullptr, nullptr};
     float *d_U0 = nullptr, *d_V0 = nullptr, *d_tU = nullptr, *d_tV = nullptr, *d_Um = nullptr, *d_Vm = nullptr;
     int field = 0;  // d_off[field] holds the last match field
+    // learned proposal (allocated on first use)
+    uint16_t *d_feat = nullptr;            // [ch][cw][128] BF16
+    uint16_t *d_nact[2] = {nullptr, nullptr};  // [ch][cw][64] BF16
+    float *d_nout = nullptr;               // [ch][cw][16]
+    int32_t *d_coff = nullptr;             // [ch][cw][2] Q8
+    int32_t *d_bsum = nullptr;             // [2][ch][cw] block sums
     // sequence mode
     int have_prev = 0, cur = 0, field_valid = 0;
     uint32_t frame_counter = 0;
@@ -874,6 +881,146 @@ int bm_flow_pair(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, uint32_t
     r = run_match(ctx, ctx->d_F[0], ctx->d_F[1], 1, nullptr, pair);
     if (r) return r;
     // an isolated pair breaks any running sequence
+    ctx->have_prev = 0;
+    ctx->field_valid = 0;
+    return run_flow_tail(ctx, ctx->d_g32[0], U, V);
+}
+
+int bm_proposal_dims(const bm_ctx *ctx, unsigned *cw, unsigned *ch)
+{
+    if (!ctx || !cw || !ch) return fail(nullptr, BM_E_PARAM, "null argument");
+    *cw = (unsigned)(ctx->g.W / bm::net::kCell);
+    *ch = (unsigned)(ctx->g.H / bm::net::kCell);
+    return BM_OK;
+}
+
+int bm_ctx_frames(bm_ctx *ctx, uint8_t **g8, uint32_t **F)
+{
+    if (!ctx || !g8 || !F) return fail(nullptr, BM_E_PARAM, "null argument");
+    for (int i = 0; i < 2; ++i) {
+        g8[i] = ctx->d_g8[i];
+        F[i] = ctx->d_F[i];
+    }
+    return BM_OK;
+}
+
+namespace {
+
+int proposal_ready(bm_ctx *ctx)
+{
+    const int cw = ctx->g.W / bm::net::kCell, ch = ctx->g.H / bm::net::kCell;
+    if (cw < 1 || ch < 1) return fail(ctx, BM_E_PARAM, "the learned proposal needs frames of at least 8x8 pixels");
+    if (ctx->d_feat) return BM_OK;
+    const size_t cells = (size_t)cw * ch;
+    int r = dev_alloc(ctx, &ctx->d_feat, cells * bm::net::kFeatC);
+    if (!r) r = dev_alloc(ctx, &ctx->d_nact[0], cells * 64);
+    if (!r) r = dev_alloc(ctx, &ctx->d_nact[1], cells * 64);
+    if (!r) r = dev_alloc(ctx, &ctx->d_nout, cells * bm::net::kOutC);
+    if (!r) r = dev_alloc(ctx, &ctx->d_coff, cells * 2);
+    if (!r) r = dev_alloc(ctx, &ctx->d_bsum, cells * 2);
+    if (r) ctx->d_feat = nullptr;
+    return r;
+}
+
+hipError_t launch_offsets(bm_ctx *ctx, const float *nout, int32_t *coff)
+{
+    const bm::Geometry &g = ctx->g;
+    const int cw = g.W / bm::net::kCell, ch = g.H / bm::net::kCell;
+    const float scx = (float)(256 * bm::net::kCell * g.sx) / (float)g.W;
+    const float scy = (float)(256 * bm::net::kCell * g.sy) / (float)g.H;
+    return bm::net::launch_proposal_offsets(nout, cw * ch, scx, scy, 256 * g.sx, 256 * g.sy, coff, ctx->stream);
+}
+
+}  // namespace
+
+int bm_proposal_features(bm_ctx *ctx, const uint8_t *g1, const uint8_t *g2, uint16_t *feat)
+{
+    int r = check(ctx, g1, "g1");
+    if (!r) r = check(ctx, g2, "g2");
+    if (!r) r = check(ctx, feat, "feat");
+    if (!r) r = proposal_ready(ctx);
+    if (r) return r;
+    const bm::Geometry &g = ctx->g;
+    const int cw = g.W / bm::net::kCell, ch = g.H / bm::net::kCell;
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, bm::net::launch_features(g1, g2, g.W, cw, ch, ctx->d_bsum, feat, ctx->stream));
+    return BM_OK;
+}
+
+int bm_coarse_flow(bm_ctx *ctx, const float *U, const float *V, float *label)
+{
+    int r = check(ctx, U, "U");
+    if (!r) r = check(ctx, V, "V");
+    if (!r) r = check(ctx, label, "label");
+    if (r) return r;
+    const int cw = ctx->g.W / bm::net::kCell, ch = ctx->g.H / bm::net::kCell;
+    if (cw < 1 || ch < 1) return fail(ctx, BM_E_PARAM, "the learned proposal needs frames of at least 8x8 pixels");
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, bm::net::launch_coarse_flow(U, V, ctx->g.W, cw, ch, label, ctx->stream));
+    return BM_OK;
+}
+
+int bm_proposal_offsets(bm_ctx *ctx, const float *nout, int32_t *coff)
+{
+    int r = check(ctx, nout, "net_out");
+    if (!r) r = check(ctx, coff, "coff");
+    if (r) return r;
+    if (ctx->g.W < bm::net::kCell || ctx->g.H < bm::net::kCell)
+        return fail(ctx, BM_E_PARAM, "the learned proposal needs frames of at least 8x8 pixels");
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, launch_offsets(ctx, nout, coff));
+    return BM_OK;
+}
+
+int bm_init_proposal(bm_ctx *ctx, const uint32_t *F1, const uint32_t *F2, int first, const uint32_t *prev,
+                     const int32_t *coff, uint32_t *off, uint16_t *cost)
+{
+    int r = check(ctx, F1, "F1");
+    if (!r) r = check(ctx, F2, "F2");
+    if (!r) r = check(ctx, coff, "coff");
+    if (!r) r = check(ctx, off, "off");
+    if (!r) r = check(ctx, cost, "cost");
+    if (r) return r;
+    if (!first && !prev) return fail(ctx, BM_E_PARAM, "temporal init needs prev_off");
+    if (!first && prev == off) return fail(ctx, BM_E_PARAM, "prev_off and off must differ");
+    const bm::Geometry &g = ctx->g;
+    const int cw = g.W / bm::net::kCell, ch = g.H / bm::net::kCell;
+    if (cw < 1 || ch < 1) return fail(ctx, BM_E_PARAM, "the learned proposal needs frames of at least 8x8 pixels");
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, bm::net::launch_init_proposal(g.NSP, F1, F2, first, prev, coff, cw, ch, g.W, g.H,
+                                                                      g.sx, g.sy, off, cost, ctx->stream));
+    return BM_OK;
+}
+
+int bm_flow_pair_learned(bm_ctx *ctx, bm_net *net, const uint8_t *rgb1, const uint8_t *rgb2, uint32_t pair, float *U,
+                         float *V)
+{
+    int r = check(ctx, net, "net");
+    if (!r) r = check(ctx, rgb1, "rgb1");
+    if (!r) r = check(ctx, rgb2, "rgb2");
+    if (!r) r = check(ctx, U, "U");
+    if (!r) r = check(ctx, V, "V");
+    if (!r) r = proposal_ready(ctx);
+    if (r) return r;
+    const bm::Geometry &g = ctx->g;
+    const int cw = g.W / bm::net::kCell, ch = g.H / bm::net::kCell;
+    BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb1, ctx->d_g8[0], ctx->d_g32[0], ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb2, ctx->d_g8[1], nullptr, ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(g, nullptr, ctx->d_g8[0], ctx->d_F[0], ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(g, nullptr, ctx->d_g8[1], ctx->d_F[1], ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, bm::net::launch_features(ctx->d_g8[0], ctx->d_g8[1], g.W, cw, ch, ctx->d_bsum,
+                                                                 ctx->d_feat, ctx->stream));
+    // the packed weights are written on the net's stream
+    BM_HIP(ctx, hipStreamWaitEvent(ctx->stream, net->ready, 0));
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, bm::net::forward(net->pk, ctx->d_feat, 1, ch, cw, ctx->d_nact[0], ctx->d_nact[1],
+                                                         ctx->d_nout, ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, launch_offsets(ctx, ctx->d_nout, ctx->d_coff));
+    const int dst = ctx->field;
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, bm::net::launch_init_proposal(g.NSP, ctx->d_F[0], ctx->d_F[1], 1, nullptr,
+                                                                      ctx->d_coff, cw, ch, g.W, g.H, g.sx, g.sy,
+                                                                      ctx->d_off[dst], ctx->d_cost[dst], ctx->stream));
+    int pd = -1;
+    for (int i = 0; i < g.iterations; ++i) {
+        r = run_iteration(ctx, ctx->d_F[0], ctx->d_F[1], pd, i, pair, true);
+        if (r) return r;
+        pd = -pd;
+    }
     ctx->have_prev = 0;
     ctx->field_valid = 0;
     return run_flow_tail(ctx, ctx->d_g32[0], U, V);

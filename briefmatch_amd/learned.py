@@ -1,0 +1,240 @@
+"""Learned PatchMatch proposal: dataset, training and evaluation drivers.
+
+The network and its kernels are in csrc/bm_net.hip; this module only drives
+them through the C ABI (briefmatch_amd.Net / Context):
+
+* ``build_dataset`` generates seeded synthetic pairs on the host cores, runs
+  the plain method on each (its output flow, with four propagation
+  iterations and no up-sampling, is the label: the random-search radius in
+  input pixels then reaches as far, in 8-pixel cells, as at 1920x1080 with
+  up-sampling 2.9) and stores, per
+  pair, the proposal feature map and the 8x8 block means of that flow (in
+  coarse pixels) in device memory;
+* ``train`` runs Adam on random mini-batches of that dataset;
+* ``evaluate`` compares, on pairs with known flow, the plain method with its
+  default iterations, the plain method with fewer iterations and the
+  proposal-seeded method with fewer iterations.
+
+    python -m briefmatch_amd.learned train --pairs 20000 --out proposal.bnet
+    python -m briefmatch_amd.learned eval --weights proposal.bnet
+"""
+import argparse
+import ctypes as C
+import math
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import Context, DeviceBuffer, Net, derive_params, epe, NET_PARAMS
+from . import synth
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+# trained on 640x480 pairs labelled by the plain method (128-bit descriptors)
+DEFAULT_WEIGHTS = os.path.join(_HERE, "weights", "proposal_640x480.bnet")
+
+# (cin, cout) per layer, as in the C ABI's parameter layout
+_LAYERS = ((128, 64), (64, 64), (64, 64), (64, 64), (64, 16))
+
+
+def init_weights(seed=0):
+    """He-uniform weights for the ReLU layers, a small uniform last layer with
+    only the two flow channels non-zero, zero biases."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for l, (ci, co) in enumerate(_LAYERS):
+        bound = math.sqrt(6.0 / (9 * ci))
+        w = rng.uniform(-bound, bound, size=(9, co, ci)).astype(np.float32)
+        if l == len(_LAYERS) - 1:
+            w *= 0.1
+            w[:, 2:, :] = 0.0
+        parts += [w.ravel(), np.zeros(co, np.float32)]
+    w = np.concatenate(parts)
+    assert w.size == NET_PARAMS
+    return w
+
+
+def _offset(buf, nbytes):
+    return C.c_void_p(buf.ptr.value + int(nbytes))
+
+
+def train_pair(seed, W=640, H=480):
+    """Training pair `seed`: moving rectangles over an affine background,
+    motion up to 40 px (the family of the 1920x1080 benchmark pair)."""
+    rng = np.random.default_rng([seed, 7])
+    n_rects = int(rng.integers(3, 11))
+    max_motion = float(rng.uniform(8.0, 40.0))
+    p = synth.make_pair(W, H, motion="rects", n_rects=n_rects, max_motion=max_motion, seed=seed)
+    return p.frame1, p.frame2
+
+
+def _gen(args):
+    seed, W, H = args
+    return train_pair(seed, W, H)
+
+
+class Dataset:
+    """Device-resident feature maps [rows][ch][cw][128] BF16 and labels
+    [rows][ch][cw][2] FP32."""
+
+    def __init__(self, rows, ch, cw):
+        self.rows, self.ch, self.cw = rows, ch, cw
+        self.feat_row = ch * cw * 128 * 2
+        self.label_row = ch * cw * 2 * 4
+        self.feat = DeviceBuffer(rows * self.feat_row)
+        self.label = DeviceBuffer(rows * self.label_row)
+
+    def labels(self):
+        return self.label.download((self.rows, self.ch, self.cw, 2), np.float32)
+
+
+def build_dataset(pairs, seed0=1000, W=640, H=480, up=1.0, N=128, iterations=4, workers=16, log=print):
+    """Generate `pairs` seeded pairs (seeds seed0..), label them with the plain
+    method (`iterations` propagation iterations) and store features and labels
+    on the device."""
+    params = derive_params(W, H, up, up, N=N)
+    params.iterations = iterations
+    ctx = Context(params, seed=0)
+    cw, ch = ctx.proposal_dims()
+    ds = Dataset(pairs, ch, cw)
+    r1, r2 = ctx.rgb_buffer(), ctx.rgb_buffer()
+    U, V = ctx.image_buffer(), ctx.image_buffer()
+    g8, F = ctx.frames()
+    t0 = time.time()
+    jobs = [(seed0 + i, W, H) for i in range(pairs)]
+    pool = mp.get_context("spawn").Pool(workers) if workers > 1 else None
+    it = pool.imap(_gen, jobs, chunksize=4) if pool else map(_gen, jobs)
+    try:
+        for i, (f1, f2) in enumerate(it):
+            ctx.sync()
+            r1.upload(f1)
+            r2.upload(f2)
+            ctx.flow_pair(r1, r2, U, V, pair=0)
+            ctx.coarse_flow(U, V, _offset(ds.label, i * ds.label_row))
+            ctx.proposal_features(g8[0], g8[1], _offset(ds.feat, i * ds.feat_row))
+            if log and ((i + 1) % 1000 == 0 or i + 1 == pairs):
+                log("[data] %d/%d pairs, %.1f s" % (i + 1, pairs, time.time() - t0))
+        ctx.sync()
+    finally:
+        if pool:
+            pool.close()
+            pool.join()
+    ctx.close()
+    return ds
+
+
+def train(net, ds, rows, epochs=20, batch=32, lr=2e-3, lr_min=5e-5, seed=0, log=print, log_every=200):
+    """Adam over random mini-batches of rows `rows` of the dataset with a
+    cosine learning-rate decay; returns the logged (step, loss) pairs."""
+    rng = np.random.default_rng(seed)
+    rows = np.asarray(rows, np.int32)
+    steps_per_epoch = len(rows) // batch
+    total = epochs * steps_per_epoch
+    hist = []
+    step = 0
+    t0 = time.time()
+    for ep in range(epochs):
+        perm = rows[rng.permutation(len(rows))]
+        for b in range(steps_per_epoch):
+            a = lr_min + 0.5 * (lr - lr_min) * (1.0 + math.cos(math.pi * step / max(total, 1)))
+            want = (step % log_every == 0) or step == total - 1
+            loss = net.train_step(ds.feat, ds.label, ds.rows, perm[b * batch:(b + 1) * batch], ds.ch, ds.cw, a,
+                                  want_loss=want)
+            if want:
+                hist.append((step, loss))
+                if log:
+                    log("[train] epoch %d step %d/%d lr %.2e loss %.4f (%.1f s)"
+                        % (ep, step, total, a, loss, time.time() - t0))
+            step += 1
+    net.sync()
+    return hist
+
+
+def validation_loss(net, ds, rows, batch=32):
+    """Mean loss over `rows` (no update): the gradient pass with lr 0 would
+    change the Adam state, so the forward output is compared on the host."""
+    rows = list(rows)
+    out = DeviceBuffer(batch * ds.ch * ds.cw * 16 * 4)
+    labels = ds.labels()
+    tot, n = 0.0, 0
+    for b in range(0, len(rows), batch):
+        idx = rows[b:b + batch]
+        for j, r in enumerate(idx):
+            net.forward(_offset(ds.feat, r * ds.feat_row), 1, ds.ch, ds.cw, _offset(out, j * ds.ch * ds.cw * 64))
+        o = out.download((len(idx), ds.ch, ds.cw, 16), np.float32)[..., :2]
+        lab = labels[idx]
+        tot += float(np.abs(o - lab).sum())
+        n += lab.size
+    return tot / max(n, 1)
+
+
+def evaluate(net, workload="c2_1920x1080_u2.9_n128", seeds=(0, 1, 2, 3), fewer=1, log=print):
+    """EPE (all pixels) of plain(default iterations), plain(fewer),
+    learned(fewer) and learned(default iterations) on seeded pairs of a
+    benchmark workload."""
+    wl = synth.WORKLOADS[workload]
+    p = derive_params(wl["W"], wl["H"], wl["up"], wl["up"], N=wl["N"])
+    default_it = p.iterations
+    pf = p.copy()
+    pf.iterations = fewer
+    c_plain = Context(p, seed=0)
+    c_few = Context(pf, seed=0)
+    res = {"plain": [], "plain_fewer": [], "learned_fewer": [], "learned": []}
+    for s in seeds:
+        pair = synth.workload_pair(workload, seed=s)
+        for key, ctx, learned in (("plain", c_plain, False), ("plain_fewer", c_few, False),
+                                  ("learned_fewer", c_few, True), ("learned", c_plain, True)):
+            if learned:
+                U, V = ctx.flow_numpy_learned(net, pair.frame1, pair.frame2)
+            else:
+                U, V = ctx.flow_numpy(pair.frame1, pair.frame2)
+            res[key].append(epe(U, V, pair.gt)[0])
+        if log:
+            log("[eval] seed %d: plain(%d) %.4f  plain(%d) %.4f  learned(%d) %.4f  learned(%d) %.4f"
+                % (s, default_it, res["plain"][-1], fewer, res["plain_fewer"][-1], fewer, res["learned_fewer"][-1],
+                   default_it, res["learned"][-1]))
+    out = {k: float(np.mean(v)) for k, v in res.items()}
+    out.update(iterations=default_it, fewer=fewer, seeds=list(seeds), workload=workload)
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--pairs", type=int, default=20000)
+    t.add_argument("--val", type=int, default=500, help="held-out pairs for the validation loss")
+    t.add_argument("--epochs", type=int, default=20)
+    t.add_argument("--batch", type=int, default=32)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--workers", type=int, default=16)
+    t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--out", required=True)
+    t.add_argument("--eval", action="store_true", help="evaluate on the 1920x1080 workload afterwards")
+    e = sub.add_parser("eval")
+    e.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    e.add_argument("--seeds", type=int, default=4)
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)
+    if a.cmd == "train":
+        ds = build_dataset(a.pairs, workers=a.workers, log=log)
+        net = Net(weights=init_weights(a.seed))
+        rows = np.arange(a.pairs - a.val)
+        val = np.arange(a.pairs - a.val, a.pairs)
+        log("[train] validation loss before: %.4f" % validation_loss(net, ds, val))
+        train(net, ds, rows, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
+        log("[train] validation loss after: %.4f" % validation_loss(net, ds, val))
+        net.save(a.out)
+        log("[train] saved %s" % a.out)
+        if a.eval:
+            log(str(evaluate(net, log=log)))
+    else:
+        net = Net(path=a.weights)
+        log(str(evaluate(net, seeds=tuple(range(a.seeds)), log=log)))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -220,6 +220,76 @@ int bm_batch_timer_start(bm_batch *b);
 int bm_batch_timer_stop(bm_batch *b, float *ms);
 const char *bm_batch_last_error(const bm_batch *b);
 
+/* ---- learned proposal --------------------------------------------------------
+ * No counterpart in the reference: its only initialisations are the zero and
+ * the temporal field (briefmatch_cuda.cu:149-184, the hook these calls extend).
+ *
+ * A convolutional network at 1/8 of the input resolution predicts a coarse
+ * flow from a block-matching cost volume; its offsets seed PatchMatch next to the plain
+ * initialisation, and each pixel keeps the strictly cheaper candidate, so the
+ * plain initialisation stays the fallback.  BF16 weights and activations on
+ * MFMA, FP32 accumulation; trained by bm_net_train_step.
+ *
+ * Feature maps are [n][ch][cw][BM_NET_FEATURES] BF16 (uint16) with
+ * cw = W / 8, ch = H / 8 (bm_proposal_dims); network outputs are
+ * [n][ch][cw][BM_NET_OUTPUTS] FP32, channels 0 and 1 = coarse flow (u, v) in
+ * coarse pixels (8 input pixels); labels are [n][ch][cw][2] FP32 in the same
+ * unit (non-finite = ignored).
+ * The parameter vector is, per layer, W[9][cout][cin] then b[cout]
+ * (layers 128->64, 64->64 dil 2, 64->64 dil 4, 64->64, 64->16). */
+#define BM_NET_FEATURES 128
+#define BM_NET_OUTPUTS 16
+#define BM_NET_PARAMS 193808
+typedef struct bm_net bm_net;
+int bm_net_create(bm_net **out);            /* all weights zero */
+void bm_net_destroy(bm_net *net);           /* waits for the net's work */
+const char *bm_net_last_error(const bm_net *net);
+int bm_net_set_weights(bm_net *net, const float *w_host, size_t count);  /* count = BM_NET_PARAMS; resets Adam */
+int bm_net_get_weights(bm_net *net, float *w_host, size_t count);
+int bm_net_load(bm_net *net, const char *path);  /* "BMNET001", uint32 count, count FP32 (little endian) */
+int bm_net_save(bm_net *net, const char *path);
+/* Adam moments decay rates and epsilon (defaults 0.9, 0.999, 1e-8). */
+int bm_net_set_optimizer(bm_net *net, float beta1, float beta2, float eps);
+/* Inference on the net's stream (synchronous on return). */
+int bm_net_forward(bm_net *net, const uint16_t *feat_dev, unsigned n, unsigned ch, unsigned cw, float *out_dev);
+/* Mean absolute error of channels 0, 1 against the labels and its gradient
+ * with respect to every parameter (host, BM_NET_PARAMS floats); no update. */
+int bm_net_gradients(bm_net *net, const uint16_t *feat_dev, const float *label_dev, unsigned n, unsigned ch,
+                     unsigned cw, float *loss, float *grad_host);
+/* One Adam step on the mini-batch rows idx_host[0..batch) (batch <= 256) of a
+ * dataset of `rows` feature maps and labels (device, row = one pair).  *loss
+ * (if not NULL) receives the batch loss before the update and makes the call
+ * wait.  Contexts must not run bm_flow_pair_learned with a net in training. */
+int bm_net_train_step(bm_net *net, const uint16_t *feat_set_dev, const float *label_set_dev, unsigned rows,
+                      const int *idx_host, unsigned batch, unsigned ch, unsigned cw, float lr, float *loss);
+int bm_net_sync(bm_net *net);
+
+/* Coarse grid of a context: cw = W / 8, ch = H / 8 (both >= 1). */
+int bm_proposal_dims(const bm_ctx *ctx, unsigned *cw, unsigned *ch);
+/* The context's own gray frames and descriptors of the last bm_flow_pair /
+ * bm_flow_pair_learned (g8[i], F[i] for frame i + 1). */
+int bm_ctx_frames(bm_ctx *ctx, uint8_t **g8_dev, uint32_t **F_dev);
+/* Feature map of a pair from its gray frames: channels 0..120 the cost volume
+ * over displacements (dx, dy) in -5..5 coarse pixels, channel (dy + 5) * 11 +
+ * dx + 5 (sum of absolute differences of 8x8 block sums over a 3x3 block
+ * window, clamp-to-edge, min(SAD, 36864) / 18432 - 1); 121, 122 the block
+ * means of both frames / 255 - 1/2; 123, 124 the lowest-cost displacement
+ * (lowest channel on ties) / 5; 125 its scaled cost; 126, 127 zero. */
+int bm_proposal_features(bm_ctx *ctx, const uint8_t *g1_dev, const uint8_t *g2_dev, uint16_t *feat_dev);
+/* 8x8 block means of a flow field divided by 8 (training labels), [ch][cw][2]. */
+int bm_coarse_flow(bm_ctx *ctx, const float *U_dev, const float *V_dev, float *label_dev);
+/* Network output -> per-cell matched-pixel offsets in Q8, [ch][cw][2] int32:
+ * rint(clamp(u * (float)(2048 sx) / (float)W, +-256 sx)) and likewise for v. */
+int bm_proposal_offsets(bm_ctx *ctx, const float *net_out_dev, int32_t *coff_dev);
+/* Proposal-seeded initialisation (the plain bm_init candidate first, then the
+ * bilinear and the four nearest cell offsets; strictly cheaper wins). */
+int bm_init_proposal(bm_ctx *ctx, const uint32_t *F1_dev, const uint32_t *F2_dev, int first,
+                     const uint32_t *prev_off_dev, const int32_t *coff_dev, uint32_t *off_dev, uint16_t *cost_dev);
+/* bm_flow_pair with the proposal-seeded initialisation, then the context's
+ * `iterations` propagation iterations. */
+int bm_flow_pair_learned(bm_ctx *ctx, bm_net *net, const uint8_t *rgb1_dev, const uint8_t *rgb2_dev, uint32_t pair,
+                         float *U_dev, float *V_dev);
+
 /* ---- timing (CudaTimer, util_cuda.h:131-141) ----------------------------- */
 
 int bm_timer_start(bm_ctx *ctx);
@@ -242,7 +312,8 @@ int bm_timer_stop(bm_ctx *ctx, float *ms);
 #define BM_STAGE_MEDIAN 8
 #define BM_STAGE_TRILATERAL 9
 #define BM_STAGE_COPY 10
-#define BM_STAGE_COUNT 11
+#define BM_STAGE_PROPOSAL 11      /* learned proposal: features, network, seeded init */
+#define BM_STAGE_COUNT 12
 int bm_profile_enable(bm_ctx *ctx, int on);
 int bm_profile_read(bm_ctx *ctx, double *ms, unsigned *launches);
 

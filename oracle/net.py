@@ -1,0 +1,298 @@
+"""CPU oracle of the learned proposal -- TEST INFRASTRUCTURE ONLY.
+
+numpy restatement of the learned-proposal path: the feature map (a block
+matching cost volume at 1/8 resolution), the coarse
+labels, the network (forward and backward, float64, optionally with the BF16
+rounding points of the device kernels), the conversion of its output to
+matched-pixel offsets and the proposal-seeded initialisation.  Tests and the
+smoke check use it as the checker; the product never imports it.
+
+The reference has no learned component.  The seeded initialisation extends the
+reference's initialisation (kernels/briefmatch_cuda.cu:149-184: zero offset on
+the first frame, else the previous field followed once); its plain candidate
+is that rule, as in oracle_bm.cpp's orc_init.
+
+Layouts: descriptors [sy, sx, K] uint32 (K >= ceil(N / 32), extra words zero),
+gray frames [H, W] uint8, feature maps [n, ch, cw, 128] as BF16 bit patterns
+(uint16), network outputs [n, ch, cw, 16] float, labels [n, ch, cw, 2]; flows
+at the coarse grid are in coarse pixels (8 input pixels).
+"""
+import numpy as np
+
+RADIUS = 5
+COST = (2 * RADIUS + 1) ** 2
+FEAT_C = 128
+OUT_C = 16
+CELL = 8
+# (cin, cout, dilation, relu) per layer
+LAYERS = ((128, 64, 1, True), (64, 64, 2, True), (64, 64, 4, True), (64, 64, 1, True), (64, 16, 1, False))
+N_PARAMS = sum(9 * co * ci + co for ci, co, _, _ in LAYERS)
+
+
+# ---------------------------------------------------------------------------
+# BF16
+
+def bf16_bits(x):
+    """float32 -> BF16 bit patterns, round to nearest even."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def bf16_float(bits):
+    return (np.asarray(bits, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def bf16_round(x):
+    """Round float values to the nearest BF16 value (returned as float64)."""
+    return bf16_float(bf16_bits(np.asarray(x, np.float32))).astype(np.float64)
+
+
+# ---------------------------------------------------------------------------
+# features, labels
+
+def _popcount_rows(a):
+    return np.bitwise_count(a).sum(axis=-1, dtype=np.int64)
+
+
+SAD_CLAMP = 9 * 64 * 64
+
+
+def block_sums(g):
+    """8x8 block sums [ch, cw] of a gray frame (remainder rows / columns dropped)."""
+    H, W = g.shape
+    cw, ch = W // CELL, H // CELL
+    return g[:ch * CELL, :cw * CELL].astype(np.int64).reshape(ch, CELL, cw, CELL).sum(axis=(1, 3))
+
+
+def features(g1, g2):
+    """Feature map [ch, cw, 128] (BF16 bits) of one pair of gray frames.
+
+    Channel (dy + 5) * 11 + dx + 5 for dx, dy in -5..5: the sum of absolute
+    differences between the 8x8 block sums of frame 1 over the 3x3 block
+    window of the coarse pixel and those of frame 2 displaced by (dx, dy)
+    blocks (coordinates clamped to the grid), as min(SAD, 36864) / 18432 - 1.
+    Then the block means of both frames / 255 - 1/2, the lowest-SAD
+    displacement (first channel on ties) as dx / 5, dy / 5, its scaled SAD,
+    and zeros."""
+    S1, S2 = block_sums(g1), block_sums(g2)
+    ch, cw = S1.shape
+    out = np.zeros((ch, cw, FEAT_C), np.float32)
+    costs = np.zeros((ch, cw, COST), np.int64)
+    yy, xx = np.mgrid[0:ch, 0:cw]
+    half = SAD_CLAMP // 2
+    for c in range(COST):
+        dx, dy = c % (2 * RADIUS + 1) - RADIUS, c // (2 * RADIUS + 1) - RADIUS
+        cost = np.zeros((ch, cw), np.int64)
+        for j in (-1, 0, 1):
+            for i in (-1, 0, 1):
+                a = S1[np.clip(yy + j, 0, ch - 1), np.clip(xx + i, 0, cw - 1)]
+                b = S2[np.clip(yy + j + dy, 0, ch - 1), np.clip(xx + i + dx, 0, cw - 1)]
+                cost += np.abs(a - b)
+        costs[..., c] = cost
+        out[..., c] = (np.minimum(cost, SAD_CLAMP) - half).astype(np.float32) / np.float32(half)
+    out[..., COST] = (S1 - 8160).astype(np.float32) / np.float32(16320)
+    out[..., COST + 1] = (S2 - 8160).astype(np.float32) / np.float32(16320)
+    bc = np.argmin(costs, axis=-1)
+    bcost = np.take_along_axis(costs, bc[..., None], -1)[..., 0]
+    out[..., COST + 2] = (bc % (2 * RADIUS + 1) - RADIUS).astype(np.float32) / np.float32(RADIUS)
+    out[..., COST + 3] = (bc // (2 * RADIUS + 1) - RADIUS).astype(np.float32) / np.float32(RADIUS)
+    out[..., COST + 4] = (np.minimum(bcost, SAD_CLAMP) - half).astype(np.float32) / np.float32(half)
+    return bf16_bits(out)
+
+
+def coarse_flow(U, V):
+    """8x8 block means divided by 8 (coarse pixels) [ch, cw, 2] in float32,
+    summed row by row in order."""
+    H, W = U.shape
+    cw, ch = W // CELL, H // CELL
+    su = np.zeros((ch, cw), np.float32)
+    sv = np.zeros((ch, cw), np.float32)
+    for yy in range(CELL):
+        for xx in range(CELL):
+            su = su + U[yy:ch * CELL:CELL, xx:cw * CELL:CELL][:ch, :cw]
+            sv = sv + V[yy:ch * CELL:CELL, xx:cw * CELL:CELL][:ch, :cw]
+    return np.stack([su * np.float32(1 / 512), sv * np.float32(1 / 512)], -1)
+
+
+# ---------------------------------------------------------------------------
+# proposal offsets and the seeded initialisation
+
+def proposal_offsets(net_out, W, H, sx, sy):
+    """Network output [.., 16] (coarse pixels) -> Q8 matched-pixel offsets
+    [.., 2] int32: rint(clamp(u * 2048 sx / W, +-256 sx))."""
+    out = np.asarray(net_out, np.float32)
+    res = []
+    for c, (s, d) in enumerate(((sx, W), (sy, H))):
+        scale = np.float32(256 * CELL * s) / np.float32(d)
+        lim = np.float32(256 * s)
+        v = out[..., c] * scale
+        v = np.fmin(np.fmax(v, -lim), lim)
+        res.append(np.rint(v).astype(np.int32))
+    return np.stack(res, -1)
+
+
+def _unpack(o):
+    o = np.asarray(o, np.uint32)
+    return ((o & 0xFFFF).astype(np.uint16).view(np.int16).astype(np.int64),
+            (o >> 16).astype(np.uint16).view(np.int16).astype(np.int64))
+
+
+def _pack(dx, dy):
+    return ((np.asarray(dx, np.int64) & 0xFFFF) | ((np.asarray(dy, np.int64) & 0xFFFF) << 16)).astype(np.uint32)
+
+
+def init_proposal(F1, F2, coff, W, H, first=True, prev_off=None):
+    """Proposal-seeded initialisation; returns (off [sy, sx] uint32, cost uint16).
+
+    Candidates in order: the plain initialisation (zero, or prev followed once
+    with clamping, briefmatch_cuda.cu:149-184), the bilinear interpolation of
+    the Q8 offsets of the four cells around the pixel (coarse coordinate
+    x * W * 32 / sx - 112 in Q8, clamped to the grid; sum rounded half up at
+    2^-24), then the four cells' offsets rounded half up.  Skip out-of-frame
+    candidates and candidates equal to the running best; take strictly lower
+    costs."""
+    sy, sx = F1.shape[:2]
+    ch, cw = coff.shape[:2]
+    yy, xx = np.mgrid[0:sy, 0:sx].astype(np.int64)
+    if first:
+        nx = np.zeros((sy, sx), np.int64)
+        ny = np.zeros((sy, sx), np.int64)
+    else:
+        dx0, dy0 = _unpack(prev_off)
+        dx0 = np.clip(dx0, -xx, sx - 1 - xx)
+        dy0 = np.clip(dy0, -yy, sy - 1 - yy)
+        dx1, dy1 = _unpack(np.asarray(prev_off)[yy + dy0, xx + dx0])
+        nx = np.clip(dx1, -xx, sx - 1 - xx)
+        ny = np.clip(dy1, -yy, sy - 1 - yy)
+    best = _popcount_rows(F1 ^ F2[yy + ny, xx + nx])
+    bo = _pack(nx, ny)
+
+    qx = np.clip(xx * W * 32 // sx - 112, 0, (cw - 1) * 256)
+    qy = np.clip(yy * H * 32 // sy - 112, 0, (ch - 1) * 256)
+    x0, y0 = qx >> 8, qy >> 8
+    x1, y1 = np.minimum(x0 + 1, cw - 1), np.minimum(y0 + 1, ch - 1)
+    ax, ay = qx & 255, qy & 255
+    co = np.asarray(coff, np.int64)
+    cands = []
+    comp = []
+    for d in range(2):
+        o00, o01, o10, o11 = co[y0, x0, d], co[y0, x1, d], co[y1, x0, d], co[y1, x1, d]
+        v = (o00 * (256 - ax) + o01 * ax) * (256 - ay) + (o10 * (256 - ax) + o11 * ax) * ay
+        comp.append([(v + (1 << 23)) >> 24, (o00 + 128) >> 8, (o01 + 128) >> 8, (o10 + 128) >> 8, (o11 + 128) >> 8])
+    for k in range(5):
+        cands.append((comp[0][k], comp[1][k]))
+    for cx, cy in cands:
+        inside = (xx + cx >= 0) & (xx + cx < sx) & (yy + cy >= 0) & (yy + cy < sy)
+        po = _pack(cx, cy)
+        tx = np.where(inside, xx + cx, xx)
+        ty = np.where(inside, yy + cy, yy)
+        c = _popcount_rows(F1 ^ F2[ty, tx])
+        take = inside & (po != bo) & (c < best)
+        best = np.where(take, c, best)
+        bo = np.where(take, po, bo)
+    return bo.astype(np.uint32), best.astype(np.uint16)
+
+
+# ---------------------------------------------------------------------------
+# the network
+
+def split_params(w):
+    """Flat parameter vector -> [(W [9, cout, cin], b [cout]) per layer]."""
+    w = np.asarray(w)
+    out, o = [], 0
+    for ci, co, _, _ in LAYERS:
+        Wl = w[o:o + 9 * co * ci].reshape(9, co, ci)
+        o += 9 * co * ci
+        out.append((Wl, w[o:o + co]))
+        o += co
+    assert o == N_PARAMS
+    return out
+
+
+def _shift(x, oy, ox):
+    """y[:, r, c] = x[:, r + oy, c + ox] (zero outside)."""
+    n, H, W, C = x.shape
+    y = np.zeros_like(x)
+    r0, r1 = max(0, -oy), min(H, H - oy)
+    c0, c1 = max(0, -ox), min(W, W - ox)
+    if r0 < r1 and c0 < c1:
+        y[:, r0:r1, c0:c1] = x[:, r0 + oy:r1 + oy, c0 + ox:c1 + ox]
+    return y
+
+
+def conv(x, Wl, b, dil):
+    """3x3 conv, zero padding: y[p, co] = b[co] + sum_tap,ci W[tap, co, ci] x[p + off(tap), ci],
+    off(tap) = ((tap // 3 - 1) dil, (tap % 3 - 1) dil)."""
+    y = np.zeros(x.shape[:3] + (Wl.shape[1],), np.float64)
+    for tap in range(9):
+        y += _shift(x, (tap // 3 - 1) * dil, (tap % 3 - 1) * dil) @ Wl[tap].T
+    return y + b
+
+
+def conv_data_grad(dz, Wl, dil):
+    """Gradient of conv with respect to its input."""
+    dx = np.zeros(dz.shape[:3] + (Wl.shape[2],), np.float64)
+    for tap in range(9):
+        dx += _shift(dz @ Wl[tap], -(tap // 3 - 1) * dil, -(tap % 3 - 1) * dil)
+    return dx
+
+
+def conv_weight_grad(dz, x, dil):
+    """Gradient of conv with respect to W [9, cout, cin] and b."""
+    co, ci = dz.shape[-1], x.shape[-1]
+    gW = np.zeros((9, co, ci), np.float64)
+    dzf = dz.reshape(-1, co)
+    for tap in range(9):
+        xs = _shift(x, (tap // 3 - 1) * dil, (tap % 3 - 1) * dil).reshape(-1, ci)
+        gW[tap] = dzf.T @ xs
+    return gW, dzf.sum(axis=0)
+
+
+def forward(w, feat, quant=True):
+    """Network output [n, ch, cw, 16] (float64) and the activations.
+
+    feat: BF16 bits [n, ch, cw, 128] or floats.  With quant, weights and the
+    ReLU activations are rounded to BF16 where the device kernels store them."""
+    x = bf16_float(feat).astype(np.float64) if np.asarray(feat).dtype == np.uint16 else np.asarray(feat, np.float64)
+    acts = [x]
+    for l, ((Wl, b), (_, _, dil, relu)) in enumerate(zip(split_params(w), LAYERS)):
+        if quant:
+            Wl = bf16_round(Wl)
+        z = conv(acts[-1], Wl, np.asarray(b, np.float64), dil)
+        if relu:
+            z = np.maximum(z, 0.0)
+            if quant:
+                z = bf16_round(z)
+        acts.append(z)
+    return acts[-1], acts
+
+
+def loss_and_grad(w, feat, label, quant=True):
+    """Mean absolute error of output channels 0, 1 against label (non-finite
+    labels contribute nothing; the mean is over 2 x all pixels) and its
+    gradient with respect to the flat parameter vector."""
+    out, acts = forward(w, feat, quant)
+    label = np.asarray(label, np.float64)
+    npix = int(np.prod(out.shape[:3]))
+    ok = np.isfinite(label)
+    d = np.where(ok, out[..., :2] - np.where(ok, label, 0.0), 0.0)
+    loss = np.abs(d).sum() / (2.0 * npix)
+    inv = float(np.float32(1.0) / (np.float32(2.0) * np.float32(npix))) if quant else 1.0 / (2.0 * npix)
+    dz = np.zeros(out.shape, np.float64)
+    dz[..., :2] = np.sign(d) * inv
+    if quant:
+        dz = bf16_round(dz)
+    params = split_params(w)
+    grads = [None] * len(LAYERS)
+    for l in range(len(LAYERS) - 1, -1, -1):
+        Wl, _ = params[l]
+        dil = LAYERS[l][2]
+        gW, gb = conv_weight_grad(dz, acts[l], dil)
+        grads[l] = (gW, gb)
+        if l > 0:
+            Wq = bf16_round(Wl) if quant else np.asarray(Wl, np.float64)
+            dx = conv_data_grad(dz, Wq, dil) * (acts[l] > 0)
+            dz = bf16_round(dx) if quant else dx
+    flat = np.concatenate([np.concatenate([g.ravel(), b.ravel()]) for g, b in grads])
+    return loss, flat

@@ -1,0 +1,225 @@
+"""Learned proposal on the GPU, through the C ABI.
+
+Integer parts (feature map, labels, offsets, seeded initialisation, the match
+field of the learned path) agree with the oracle bit for bit.  The network's
+forward output and gradients are compared with the oracle's float64 network
+evaluated with the same BF16 rounding points: the device accumulates in FP32
+in another order, so they agree to a relative 2e-2 per layer (gradients) and
+2e-2 of the output range (forward).  Training lowers the loss and is
+bit-reproducible.  With the committed weights, the learned path with one
+iteration reaches the plain method's two-iteration EPE within 2% and beats
+the plain method with one iteration."""
+import os
+
+import numpy as np
+import pytest
+
+import briefmatch_amd as bm
+import oracle
+from briefmatch_amd import learned, synth
+from conftest import say
+from oracle import net as onet
+
+pytestmark = pytest.mark.gpu
+
+FLOW_TOL = 1e-3
+
+
+def _flow_ctx(W, H, up, N, seed=3, **kw):
+    p = bm.derive_params(W, H, up, up, N=N, **kw)
+    ctx = bm.Context(p, seed=seed)
+    pr = synth.make_pair(W, H, motion="rects", max_motion=min(W, H) / 10.0, seed=seed)
+    r1 = bm.DeviceBuffer(array=pr.frame1)
+    r2 = bm.DeviceBuffer(array=pr.frame2)
+    U, V = ctx.image_buffer(), ctx.image_buffer()
+    ctx.flow_pair(r1, r2, U, V)
+    ctx.sync()
+    return ctx, pr, (r1, r2, U, V)
+
+
+def _frames(ctx):
+    g8, F = ctx.frames()
+    Fs = [bm._download_raw(f, (ctx.sy, ctx.sx, ctx.NSP), np.uint32) for f in F]
+    gs = [bm._download_raw(g, (ctx.H, ctx.W), np.uint8) for g in g8]
+    return g8, F, Fs, gs
+
+
+@pytest.mark.parametrize("W,H,up,N", [(320, 240, 1.5, 32), (200, 136, 2.9, 128), (130, 75, 1.0, 512),
+                                      (97, 43, 2.2, 64)])
+def test_features_and_labels_bit_exact(W, H, up, N):
+    say("features", W, H, up, N)
+    ctx, pr, (r1, r2, U, V) = _flow_ctx(W, H, up, N)
+    cw, ch = ctx.proposal_dims()
+    assert (cw, ch) == (W // 8, H // 8)
+    g8, F, Fs, gs = _frames(ctx)
+    feat = bm.DeviceBuffer(ch * cw * 128 * 2)
+    lab = bm.DeviceBuffer(ch * cw * 2 * 4)
+    ctx.proposal_features(g8[0], g8[1], feat)
+    ctx.coarse_flow(U, V, lab)
+    ctx.sync()
+    want = onet.features(gs[0], gs[1])
+    assert np.array_equal(feat.download((ch, cw, 128), np.uint16), want)
+    Uh, Vh = U.download((H, W), np.float32), V.download((H, W), np.float32)
+    assert np.array_equal(lab.download((ch, cw, 2), np.float32), onet.coarse_flow(Uh, Vh))
+
+
+@pytest.mark.parametrize("W,H,up,N", [(320, 240, 1.5, 32), (200, 136, 2.9, 128), (97, 43, 2.2, 512)])
+def test_offsets_and_seeded_init_bit_exact(W, H, up, N):
+    say("seeded init", W, H, up, N)
+    ctx, pr, _ = _flow_ctx(W, H, up, N, seed=5)
+    cw, ch = ctx.proposal_dims()
+    g8, F, Fs, gs = _frames(ctx)
+    rng = np.random.default_rng(W + N)
+    out = (rng.standard_normal((ch, cw, 16)) * 12.0).astype(np.float32)
+    out[0, 0, 0] = np.nan
+    out[-1, -1, 1] = 1e30
+    dout = bm.DeviceBuffer(array=out)
+    coff = bm.DeviceBuffer(ch * cw * 2 * 4)
+    ctx.proposal_offsets(dout, coff)
+    ctx.sync()
+    got_coff = coff.download((ch, cw, 2), np.int32)
+    want_coff = onet.proposal_offsets(out, W, H, ctx.sx, ctx.sy)
+    assert np.array_equal(got_coff, want_coff)
+    M = ctx.sx * ctx.sy
+    dx = rng.integers(-9, 10, (ctx.sy, ctx.sx))
+    dy = rng.integers(-9, 10, (ctx.sy, ctx.sx))
+    prev = ((dx & 0xFFFF) | ((dy & 0xFFFF) << 16)).astype(np.uint32)
+    dprev = bm.DeviceBuffer(array=prev)
+    off = bm.DeviceBuffer(M * 4)
+    cost = bm.DeviceBuffer(M * 2)
+    for first in (True, False):
+        ctx.init_proposal(F[0], F[1], coff, off, cost, first=first, prev_off=None if first else dprev)
+        ctx.sync()
+        eo, ec = onet.init_proposal(Fs[0], Fs[1], want_coff, W, H, first=first, prev_off=prev)
+        assert np.array_equal(off.download((ctx.sy, ctx.sx), np.uint32), eo)
+        assert np.array_equal(cost.download((ctx.sy, ctx.sx), np.uint16), ec)
+
+
+def _random_weights(seed):
+    rng = np.random.default_rng(seed)
+    w = learned.init_weights(seed)
+    w[-16:-14] = rng.standard_normal(2).astype(np.float32)
+    # non-zero biases everywhere so that the bias gradients are exercised
+    for (W_, b), (ci, co, _, _) in zip(onet.split_params(w), onet.LAYERS):
+        b[:] = rng.standard_normal(co).astype(np.float32) * 0.05
+    return w
+
+
+def _random_feat(rng, n, ch, cw):
+    f = rng.uniform(-1, 1, (n, ch, cw, 128)).astype(np.float32)
+    f[..., 123:] = 0.0
+    return onet.bf16_bits(f)
+
+
+# 4480, 65536 and 262144 pixels: one, two and four pixel tiles per wave
+@pytest.mark.parametrize("n,ch,cw", [(1, 9, 13), (3, 15, 20), (1, 64, 70), (1, 256, 256), (4, 256, 256)])
+def test_forward_matches_oracle(n, ch, cw):
+    say("net forward", n, ch, cw)
+    rng = np.random.default_rng(ch)
+    w = _random_weights(ch)
+    feat = _random_feat(rng, n, ch, cw)
+    net = bm.Net(weights=w)
+    assert np.array_equal(net.weights(), w)
+    dfeat = bm.DeviceBuffer(array=feat)
+    dout = bm.DeviceBuffer(n * ch * cw * 16 * 4)
+    net.forward(dfeat, n, ch, cw, dout)
+    got = dout.download((n, ch, cw, 16), np.float32)
+    want, _ = onet.forward(w, feat, quant=True)
+    scale = np.abs(want[..., :2]).max()
+    err = np.abs(got[..., :2] - want[..., :2])
+    print("[net] forward max err %.3e of range %.3e, mean %.3e" % (err.max(), scale, err.mean()), flush=True)
+    assert err.max() <= 2e-2 * scale
+    assert err.mean() <= 2e-3 * scale
+    assert not got[..., 2:].any() or np.abs(got[..., 2:] - want[..., 2:]).max() <= 2e-2 * scale
+
+
+@pytest.mark.parametrize("n,ch,cw", [(1, 9, 13), (2, 30, 40), (2, 160, 220)])
+def test_gradients_match_oracle(n, ch, cw):
+    say("net gradients", n, ch, cw)
+    rng = np.random.default_rng(100 + ch)
+    w = _random_weights(7 + ch)
+    feat = _random_feat(rng, n, ch, cw)
+    label = (rng.standard_normal((n, ch, cw, 2)) * 3.0).astype(np.float32)
+    label[0, 0, 0, 0] = np.nan
+    net = bm.Net(weights=w)
+    loss, g = net.gradients(bm.DeviceBuffer(array=feat), bm.DeviceBuffer(array=label), n, ch, cw)
+    wl, wg = onet.loss_and_grad(w, feat, label, quant=True)
+    print("[net] loss gpu %.6f oracle %.6f" % (loss, wl), flush=True)
+    assert abs(loss - wl) <= 1e-4 * abs(wl)
+    for l, ((gW, gb), (eW, eb)) in enumerate(zip(onet.split_params(g), onet.split_params(wg))):
+        for name, a, b in (("W", gW, eW), ("b", gb, eb)):
+            if l == len(onet.LAYERS) - 1:
+                assert not a[2:].any() if name == "b" else not a[:, 2:].any()
+            rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+            print("[net] layer %d %s rel err %.2e" % (l, name, rel), flush=True)
+            assert rel <= 2e-2, (l, name, rel)
+
+
+def _tiny_dataset(pairs=24, W=160, H=120):
+    return learned.build_dataset(pairs, seed0=77, W=W, H=H, up=1.0, N=64, workers=1, log=None)
+
+
+def test_training_lowers_loss_and_is_reproducible():
+    say("training")
+    ds = _tiny_dataset()
+    w0 = learned.init_weights(3)
+    nets = [bm.Net(weights=w0), bm.Net(weights=w0)]
+    before, _ = nets[0].gradients(ds.feat, ds.label, ds.rows, ds.ch, ds.cw)
+    for net in nets:
+        learned.train(net, ds, np.arange(ds.rows), epochs=40, batch=8, lr=3e-3, seed=1, log=None)
+    after, _ = nets[0].gradients(ds.feat, ds.label, ds.rows, ds.ch, ds.cw)
+    print("[train] loss over the set %.4f -> %.4f" % (before, after), flush=True)
+    assert after < 0.8 * before
+    assert np.array_equal(nets[0].weights(), nets[1].weights())
+
+
+def test_learned_path_equals_composed_oracle():
+    """bm_flow_pair_learned = features -> network -> offsets -> seeded init ->
+    propagation + random search -> refinement, each stage on the oracle except
+    the network (its device output is the oracle's input)."""
+    say("learned path c1")
+    W, H, up, N = 320, 240, 1.5, 32
+    p = bm.derive_params(W, H, up, up, N=N)
+    p.iterations = 1
+    ctx = bm.Context(p, seed=11)
+    path = learned.DEFAULT_WEIGHTS
+    net = bm.Net(path=path) if os.path.exists(path) else bm.Net(weights=_random_weights(1))
+    pr = synth.workload_pair("c1_320x240_u1.5_n32", seed=2)
+    r1 = bm.DeviceBuffer(array=pr.frame1)
+    r2 = bm.DeviceBuffer(array=pr.frame2)
+    U, V = ctx.image_buffer(), ctx.image_buffer()
+    ctx.flow_pair_learned(net, r1, r2, U, V, pair=6)
+    ctx.sync()
+    off, cost = ctx.field()
+    g8, F, Fs, gs = _frames(ctx)
+    cw, ch = ctx.proposal_dims()
+    feat = bm.DeviceBuffer(ch * cw * 256)
+    ctx.proposal_features(g8[0], g8[1], feat)
+    ctx.sync()
+    nout = bm.DeviceBuffer(ch * cw * 64)
+    net.forward(feat, 1, ch, cw, nout)
+    out = nout.download((ch, cw, 16), np.float32)
+    coff = onet.proposal_offsets(out, W, H, ctx.sx, ctx.sy)
+    oF1, oF2 = Fs[0][..., :ctx.NS].copy(), Fs[1][..., :ctx.NS].copy()
+    eo, ec = onet.init_proposal(oF1, oF2, coff, W, H)
+    eo, ec = oracle.propagate(oF1, oF2, -1, p.jfMax, True, eo, ec)
+    eo, ec = oracle.random_search(oF1, oF2, 0, 6, 11, eo, ec)
+    assert np.array_equal(off, eo) and np.array_equal(cost, ec)
+    oU, oV = oracle.downsample(eo, W, H, ctx.gauss_kernel())
+    oUm = oracle.median1d(oracle.median1d(oU, 1, p.fsize), 0, p.fsize)
+    oVm = oracle.median1d(oracle.median1d(oV, 1, p.fsize), 0, p.fsize)
+    tU, tV = oracle.trilateral(oU, oV, oUm, oVm, gs[0].astype(np.float32), p.sEPE ** 2, p.sI ** 2, p.sS ** 2)
+    assert np.abs(U.download((H, W), np.float32) - tU).max() <= FLOW_TOL
+    assert np.abs(V.download((H, W), np.float32) - tV).max() <= FLOW_TOL
+
+
+@pytest.mark.skipif(not os.path.exists(learned.DEFAULT_WEIGHTS), reason="no trained proposal weights")
+def test_trained_proposal_gain():
+    """1920x1080 / u2.9 / 128-bit pairs: the learned path with one iteration
+    reaches the plain two-iteration EPE within 2% and improves on the plain
+    method with one iteration."""
+    say("learned gain c2")
+    net = bm.Net(path=learned.DEFAULT_WEIGHTS)
+    r = learned.evaluate(net, seeds=(0, 1), fewer=1, log=lambda m: print(m, flush=True))
+    assert r["learned_fewer"] <= 1.02 * r["plain"]
+    assert r["learned_fewer"] < r["plain_fewer"]

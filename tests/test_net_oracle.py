@@ -1,0 +1,216 @@
+"""Pins the learned-proposal oracle (oracle/net.py) on the CPU: BF16 rounding
+against exact nearest-value search, the conv against its textbook definition,
+the data and weight gradients against the adjoint identities, the whole
+backward pass against central finite differences, and the feature map,
+labels, offsets and seeded initialisation against element-by-element loops."""
+import numpy as np
+import pytest
+
+from oracle import net as onet
+
+
+def test_bf16_known_values_and_ties():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 0.0, 1.0 + 2 ** -8 + 2 ** -20], np.float32)
+    bits = onet.bf16_bits(x)
+    assert bits[0] == 0x3F80
+    assert bits[1] == 0x3F80          # tie -> even
+    assert bits[2] == 0x3F82          # tie -> even (up)
+    assert bits[3] == 0xC020
+    assert bits[4] == 0
+    assert bits[5] == 0x3F81          # just above the tie -> up
+
+
+def test_bf16_is_nearest():
+    rng = np.random.default_rng(1)
+    x = (rng.standard_normal(20000) * 10.0 ** rng.uniform(-6, 6, 20000)).astype(np.float32)
+    r = onet.bf16_float(onet.bf16_bits(x)).astype(np.float64)
+    # the two BF16 neighbours of x: truncation and one step away from zero
+    t = (x.view(np.uint32) & 0xFFFF0000).view(np.float32).astype(np.float64)
+    step = ((x.view(np.uint32) & 0xFFFF0000) + 0x10000).view(np.float32).astype(np.float64)
+    xd = x.astype(np.float64)
+    best = np.where(np.abs(xd - t) <= np.abs(step - xd), t, step)
+    tie = np.abs(xd - t) == np.abs(step - xd)
+    assert np.array_equal(r[~tie], best[~tie])
+
+
+def _conv_loops(x, Wl, b, dil):
+    n, H, W, ci = x.shape
+    co = Wl.shape[1]
+    y = np.zeros((n, H, W, co))
+    for i in range(n):
+        for r in range(H):
+            for c in range(W):
+                acc = b.astype(np.float64).copy()
+                for tap in range(9):
+                    rr, cc = r + (tap // 3 - 1) * dil, c + (tap % 3 - 1) * dil
+                    if 0 <= rr < H and 0 <= cc < W:
+                        acc += Wl[tap] @ x[i, rr, cc]
+                y[i, r, c] = acc
+    return y
+
+
+@pytest.mark.parametrize("dil", [1, 2, 4])
+def test_conv_matches_definition(dil):
+    rng = np.random.default_rng(dil)
+    x = rng.standard_normal((2, 7, 9, 5))
+    Wl = rng.standard_normal((9, 3, 5))
+    b = rng.standard_normal(3)
+    assert np.allclose(onet.conv(x, Wl, b, dil), _conv_loops(x, Wl, b, dil), atol=1e-12)
+
+
+@pytest.mark.parametrize("dil", [1, 2, 4])
+def test_conv_gradients_are_adjoint(dil):
+    """<conv(x, W), y> = <x, dgrad(y, W)> = <W, wgrad(y, x)>."""
+    rng = np.random.default_rng(10 + dil)
+    x = rng.standard_normal((2, 6, 11, 4))
+    Wl = rng.standard_normal((9, 3, 4))
+    y = rng.standard_normal((2, 6, 11, 3))
+    lhs = np.sum(onet.conv(x, Wl, np.zeros(3), dil) * y)
+    assert np.isclose(lhs, np.sum(x * onet.conv_data_grad(y, Wl, dil)), rtol=1e-12)
+    gW, gb = onet.conv_weight_grad(y, x, dil)
+    assert np.isclose(lhs, np.sum(Wl * gW), rtol=1e-12)
+    assert np.allclose(gb, y.reshape(-1, 3).sum(0))
+
+
+def test_backward_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    from briefmatch_amd.learned import init_weights
+    w = init_weights(5).astype(np.float64)
+    w[-16:] = rng.standard_normal(16) * 0.1          # non-zero biases of the last layer
+    feat = rng.uniform(-1, 1, (1, 6, 7, 128))
+    label = rng.standard_normal((1, 6, 7, 2)) * 0.05
+    label[0, 0, 0, 1] = np.nan                        # ignored label
+    loss, g = onet.loss_and_grad(w, feat, label, quant=False)
+    idx = np.concatenate([rng.choice(onet.N_PARAMS - 9 * 16 * 64 - 16, 50, replace=False),
+                          onet.N_PARAMS - 16 + np.arange(2)])
+    h = 1e-6
+    for i in idx:
+        wp, wm = w.copy(), w.copy()
+        wp[i] += h
+        wm[i] -= h
+        fd = (onet.loss_and_grad(wp, feat, label, quant=False)[0] - onet.loss_and_grad(wm, feat, label, quant=False)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(g[i]), (i, fd, g[i])
+    # channels 2..15 of the last layer receive no gradient
+    last = onet.split_params(g)[-1]
+    assert not last[0][:, 2:].any() and not last[1][2:].any()
+
+
+def test_features_match_loops():
+    rng = np.random.default_rng(4)
+    W, H = 56, 44                        # 7 x 5 blocks, remainder dropped
+    g1 = rng.integers(0, 256, (H, W), dtype=np.uint8)
+    g2 = rng.integers(0, 256, (H, W), dtype=np.uint8)
+    g2[:, 8:16] = g1[:, 0:8]             # some exact matches (zero SAD)
+    got = onet.bf16_float(onet.features(g1, g2))
+    cw, ch = W // 8, H // 8
+    S = [[[int(g[8 * y:8 * y + 8, 8 * x:8 * x + 8].astype(np.int64).sum()) for x in range(cw)] for y in range(ch)]
+         for g in (g1, g2)]
+    cl = lambda v, n: min(max(v, 0), n - 1)
+    for cy in range(ch):
+        for cx in range(cw):
+            best = None
+            for c in range(128):
+                if c < 121:
+                    dx, dy = c % 11 - 5, c // 11 - 5
+                    sad = sum(abs(S[0][cl(cy + j, ch)][cl(cx + i, cw)] - S[1][cl(cy + j + dy, ch)][cl(cx + i + dx, cw)])
+                              for j in (-1, 0, 1) for i in (-1, 0, 1))
+                    want = min(sad, 36864) / 18432.0 - 1.0
+                    if best is None or sad < best[0]:
+                        best = (sad, dx, dy)
+                elif c < 123:
+                    want = S[c - 121][cy][cx] / (64 * 255.0) - 0.5
+                elif c < 126:
+                    want = (best[1] / 5.0, best[2] / 5.0, min(best[0], 36864) / 18432.0 - 1.0)[c - 123]
+                else:
+                    want = 0.0
+                assert abs(got[cy, cx, c] - want) <= 2 ** -8 * abs(want) + 1e-12, (cy, cx, c)
+
+
+def test_coarse_flow_is_block_mean():
+    rng = np.random.default_rng(5)
+    U = rng.standard_normal((20, 27)).astype(np.float32) * 10
+    V = rng.standard_normal((20, 27)).astype(np.float32) * 10
+    got = onet.coarse_flow(U, V)
+    assert got.shape == (2, 3, 2)
+    want = U[:16, :24].astype(np.float64).reshape(2, 8, 3, 8).mean(axis=(1, 3)) / 8
+    assert np.allclose(got[..., 0], want, atol=1e-6)
+
+
+def test_proposal_offsets_known_values():
+    out = np.zeros((1, 2, 16), np.float32)
+    out[0, 0, :2] = [10.0, -3.0]
+    out[0, 1, :2] = [np.nan, 1e30]
+    got = onet.proposal_offsets(out, W=100, H=50, sx=290, sy=145)
+    assert got[0, 0, 0] == round(10.0 * 2048 * 290 / 100)
+    assert got[0, 0, 1] == round(-3.0 * 2048 * 145 / 50)
+    assert got[0, 1, 0] == -256 * 290 and got[0, 1, 1] == 256 * 145
+
+
+def _init_loops(F1, F2, coff, W, H, first, prev):
+    sy, sx = F1.shape[:2]
+    ch, cw = coff.shape[:2]
+    pop = lambda a: int(sum(bin(int(v)).count("1") for v in a))
+    unp = lambda o: (int(np.int16(np.uint16(o & 0xFFFF))), int(np.int16(np.uint16(o >> 16))))
+    pk = lambda dx, dy: (dx & 0xFFFF) | ((dy & 0xFFFF) << 16)
+    off = np.zeros((sy, sx), np.uint32)
+    cost = np.zeros((sy, sx), np.uint16)
+    for y in range(sy):
+        for x in range(sx):
+            nx = ny = 0
+            if not first:
+                dx0, dy0 = unp(int(prev[y, x]))
+                dx0, dy0 = min(max(dx0, -x), sx - 1 - x), min(max(dy0, -y), sy - 1 - y)
+                dx1, dy1 = unp(int(prev[y + dy0, x + dx0]))
+                nx, ny = min(max(dx1, -x), sx - 1 - x), min(max(dy1, -y), sy - 1 - y)
+            best, bo = pop(F1[y, x] ^ F2[y + ny, x + nx]), pk(nx, ny)
+            fx = min(max(x * W * 32 // sx - 112, 0), (cw - 1) * 256)
+            fy = min(max(y * H * 32 // sy - 112, 0), (ch - 1) * 256)
+            x0, y0 = fx >> 8, fy >> 8
+            x1, y1 = min(x0 + 1, cw - 1), min(y0 + 1, ch - 1)
+            ax, ay = (fx & 255) / 256.0, (fy & 255) / 256.0
+            cells = [(y0, x0), (y0, x1), (y1, x0), (y1, x1)]
+            o = [coff[r, c].astype(np.float64) / 256.0 for r, c in cells]
+            bil = ((o[0] * (1 - ax) + o[1] * ax) * (1 - ay) + (o[2] * (1 - ax) + o[3] * ax) * ay)
+            cands = [tuple(int(np.floor(v + 0.5)) for v in bil)]
+            cands += [tuple(int(np.floor(v + 0.5)) for v in oc) for oc in o]
+            for cx, cy in cands:
+                if not (0 <= x + cx < sx and 0 <= y + cy < sy) or pk(cx, cy) == bo:
+                    continue
+                c = pop(F1[y, x] ^ F2[y + cy, x + cx])
+                if c < best:
+                    best, bo = c, pk(cx, cy)
+            off[y, x], cost[y, x] = bo, best
+    return off, cost
+
+
+@pytest.mark.parametrize("first", [True, False])
+def test_init_proposal_matches_loops(first):
+    rng = np.random.default_rng(6 + first)
+    W, H, sx, sy = 32, 24, 47, 35
+    # few distinct descriptors so that candidates tie and collide
+    pal = rng.integers(0, 2 ** 32, (6, 2), dtype=np.uint64).astype(np.uint32)
+    F1 = pal[rng.integers(0, 6, (sy, sx))]
+    F2 = pal[rng.integers(0, 6, (sy, sx))]
+    coff = rng.integers(-12 * 256, 12 * 256, (H // 8, W // 8, 2)).astype(np.int32)
+    coff[0, 0] = [0, 0]
+    prev = None
+    if not first:
+        dx = rng.integers(-6, 7, (sy, sx))
+        dy = rng.integers(-6, 7, (sy, sx))
+        prev = ((dx & 0xFFFF) | ((dy & 0xFFFF) << 16)).astype(np.uint32)
+    got = onet.init_proposal(F1, F2, coff, W, H, first=first, prev_off=prev)
+    want = _init_loops(F1, F2, coff, W, H, first, prev)
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+
+
+def test_oracle_plain_candidate_is_orc_init():
+    """With a proposal that never wins, the seeded initialisation is the plain one."""
+    import oracle
+    rng = np.random.default_rng(8)
+    sx, sy = 40, 30
+    F1 = rng.integers(0, 2 ** 32, (sy, sx, 1), dtype=np.uint64).astype(np.uint32)
+    F2 = F1.copy()                                  # zero offset is optimal (cost 0)
+    coff = rng.integers(-2000, 2000, (2, 3, 2)).astype(np.int32)
+    off, cost = onet.init_proposal(F1, F2, coff, 24, 16)
+    o2, c2 = oracle.init(F1, F2, first=True)
+    assert np.array_equal(off, o2) and np.array_equal(cost, c2)
