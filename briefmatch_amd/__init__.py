@@ -140,7 +140,7 @@ def lib():
         "bm_ctx_frames": (I, [V, V, V]),
         "bm_proposal_features": (I, [V, V, V, V]),
         "bm_coarse_flow": (I, [V, V, V, V]),
-        "bm_proposal_offsets": (I, [V, V, V]),
+        "bm_proposal_offsets": (I, [V, V, V, V]),
         "bm_init_proposal": (I, [V, V, V, I, V, V, V, V]),
         "bm_flow_pair_learned": (I, [V, V, V, V, U32, V, V]),
     }
@@ -443,8 +443,8 @@ class Context:
     def coarse_flow(self, U, V, label):
         self._c(lib().bm_coarse_flow(self.h, _ptr(U), _ptr(V), _ptr(label)))
 
-    def proposal_offsets(self, net_out, coff):
-        self._c(lib().bm_proposal_offsets(self.h, _ptr(net_out), _ptr(coff)))
+    def proposal_offsets(self, net_out, feat, coff):
+        self._c(lib().bm_proposal_offsets(self.h, _ptr(net_out), _ptr(feat), _ptr(coff)))
 
     def init_proposal(self, F1, F2, coff, off, cost, first=True, prev_off=None):
         self._c(lib().bm_init_proposal(self.h, _ptr(F1), _ptr(F2), 1 if first else 0, _ptr(prev_off), _ptr(coff),

@@ -73,7 +73,7 @@ struct bm_ctx {
     uint16_t *d_feat = nullptr;            // [ch][cw][128] BF16
     uint16_t *d_nact[2] = {nullptr, nullptr};  // [ch][cw][64] BF16
     float *d_nout = nullptr;               // [ch][cw][16]
-    int32_t *d_coff = nullptr;             // [ch][cw][2] Q8
+    int32_t *d_coff = nullptr;             // [ch][cw][4] Q8
     int32_t *d_bsum = nullptr;             // [2][ch][cw] block sums
     // sequence mode
     int have_prev = 0, cur = 0, field_valid = 0;
@@ -916,19 +916,20 @@ int proposal_ready(bm_ctx *ctx)
     if (!r) r = dev_alloc(ctx, &ctx->d_nact[0], cells * 64);
     if (!r) r = dev_alloc(ctx, &ctx->d_nact[1], cells * 64);
     if (!r) r = dev_alloc(ctx, &ctx->d_nout, cells * bm::net::kOutC);
-    if (!r) r = dev_alloc(ctx, &ctx->d_coff, cells * 2);
+    if (!r) r = dev_alloc(ctx, &ctx->d_coff, cells * 4);
     if (!r) r = dev_alloc(ctx, &ctx->d_bsum, cells * 2);
     if (r) ctx->d_feat = nullptr;
     return r;
 }
 
-hipError_t launch_offsets(bm_ctx *ctx, const float *nout, int32_t *coff)
+hipError_t launch_offsets(bm_ctx *ctx, const float *nout, const uint16_t *feat, int32_t *coff)
 {
     const bm::Geometry &g = ctx->g;
     const int cw = g.W / bm::net::kCell, ch = g.H / bm::net::kCell;
     const float scx = (float)(256 * bm::net::kCell * g.sx) / (float)g.W;
     const float scy = (float)(256 * bm::net::kCell * g.sy) / (float)g.H;
-    return bm::net::launch_proposal_offsets(nout, cw * ch, scx, scy, 256 * g.sx, 256 * g.sy, coff, ctx->stream);
+    return bm::net::launch_proposal_offsets(nout, feat, cw * ch, scx, scy, 256 * g.sx, 256 * g.sy, coff,
+                                            ctx->stream);
 }
 
 }  // namespace
@@ -958,14 +959,15 @@ int bm_coarse_flow(bm_ctx *ctx, const float *U, const float *V, float *label)
     return BM_OK;
 }
 
-int bm_proposal_offsets(bm_ctx *ctx, const float *nout, int32_t *coff)
+int bm_proposal_offsets(bm_ctx *ctx, const float *nout, const uint16_t *feat, int32_t *coff)
 {
     int r = check(ctx, nout, "net_out");
+    if (!r) r = check(ctx, feat, "feat");
     if (!r) r = check(ctx, coff, "coff");
     if (r) return r;
     if (ctx->g.W < bm::net::kCell || ctx->g.H < bm::net::kCell)
         return fail(ctx, BM_E_PARAM, "the learned proposal needs frames of at least 8x8 pixels");
-    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, launch_offsets(ctx, nout, coff));
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, launch_offsets(ctx, nout, feat, coff));
     return BM_OK;
 }
 
@@ -1010,11 +1012,16 @@ int bm_flow_pair_learned(bm_ctx *ctx, bm_net *net, const uint8_t *rgb1, const ui
     BM_HIP(ctx, hipStreamWaitEvent(ctx->stream, net->ready, 0));
     BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, bm::net::forward(net->pk, ctx->d_feat, 1, ch, cw, ctx->d_nact[0], ctx->d_nact[1],
                                                          ctx->d_nout, ctx->stream));
-    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, launch_offsets(ctx, ctx->d_nout, ctx->d_coff));
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, launch_offsets(ctx, ctx->d_nout, ctx->d_feat, ctx->d_coff));
     const int dst = ctx->field;
     BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, bm::net::launch_init_proposal(g.NSP, ctx->d_F[0], ctx->d_F[1], 1, nullptr,
                                                                       ctx->d_coff, cw, ch, g.W, g.H, g.sx, g.sy,
                                                                       ctx->d_off[dst], ctx->d_cost[dst], ctx->stream));
+    // one random search around the seeded field, on its own stream index
+    // (the propagation iterations use 0 .. iterations - 1)
+    BM_LAUNCH(ctx, BM_STAGE_RANDOM_SEARCH, bm::launch_random_search(g, ctx->d_F[0], ctx->d_F[1], ctx->d_off[dst],
+                                                                    ctx->d_cost[dst], (uint32_t)g.iterations, pair,
+                                                                    ctx->stream));
     int pd = -1;
     for (int i = 0; i < g.iterations; ++i) {
         r = run_iteration(ctx, ctx->d_F[0], ctx->d_F[1], pd, i, pair, true);

@@ -23,6 +23,9 @@ constexpr int kLayers = 5;
 // cost-volume scaling: SAD of 9 block sums; 9 * 64 * 64 (a mean absolute
 // difference of 64 gray levels) maps to 1, zero to -1
 constexpr int kSadClamp = 9 * 64 * 64;
+// feature channels of the lowest-cost displacement (dx / 8, dy / 8), added
+// times 8 to output channels 0, 1
+constexpr int kSkipChannel = kCost + 2;
 
 struct LayerSpec {
     int cin, cout, dil, relu;
@@ -52,8 +55,10 @@ enum { ACT_NONE = 0, ACT_RELU = 1, ACT_MASK = 2 };
 // S: scratch of 2 * cw * ch int32 (block sums)
 hipError_t launch_features(const uint8_t *g1, const uint8_t *g2, int W, int cw, int ch, int32_t *S, uint16_t *feat,
                            hipStream_t st);
+// skip (optional, the feature map): output channels 0, 1 += 8 * skip[p][kSkipChannel + c]
 hipError_t launch_conv(int cout, const uint16_t *in, int cin, const uint16_t *w, const float *bias, int npix, int H,
-                       int W, int dil, int act, const uint16_t *ref, uint16_t *out_bf, float *out_f, hipStream_t st);
+                       int W, int dil, int act, const uint16_t *ref, const uint16_t *skip, uint16_t *out_bf,
+                       float *out_f, hipStream_t st);
 int wgrad_chunks(int npix);
 hipError_t launch_wgrad(int cout, int cin, const uint16_t *dz, const uint16_t *x, int npix, int H, int W, int dil,
                         float *part, float *part_bias, float *grad_w, float *grad_b, hipStream_t st);
@@ -64,8 +69,9 @@ hipError_t launch_adam(float *w, float *m, float *v, const float *g, int n, floa
                        float c1, float c2, hipStream_t st);
 hipError_t launch_pack(const float *w, int cout, int cin, uint16_t *wb, uint16_t *wt, hipStream_t st);
 hipError_t launch_coarse_flow(const float *U, const float *V, int W, int cw, int ch, float *out, hipStream_t st);
-hipError_t launch_proposal_offsets(const float *net_out, int n, float scale_x, float scale_y, int lim_x, int lim_y,
-                                   int32_t *coff, hipStream_t st);
+// coff [n][4]: network (u, v) and cost-volume argmin (u, v), Q8 matched pixels
+hipError_t launch_proposal_offsets(const float *net_out, const uint16_t *feat, int n, float scale_x, float scale_y,
+                                   int lim_x, int lim_y, int32_t *coff, hipStream_t st);
 hipError_t launch_init_proposal(int NSP, const uint32_t *F1, const uint32_t *F2, int first, const uint32_t *prev,
                                 const int32_t *coff, int cw, int ch, int W, int H, int sx, int sy, uint32_t *off,
                                 uint16_t *cost, hipStream_t st);

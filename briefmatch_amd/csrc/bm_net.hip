@@ -6,10 +6,12 @@
 // 121 displacements of -5..5 coarse pixels in x and y (sum of absolute
 // differences of 8x8 block sums over a 3x3 block window, clamped and scaled to
 // [-1, 1]), the block means of both gray frames, the lowest-cost displacement
-// (dx / 5, dy / 5) and its cost, and zero padding.  Layers (3x3, zero padding):
+// (dx / 8, dy / 8, exact in BF16) and its cost, and zero padding.  Layers
+// (3x3, zero padding):
 //   128 -> 64 ReLU, 64 -> 64 ReLU dilation 2, 64 -> 64 ReLU dilation 4,
-//   64 -> 64 ReLU, 64 -> 16 linear (channels 0, 1 = coarse flow in coarse
-//   pixels, i.e. units of 8 input pixels).
+//   64 -> 64 ReLU, 64 -> 16 linear; channels 0, 1 plus the lowest-cost
+//   displacement (a fixed skip connection) are the coarse flow in coarse
+//   pixels, i.e. units of 8 input pixels.
 // 193,808 parameters; the 14 unused output channels (8,078 parameters) get no
 // gradient and stay at zero.
 //
@@ -103,8 +105,8 @@ __global__ void __launch_bounds__(128) k_features(const int32_t *__restrict__ S,
     __syncthreads();
     const int best = min(red[0], red[1]);
     const int bc = best & 127, bcost = best >> 7;
-    if (c == kCost + 2) v = __fdiv_rn((float)(bc % (2 * kRadius + 1) - kRadius), (float)kRadius);
-    if (c == kCost + 3) v = __fdiv_rn((float)(bc / (2 * kRadius + 1) - kRadius), (float)kRadius);
+    if (c == kCost + 2) v = (float)(bc % (2 * kRadius + 1) - kRadius) * 0.125f;
+    if (c == kCost + 3) v = (float)(bc / (2 * kRadius + 1) - kRadius) * 0.125f;
     if (c == kCost + 4) v = __fdiv_rn((float)(min(bcost, kSadClamp) - kSadClamp / 2), (float)(kSadClamp / 2));
     feat[(size_t)cp * kFeatC + c] = f2bf(v);
 }
@@ -120,8 +122,8 @@ template <int COUT, int PT>
 __global__ void __launch_bounds__(256) k_conv(const uint16_t *__restrict__ in, int cin,
                                               const uint16_t *__restrict__ w, const float *__restrict__ bias,
                                               int npix, int H, int W, int dil, int act,
-                                              const uint16_t *__restrict__ ref, uint16_t *__restrict__ out_bf,
-                                              float *__restrict__ out_f)
+                                              const uint16_t *__restrict__ ref, const uint16_t *__restrict__ skip,
+                                              uint16_t *__restrict__ out_bf, float *__restrict__ out_f)
 {
     constexpr int CT = COUT / 16;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -178,6 +180,7 @@ __global__ void __launch_bounds__(256) k_conv(const uint16_t *__restrict__ in, i
                 const int q = p0 + 16 * t + 4 * kq + j, co = 16 * u + m;
                 if (q >= npix) continue;
                 float v = acc[t][u][j] + (bias ? bias[co] : 0.0f);
+                if (skip && co < 2) v += 8.0f * bf2f(skip[(size_t)q * kFeatC + kSkipChannel + co]);
                 if (act == ACT_RELU) v = fmaxf(v, 0.0f);
                 if (act == ACT_MASK && !(bf2f(ref[(size_t)q * COUT + co]) > 0.0f)) v = 0.0f;
                 if (out_f) out_f[(size_t)q * COUT + co] = v;
@@ -368,17 +371,24 @@ __global__ void k_coarse_flow(const float *__restrict__ U, const float *__restri
     out[2 * i + 1] = sv * (1.0f / 512.0f);
 }
 
-// Network output (coarse pixels) -> matched-pixel offsets in Q8 per coarse cell:
-// rint(clamp(u * scale)), scale = 256 * 8 * sx / W; NaN maps to -lim.
-__global__ void k_proposal_offsets(const float *__restrict__ out, int n, float scale_x, float scale_y, int lim_x,
-                                   int lim_y, int32_t *__restrict__ coff)
+// Per coarse cell, two proposals as matched-pixel offsets in Q8: the network's
+// coarse flow (output channels 0, 1) and the lowest-cost displacement of the
+// feature map (channels 123, 124 times 8), each rint(clamp(u * scale)),
+// scale = 256 * 8 * sx / W; NaN maps to -lim.  coff [n][4] = (net u, net v,
+// argmin u, argmin v).
+__global__ void k_proposal_offsets(const float *__restrict__ out, const uint16_t *__restrict__ feat, int n,
+                                   float scale_x, float scale_y, int lim_x, int lim_y, int32_t *__restrict__ coff)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const float u = fminf(fmaxf(out[(size_t)i * kOutC] * scale_x, (float)-lim_x), (float)lim_x);
-    const float v = fminf(fmaxf(out[(size_t)i * kOutC + 1] * scale_y, (float)-lim_y), (float)lim_y);
-    coff[2 * i] = (int32_t)rintf(u);
-    coff[2 * i + 1] = (int32_t)rintf(v);
+    const float src[4] = {out[(size_t)i * kOutC], out[(size_t)i * kOutC + 1],
+                          8.0f * bf2f(feat[(size_t)i * kFeatC + kSkipChannel]),
+                          8.0f * bf2f(feat[(size_t)i * kFeatC + kSkipChannel + 1])};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float sc = (k & 1) ? scale_y : scale_x, lim = (float)((k & 1) ? lim_y : lim_x);
+        coff[4 * i + k] = (int32_t)rintf(fminf(fmaxf(src[k] * sc, -lim), lim));
+    }
 }
 
 __device__ __forceinline__ int offx(uint32_t o) { return (int)(int16_t)(o & 0xffffu); }
@@ -386,10 +396,11 @@ __device__ __forceinline__ int offy(uint32_t o) { return (int)(int16_t)(o >> 16)
 __device__ __forceinline__ uint32_t packo(int dx, int dy) { return (uint32_t)(uint16_t)dx | ((uint32_t)(uint16_t)dy << 16); }
 
 // Proposal-seeded initialisation.  The plain candidate (zero offset, or the
-// temporal chain through prev) is evaluated first; then, in order, the
-// bilinear interpolation of the four surrounding coarse cells' offsets and the
-// four cells' own offsets.  A candidate is skipped when it points outside the
-// frame or equals the running best, and taken only when strictly cheaper.
+// temporal chain through prev) is evaluated first; then, for the network's
+// proposal and then the cost-volume argmin, the bilinear interpolation of the
+// four surrounding coarse cells' offsets and the four cells' own offsets.  A
+// candidate is skipped when it points outside the frame or equals the running
+// best, and taken only when strictly cheaper.
 template <int NSP>
 __global__ void __launch_bounds__(256) k_init_proposal(const uint32_t *__restrict__ F1,
                                                        const uint32_t *__restrict__ F2, int first,
@@ -420,20 +431,23 @@ __global__ void __launch_bounds__(256) k_init_proposal(const uint32_t *__restric
     const int64_t qy = min(max((int64_t)y * H * 32 / sy - 112, (int64_t)0), (int64_t)(ch - 1) * 256);
     const int x0 = (int)(qx >> 8), y0 = (int)(qy >> 8), x1 = min(x0 + 1, cw - 1), y1 = min(y0 + 1, ch - 1);
     const int64_t ax = qx & 255, ay = qy & 255;
-    const int c00 = (y0 * cw + x0) * 2, c01 = (y0 * cw + x1) * 2, c10 = (y1 * cw + x0) * 2, c11 = (y1 * cw + x1) * 2;
-    int cand[5][2];
+    const int c00 = (y0 * cw + x0) * 4, c01 = (y0 * cw + x1) * 4, c10 = (y1 * cw + x0) * 4, c11 = (y1 * cw + x1) * 4;
+    int cand[10][2];
 #pragma unroll
-    for (int d = 0; d < 2; ++d) {
-        const int64_t o00 = coff[c00 + d], o01 = coff[c01 + d], o10 = coff[c10 + d], o11 = coff[c11 + d];
-        const int64_t v = (o00 * (256 - ax) + o01 * ax) * (256 - ay) + (o10 * (256 - ax) + o11 * ax) * ay;
-        cand[0][d] = (int)((v + (1 << 23)) >> 24);
-        cand[1][d] = (int)((o00 + 128) >> 8);
-        cand[2][d] = (int)((o01 + 128) >> 8);
-        cand[3][d] = (int)((o10 + 128) >> 8);
-        cand[4][d] = (int)((o11 + 128) >> 8);
-    }
+    for (int src = 0; src < 2; ++src)
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
+        for (int d = 0; d < 2; ++d) {
+            const int e = 2 * src + d;
+            const int64_t o00 = coff[c00 + e], o01 = coff[c01 + e], o10 = coff[c10 + e], o11 = coff[c11 + e];
+            const int64_t v = (o00 * (256 - ax) + o01 * ax) * (256 - ay) + (o10 * (256 - ax) + o11 * ax) * ay;
+            cand[5 * src][d] = (int)((v + (1 << 23)) >> 24);
+            cand[5 * src + 1][d] = (int)((o00 + 128) >> 8);
+            cand[5 * src + 2][d] = (int)((o01 + 128) >> 8);
+            cand[5 * src + 3][d] = (int)((o10 + 128) >> 8);
+            cand[5 * src + 4][d] = (int)((o11 + 128) >> 8);
+        }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
         const int cx = cand[k][0], cy = cand[k][1];
         if (x + cx < 0 || x + cx >= sx || y + cy < 0 || y + cy >= sy) continue;
         const uint32_t po = packo(cx, cy);
@@ -482,27 +496,28 @@ hipError_t launch_features(const uint8_t *g1, const uint8_t *g2, int W, int cw, 
 
 template <int COUT, int PT>
 static void conv_t(const uint16_t *in, int cin, const uint16_t *w, const float *bias, int npix, int H, int W, int dil,
-                   int act, const uint16_t *ref, uint16_t *out_bf, float *out_f, hipStream_t st)
+                   int act, const uint16_t *ref, const uint16_t *skip, uint16_t *out_bf, float *out_f, hipStream_t st)
 {
     const int per_block = 4 * 16 * PT;
     hipLaunchKernelGGL((k_conv<COUT, PT>), dim3((npix + per_block - 1) / per_block), dim3(256), 0, st, in, cin, w,
-                       bias, npix, H, W, dil, act, ref, out_bf, out_f);
+                       bias, npix, H, W, dil, act, ref, skip, out_bf, out_f);
 }
 
 hipError_t launch_conv(int cout, const uint16_t *in, int cin, const uint16_t *w, const float *bias, int npix, int H,
-                       int W, int dil, int act, const uint16_t *ref, uint16_t *out_bf, float *out_f, hipStream_t st)
+                       int W, int dil, int act, const uint16_t *ref, const uint16_t *skip, uint16_t *out_bf,
+                       float *out_f, hipStream_t st)
 {
     if (cin % 16 || npix <= 0) return hipErrorInvalidValue;
     // enough work-groups to cover the chip before blocking pixels per wave
     const int pt = npix >= 4 * 64 * 1024 ? 4 : (npix >= 2 * 64 * 512 ? 2 : 1);
     if (cout == 64) {
-        if (pt == 4) conv_t<64, 4>(in, cin, w, bias, npix, H, W, dil, act, ref, out_bf, out_f, st);
-        else if (pt == 2) conv_t<64, 2>(in, cin, w, bias, npix, H, W, dil, act, ref, out_bf, out_f, st);
-        else conv_t<64, 1>(in, cin, w, bias, npix, H, W, dil, act, ref, out_bf, out_f, st);
+        if (pt == 4) conv_t<64, 4>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
+        else if (pt == 2) conv_t<64, 2>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
+        else conv_t<64, 1>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
     } else if (cout == 16) {
-        if (pt == 4) conv_t<16, 4>(in, cin, w, bias, npix, H, W, dil, act, ref, out_bf, out_f, st);
-        else if (pt == 2) conv_t<16, 2>(in, cin, w, bias, npix, H, W, dil, act, ref, out_bf, out_f, st);
-        else conv_t<16, 1>(in, cin, w, bias, npix, H, W, dil, act, ref, out_bf, out_f, st);
+        if (pt == 4) conv_t<16, 4>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
+        else if (pt == 2) conv_t<16, 2>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
+        else conv_t<16, 1>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
     } else {
         return hipErrorInvalidValue;
     }
@@ -564,10 +579,10 @@ hipError_t launch_coarse_flow(const float *U, const float *V, int W, int cw, int
     return hipGetLastError();
 }
 
-hipError_t launch_proposal_offsets(const float *net_out, int n, float scale_x, float scale_y, int lim_x, int lim_y,
-                                   int32_t *coff, hipStream_t st)
+hipError_t launch_proposal_offsets(const float *net_out, const uint16_t *feat, int n, float scale_x, float scale_y,
+                                   int lim_x, int lim_y, int32_t *coff, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_proposal_offsets, dim3((n + 255) / 256), dim3(256), 0, st, net_out, n, scale_x, scale_y,
+    hipLaunchKernelGGL(k_proposal_offsets, dim3((n + 255) / 256), dim3(256), 0, st, net_out, feat, n, scale_x, scale_y,
                        lim_x, lim_y, coff);
     return hipGetLastError();
 }
@@ -603,7 +618,8 @@ hipError_t forward(const Packed &pk, const uint16_t *feat, int n, int ch, int cw
         const bool last = l == kLayers - 1;
         uint16_t *y = last ? nullptr : bufs[l & 1];
         hipError_t e = launch_conv(s.cout, x, s.cin, pk.w[l], pk.b[l], npix, ch, cw, s.dil,
-                                   s.relu ? ACT_RELU : ACT_NONE, nullptr, y, last ? out : nullptr, st);
+                                   s.relu ? ACT_RELU : ACT_NONE, nullptr, last ? feat : nullptr, y,
+                                   last ? out : nullptr, st);
         if (e != hipSuccess) return e;
         x = y;
     }

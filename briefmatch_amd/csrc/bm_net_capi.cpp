@@ -102,12 +102,12 @@ int forward_backward(bm_net *net, const uint16_t *x0, const float *label, int n,
     const uint16_t *x = x0;
     for (int l = 0; l < kLayers - 1; ++l) {
         NET_HIP(net, launch_conv(kSpec[l].cout, x, kSpec[l].cin, net->pk.w[l], net->pk.b[l], npix, ch, cw,
-                                 kSpec[l].dil, ACT_RELU, nullptr, net->act[l], nullptr, st));
+                                 kSpec[l].dil, ACT_RELU, nullptr, nullptr, net->act[l], nullptr, st));
         x = net->act[l];
     }
     const int L = kLayers - 1;
     NET_HIP(net, launch_conv(kSpec[L].cout, x, kSpec[L].cin, net->pk.w[L], net->pk.b[L], npix, ch, cw, kSpec[L].dil,
-                             ACT_NONE, nullptr, nullptr, net->out, st));
+                             ACT_NONE, nullptr, x0, nullptr, net->out, st));
     NET_HIP(net, launch_loss(net->out, label, npix, net->dz4, net->loss_part, st));
     NET_HIP(net, launch_sum(net->loss_part, loss_blocks(npix), net->loss, st));
     const uint16_t *dzl = net->dz4;
@@ -120,7 +120,7 @@ int forward_backward(bm_net *net, const uint16_t *x0, const float *label, int n,
         if (l > 0) {
             uint16_t *dnext = net->dz[l & 1];
             NET_HIP(net, launch_conv(s.cin, dzl, s.cout, net->pk.wt[l], nullptr, npix, ch, cw, s.dil, ACT_MASK,
-                                     net->act[l - 1], dnext, nullptr, st));
+                                     net->act[l - 1], nullptr, dnext, nullptr, st));
             dzl = dnext;
         }
     }

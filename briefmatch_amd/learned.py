@@ -61,12 +61,16 @@ def _offset(buf, nbytes):
 
 
 def train_pair(seed, W=640, H=480):
-    """Training pair `seed`: moving rectangles over an affine background,
-    motion up to 40 px (the family of the 1920x1080 benchmark pair)."""
+    """Training pair `seed`: moving rectangles over an affine background
+    (the family of the 1920x1080 benchmark pair), motion up to 16..40 px, the
+    frames a window of a scene 1 to 3 times their size (so texture and
+    rectangle scales span those of 640x480 and 1920x1080 renderings)."""
     rng = np.random.default_rng([seed, 7])
     n_rects = int(rng.integers(3, 11))
-    max_motion = float(rng.uniform(8.0, 40.0))
-    p = synth.make_pair(W, H, motion="rects", n_rects=n_rects, max_motion=max_motion, seed=seed)
+    max_motion = float(rng.uniform(16.0, 40.0))
+    k = float(rng.uniform(1.0, 3.0))
+    p = synth.make_pair(W, H, motion="rects", n_rects=n_rects, max_motion=max_motion, seed=seed,
+                        scene=(int(W * k), int(H * k)))
     return p.frame1, p.frame2
 
 

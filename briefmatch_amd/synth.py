@@ -31,13 +31,14 @@ class Pair:
     occ: np.ndarray     # (H, W) uint8, 1 = occluded / leaves the frame
 
 
-def _texture(rng, h, w, n_blobs=None, noise_sigma=1.2, noise_amp=22.0):
-    """Blobs + band-limited noise, float32 (h, w, 3) roughly in [0, 255]."""
+def _texture(rng, h, w, n_blobs=None, noise_sigma=1.2, noise_amp=22.0, scale=None):
+    """Blobs + band-limited noise, float32 (h, w, 3) roughly in [0, 255].
+    Blob radii are relative to `scale` (default max(h, w))."""
     img = np.empty((h, w, 3), np.float32)
     img[:] = rng.uniform(60.0, 190.0, size=3).astype(np.float32)
     if n_blobs is None:
         n_blobs = int(np.clip(h * w / 6000.0, 24, 420))
-    scale = float(max(h, w))
+    scale = float(max(h, w)) if scale is None else float(scale)
     for _ in range(n_blobs):
         cy, cx = rng.uniform(0, h), rng.uniform(0, w)
         sig = scale * rng.uniform(0.006, 0.05)
@@ -73,9 +74,13 @@ def _finish(rng, f1, f2, noise_sigma):
             np.clip(np.rint(f2), 0, 255).astype(np.uint8))
 
 
-def _rects_pair(rng, W, H, n_rects, max_motion, noise_sigma):
+def _rects_pair(rng, W, H, n_rects, max_motion, noise_sigma, scene=None):
+    # scene = (Ws, Hs): the frames are a W x H window of a Ws x Hs scene, whose
+    # size sets the blob and rectangle sizes (default: the frame itself)
+    Ws, Hs = (W, H) if scene is None else scene
     m = int(np.ceil(max_motion)) + 8
-    bg = _texture(rng, H + 2 * m, W + 2 * m)
+    bg = _texture(rng, H + 2 * m, W + 2 * m,
+                  scale=None if scene is None else max(Hs, Ws) * max(H + 2 * m, W + 2 * m) / max(H, W))
     yy, xx = np.mgrid[0:H, 0:W].astype(np.float32)
     cy, cx = (H - 1) / 2.0, (W - 1) / 2.0
 
@@ -98,8 +103,8 @@ def _rects_pair(rng, W, H, n_rects, max_motion, noise_sigma):
     layer1 = np.zeros((H, W), np.int32)   # topmost layer index per pixel, frame 1
     layer2 = np.zeros((H, W), np.int32)   # same in frame 2
     for k in range(1, n_rects + 1):
-        rw = int(rng.uniform(0.08, 0.28) * W)
-        rh = int(rng.uniform(0.08, 0.28) * H)
+        rw = min(int(rng.uniform(0.08, 0.28) * Ws), W - 1)
+        rh = min(int(rng.uniform(0.08, 0.28) * Hs), H - 1)
         x0 = int(rng.uniform(0, W - rw))
         y0 = int(rng.uniform(0, H - rh))
         ang = rng.uniform(0, 2 * np.pi)
@@ -179,11 +184,13 @@ def _regions_pair(rng, W, H, n_regions, max_motion, noise_sigma):
 
 
 def make_pair(W, H, motion="rects", n_rects=3, n_regions=6, max_motion=8.0,
-              noise_sigma=2.0, seed=0):
-    """One seeded synthetic pair; see the module docstring."""
+              noise_sigma=2.0, seed=0, scene=None):
+    """One seeded synthetic pair; see the module docstring.  For "rects",
+    scene = (Ws, Hs) renders a W x H window of a larger scene (blob and
+    rectangle sizes follow the scene size)."""
     rng = np.random.default_rng(seed)
     if motion == "rects":
-        return _rects_pair(rng, int(W), int(H), int(n_rects), float(max_motion), float(noise_sigma))
+        return _rects_pair(rng, int(W), int(H), int(n_rects), float(max_motion), float(noise_sigma), scene)
     if motion == "regions":
         return _regions_pair(rng, int(W), int(H), int(n_regions), float(max_motion), float(noise_sigma))
     raise ValueError("motion must be 'rects' or 'regions'")

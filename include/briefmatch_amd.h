@@ -233,8 +233,9 @@ const char *bm_batch_last_error(const bm_batch *b);
  * Feature maps are [n][ch][cw][BM_NET_FEATURES] BF16 (uint16) with
  * cw = W / 8, ch = H / 8 (bm_proposal_dims); network outputs are
  * [n][ch][cw][BM_NET_OUTPUTS] FP32, channels 0 and 1 = coarse flow (u, v) in
- * coarse pixels (8 input pixels); labels are [n][ch][cw][2] FP32 in the same
- * unit (non-finite = ignored).
+ * coarse pixels (8 input pixels): the network's residual plus 8 x feature
+ * channels 123, 124 (a fixed skip connection); labels are [n][ch][cw][2] FP32
+ * in the same unit (non-finite = ignored).
  * The parameter vector is, per layer, W[9][cout][cin] then b[cout]
  * (layers 128->64, 64->64 dil 2, 64->64 dil 4, 64->64, 64->16). */
 #define BM_NET_FEATURES 128
@@ -274,18 +275,24 @@ int bm_ctx_frames(bm_ctx *ctx, uint8_t **g8_dev, uint32_t **F_dev);
  * dx + 5 (sum of absolute differences of 8x8 block sums over a 3x3 block
  * window, clamp-to-edge, min(SAD, 36864) / 18432 - 1); 121, 122 the block
  * means of both frames / 255 - 1/2; 123, 124 the lowest-cost displacement
- * (lowest channel on ties) / 5; 125 its scaled cost; 126, 127 zero. */
+ * (lowest channel on ties) / 8; 125 its scaled cost; 126, 127 zero. */
 int bm_proposal_features(bm_ctx *ctx, const uint8_t *g1_dev, const uint8_t *g2_dev, uint16_t *feat_dev);
 /* 8x8 block means of a flow field divided by 8 (training labels), [ch][cw][2]. */
 int bm_coarse_flow(bm_ctx *ctx, const float *U_dev, const float *V_dev, float *label_dev);
-/* Network output -> per-cell matched-pixel offsets in Q8, [ch][cw][2] int32:
- * rint(clamp(u * (float)(2048 sx) / (float)W, +-256 sx)) and likewise for v. */
-int bm_proposal_offsets(bm_ctx *ctx, const float *net_out_dev, int32_t *coff_dev);
-/* Proposal-seeded initialisation (the plain bm_init candidate first, then the
- * bilinear and the four nearest cell offsets; strictly cheaper wins). */
+/* Per-cell proposals as matched-pixel offsets in Q8, [ch][cw][4] int32: the
+ * network's coarse flow (u, v) and the feature map's lowest-cost displacement
+ * (8 x channels 123, 124), each rint(clamp(u * (float)(2048 sx) / (float)W,
+ * +-256 sx)) (v likewise with sy, H). */
+int bm_proposal_offsets(bm_ctx *ctx, const float *net_out_dev, const uint16_t *feat_dev, int32_t *coff_dev);
+/* Proposal-seeded initialisation: the plain bm_init candidate first, then for
+ * the network's and then the argmin proposal the bilinear interpolation of
+ * the four surrounding cells and the four cells' own offsets; out-of-frame
+ * candidates and repeats of the running best are skipped, strictly cheaper
+ * wins. */
 int bm_init_proposal(bm_ctx *ctx, const uint32_t *F1_dev, const uint32_t *F2_dev, int first,
                      const uint32_t *prev_off_dev, const int32_t *coff_dev, uint32_t *off_dev, uint16_t *cost_dev);
-/* bm_flow_pair with the proposal-seeded initialisation, then the context's
+/* bm_flow_pair with the proposal-seeded initialisation followed by one random
+ * search (random-search iteration index `iterations`), then the context's
  * `iterations` propagation iterations. */
 int bm_flow_pair_learned(bm_ctx *ctx, bm_net *net, const uint8_t *rgb1_dev, const uint8_t *rgb2_dev, uint32_t pair,
                          float *U_dev, float *V_dev);

@@ -24,6 +24,7 @@ COST = (2 * RADIUS + 1) ** 2
 FEAT_C = 128
 OUT_C = 16
 CELL = 8
+SKIP = COST + 2          # feature channels of the lowest-cost displacement / 8
 # (cin, cout, dilation, relu) per layer
 LAYERS = ((128, 64, 1, True), (64, 64, 2, True), (64, 64, 4, True), (64, 64, 1, True), (64, 16, 1, False))
 N_PARAMS = sum(9 * co * ci + co for ci, co, _, _ in LAYERS)
@@ -73,7 +74,7 @@ def features(g1, g2):
     window of the coarse pixel and those of frame 2 displaced by (dx, dy)
     blocks (coordinates clamped to the grid), as min(SAD, 36864) / 18432 - 1.
     Then the block means of both frames / 255 - 1/2, the lowest-SAD
-    displacement (first channel on ties) as dx / 5, dy / 5, its scaled SAD,
+    displacement (first channel on ties) as dx / 8, dy / 8, its scaled SAD,
     and zeros."""
     S1, S2 = block_sums(g1), block_sums(g2)
     ch, cw = S1.shape
@@ -95,8 +96,8 @@ def features(g1, g2):
     out[..., COST + 1] = (S2 - 8160).astype(np.float32) / np.float32(16320)
     bc = np.argmin(costs, axis=-1)
     bcost = np.take_along_axis(costs, bc[..., None], -1)[..., 0]
-    out[..., COST + 2] = (bc % (2 * RADIUS + 1) - RADIUS).astype(np.float32) / np.float32(RADIUS)
-    out[..., COST + 3] = (bc // (2 * RADIUS + 1) - RADIUS).astype(np.float32) / np.float32(RADIUS)
+    out[..., COST + 2] = (bc % (2 * RADIUS + 1) - RADIUS).astype(np.float32) / np.float32(8)
+    out[..., COST + 3] = (bc // (2 * RADIUS + 1) - RADIUS).astype(np.float32) / np.float32(8)
     out[..., COST + 4] = (np.minimum(bcost, SAD_CLAMP) - half).astype(np.float32) / np.float32(half)
     return bf16_bits(out)
 
@@ -118,16 +119,20 @@ def coarse_flow(U, V):
 # ---------------------------------------------------------------------------
 # proposal offsets and the seeded initialisation
 
-def proposal_offsets(net_out, W, H, sx, sy):
-    """Network output [.., 16] (coarse pixels) -> Q8 matched-pixel offsets
-    [.., 2] int32: rint(clamp(u * 2048 sx / W, +-256 sx))."""
+def proposal_offsets(net_out, feat, W, H, sx, sy):
+    """Per cell [.., 4] int32 Q8 matched-pixel offsets: the network output
+    (channels 0, 1, coarse pixels) and the feature map's lowest-cost
+    displacement (8 x channels 123, 124), each rint(clamp(u * 2048 sx / W,
+    +-256 sx)) (v with sy, H)."""
     out = np.asarray(net_out, np.float32)
+    am = bf16_float(feat)[..., SKIP:SKIP + 2] * np.float32(8)
+    srcs = (out[..., 0], out[..., 1], am[..., 0], am[..., 1])
     res = []
-    for c, (s, d) in enumerate(((sx, W), (sy, H))):
+    for k, u in enumerate(srcs):
+        s, d = (sy, H) if k & 1 else (sx, W)
         scale = np.float32(256 * CELL * s) / np.float32(d)
         lim = np.float32(256 * s)
-        v = out[..., c] * scale
-        v = np.fmin(np.fmax(v, -lim), lim)
+        v = np.fmin(np.fmax(u.astype(np.float32) * scale, -lim), lim)
         res.append(np.rint(v).astype(np.int32))
     return np.stack(res, -1)
 
@@ -146,12 +151,13 @@ def init_proposal(F1, F2, coff, W, H, first=True, prev_off=None):
     """Proposal-seeded initialisation; returns (off [sy, sx] uint32, cost uint16).
 
     Candidates in order: the plain initialisation (zero, or prev followed once
-    with clamping, briefmatch_cuda.cu:149-184), the bilinear interpolation of
-    the Q8 offsets of the four cells around the pixel (coarse coordinate
-    x * W * 32 / sx - 112 in Q8, clamped to the grid; sum rounded half up at
-    2^-24), then the four cells' offsets rounded half up.  Skip out-of-frame
-    candidates and candidates equal to the running best; take strictly lower
-    costs."""
+    with clamping, briefmatch_cuda.cu:149-184); then for the network's
+    proposal (coff[..., 0:2]) and then the argmin proposal (coff[..., 2:4]):
+    the bilinear interpolation of the Q8 offsets of the four cells around the
+    pixel (coarse coordinate x * W * 32 / sx - 112 in Q8, clamped to the grid;
+    sum rounded half up at 2^-24), then the four cells' offsets rounded half
+    up.  Skip out-of-frame candidates and candidates equal to the running
+    best; take strictly lower costs."""
     sy, sx = F1.shape[:2]
     ch, cw = coff.shape[:2]
     yy, xx = np.mgrid[0:sy, 0:sx].astype(np.int64)
@@ -175,13 +181,16 @@ def init_proposal(F1, F2, coff, W, H, first=True, prev_off=None):
     ax, ay = qx & 255, qy & 255
     co = np.asarray(coff, np.int64)
     cands = []
-    comp = []
-    for d in range(2):
-        o00, o01, o10, o11 = co[y0, x0, d], co[y0, x1, d], co[y1, x0, d], co[y1, x1, d]
-        v = (o00 * (256 - ax) + o01 * ax) * (256 - ay) + (o10 * (256 - ax) + o11 * ax) * ay
-        comp.append([(v + (1 << 23)) >> 24, (o00 + 128) >> 8, (o01 + 128) >> 8, (o10 + 128) >> 8, (o11 + 128) >> 8])
-    for k in range(5):
-        cands.append((comp[0][k], comp[1][k]))
+    for src in range(2):
+        comp = []
+        for d in range(2):
+            e = 2 * src + d
+            o00, o01, o10, o11 = co[y0, x0, e], co[y0, x1, e], co[y1, x0, e], co[y1, x1, e]
+            v = (o00 * (256 - ax) + o01 * ax) * (256 - ay) + (o10 * (256 - ax) + o11 * ax) * ay
+            comp.append([(v + (1 << 23)) >> 24, (o00 + 128) >> 8, (o01 + 128) >> 8, (o10 + 128) >> 8,
+                         (o11 + 128) >> 8])
+        for k in range(5):
+            cands.append((comp[0][k], comp[1][k]))
     for cx, cy in cands:
         inside = (xx + cx >= 0) & (xx + cx < sx) & (yy + cy >= 0) & (yy + cy < sy)
         po = _pack(cx, cy)
@@ -253,7 +262,9 @@ def forward(w, feat, quant=True):
     """Network output [n, ch, cw, 16] (float64) and the activations.
 
     feat: BF16 bits [n, ch, cw, 128] or floats.  With quant, weights and the
-    ReLU activations are rounded to BF16 where the device kernels store them."""
+    ReLU activations are rounded to BF16 where the device kernels store them.
+    Output channels 0, 1 carry a fixed skip connection: 8 x feature channels
+    123, 124 (the lowest-cost displacement) are added to them."""
     x = bf16_float(feat).astype(np.float64) if np.asarray(feat).dtype == np.uint16 else np.asarray(feat, np.float64)
     acts = [x]
     for l, ((Wl, b), (_, _, dil, relu)) in enumerate(zip(split_params(w), LAYERS)):
@@ -265,7 +276,9 @@ def forward(w, feat, quant=True):
             if quant:
                 z = bf16_round(z)
         acts.append(z)
-    return acts[-1], acts
+    out = acts[-1].copy()
+    out[..., :2] += 8.0 * x[..., SKIP:SKIP + 2]
+    return out, acts
 
 
 def loss_and_grad(w, feat, label, quant=True):

@@ -70,15 +70,17 @@ def test_offsets_and_seeded_init_bit_exact(W, H, up, N):
     cw, ch = ctx.proposal_dims()
     g8, F, Fs, gs = _frames(ctx)
     rng = np.random.default_rng(W + N)
-    out = (rng.standard_normal((ch, cw, 16)) * 12.0).astype(np.float32)
+    out = (rng.standard_normal((ch, cw, 16)) * 1.5).astype(np.float32)
     out[0, 0, 0] = np.nan
     out[-1, -1, 1] = 1e30
     dout = bm.DeviceBuffer(array=out)
-    coff = bm.DeviceBuffer(ch * cw * 2 * 4)
-    ctx.proposal_offsets(dout, coff)
+    feat = bm.DeviceBuffer(ch * cw * 256)
+    ctx.proposal_features(g8[0], g8[1], feat)
+    coff = bm.DeviceBuffer(ch * cw * 4 * 4)
+    ctx.proposal_offsets(dout, feat, coff)
     ctx.sync()
-    got_coff = coff.download((ch, cw, 2), np.int32)
-    want_coff = onet.proposal_offsets(out, W, H, ctx.sx, ctx.sy)
+    got_coff = coff.download((ch, cw, 4), np.int32)
+    want_coff = onet.proposal_offsets(out, onet.features(gs[0], gs[1]), W, H, ctx.sx, ctx.sy)
     assert np.array_equal(got_coff, want_coff)
     M = ctx.sx * ctx.sy
     dx = rng.integers(-9, 10, (ctx.sy, ctx.sx))
@@ -155,28 +157,28 @@ def test_gradients_match_oracle(n, ch, cw):
             assert rel <= 2e-2, (l, name, rel)
 
 
-def _tiny_dataset(pairs=24, W=160, H=120):
-    return learned.build_dataset(pairs, seed0=77, W=W, H=H, up=1.0, N=64, workers=1, log=None)
-
-
 def test_training_lowers_loss_and_is_reproducible():
+    """150 Adam steps on one mini-batch of four 160x120 pairs (the network can
+    fit it: the float64 oracle network goes from 1.75 to 0.70 in the same
+    schedule) lower its loss below 70%; two runs give identical weights."""
     say("training")
-    ds = _tiny_dataset()
+    ds = learned.build_dataset(4, seed0=77, W=160, H=120, up=1.0, N=64, workers=1, log=None)
     w0 = learned.init_weights(3)
     nets = [bm.Net(weights=w0), bm.Net(weights=w0)]
     before, _ = nets[0].gradients(ds.feat, ds.label, ds.rows, ds.ch, ds.cw)
     for net in nets:
-        learned.train(net, ds, np.arange(ds.rows), epochs=40, batch=8, lr=3e-3, seed=1, log=None)
+        for _ in range(150):
+            net.train_step(ds.feat, ds.label, ds.rows, [0, 1, 2, 3], ds.ch, ds.cw, 1e-3)
     after, _ = nets[0].gradients(ds.feat, ds.label, ds.rows, ds.ch, ds.cw)
-    print("[train] loss over the set %.4f -> %.4f" % (before, after), flush=True)
-    assert after < 0.8 * before
+    print("[train] loss %.4f -> %.4f" % (before, after), flush=True)
+    assert after < 0.7 * before
     assert np.array_equal(nets[0].weights(), nets[1].weights())
 
 
 def test_learned_path_equals_composed_oracle():
     """bm_flow_pair_learned = features -> network -> offsets -> seeded init ->
-    propagation + random search -> refinement, each stage on the oracle except
-    the network (its device output is the oracle's input)."""
+    random search -> propagation + random search -> refinement, each stage on
+    the oracle except the network (its device output is the oracle's input)."""
     say("learned path c1")
     W, H, up, N = 320, 240, 1.5, 32
     p = bm.derive_params(W, H, up, up, N=N)
@@ -199,9 +201,10 @@ def test_learned_path_equals_composed_oracle():
     nout = bm.DeviceBuffer(ch * cw * 64)
     net.forward(feat, 1, ch, cw, nout)
     out = nout.download((ch, cw, 16), np.float32)
-    coff = onet.proposal_offsets(out, W, H, ctx.sx, ctx.sy)
+    coff = onet.proposal_offsets(out, onet.features(gs[0], gs[1]), W, H, ctx.sx, ctx.sy)
     oF1, oF2 = Fs[0][..., :ctx.NS].copy(), Fs[1][..., :ctx.NS].copy()
     eo, ec = onet.init_proposal(oF1, oF2, coff, W, H)
+    eo, ec = oracle.random_search(oF1, oF2, 1, 6, 11, eo, ec)     # after the seeded init: index = iterations
     eo, ec = oracle.propagate(oF1, oF2, -1, p.jfMax, True, eo, ec)
     eo, ec = oracle.random_search(oF1, oF2, 0, 6, 11, eo, ec)
     assert np.array_equal(off, eo) and np.array_equal(cost, ec)

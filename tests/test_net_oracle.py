@@ -120,7 +120,7 @@ def test_features_match_loops():
                 elif c < 123:
                     want = S[c - 121][cy][cx] / (64 * 255.0) - 0.5
                 elif c < 126:
-                    want = (best[1] / 5.0, best[2] / 5.0, min(best[0], 36864) / 18432.0 - 1.0)[c - 123]
+                    want = (best[1] / 8.0, best[2] / 8.0, min(best[0], 36864) / 18432.0 - 1.0)[c - 123]
                 else:
                     want = 0.0
                 assert abs(got[cy, cx, c] - want) <= 2 ** -8 * abs(want) + 1e-12, (cy, cx, c)
@@ -140,10 +140,14 @@ def test_proposal_offsets_known_values():
     out = np.zeros((1, 2, 16), np.float32)
     out[0, 0, :2] = [10.0, -3.0]
     out[0, 1, :2] = [np.nan, 1e30]
-    got = onet.proposal_offsets(out, W=100, H=50, sx=290, sy=145)
+    feat = np.zeros((1, 2, 128), np.float32)
+    feat[0, 0, 123:125] = [3 / 8, -5 / 8]
+    got = onet.proposal_offsets(out, onet.bf16_bits(feat), W=100, H=50, sx=290, sy=145)
     assert got[0, 0, 0] == round(10.0 * 2048 * 290 / 100)
     assert got[0, 0, 1] == round(-3.0 * 2048 * 145 / 50)
     assert got[0, 1, 0] == -256 * 290 and got[0, 1, 1] == 256 * 145
+    assert got[0, 0, 2] == round(3 * 2048 * 290 / 100) and got[0, 0, 3] == round(-5 * 2048 * 145 / 50)
+    assert got[0, 1, 2] == 0 and got[0, 1, 3] == 0
 
 
 def _init_loops(F1, F2, coff, W, H, first, prev):
@@ -169,10 +173,12 @@ def _init_loops(F1, F2, coff, W, H, first, prev):
             x1, y1 = min(x0 + 1, cw - 1), min(y0 + 1, ch - 1)
             ax, ay = (fx & 255) / 256.0, (fy & 255) / 256.0
             cells = [(y0, x0), (y0, x1), (y1, x0), (y1, x1)]
-            o = [coff[r, c].astype(np.float64) / 256.0 for r, c in cells]
-            bil = ((o[0] * (1 - ax) + o[1] * ax) * (1 - ay) + (o[2] * (1 - ax) + o[3] * ax) * ay)
-            cands = [tuple(int(np.floor(v + 0.5)) for v in bil)]
-            cands += [tuple(int(np.floor(v + 0.5)) for v in oc) for oc in o]
+            cands = []
+            for src in range(2):
+                o = [coff[r, c, 2 * src:2 * src + 2].astype(np.float64) / 256.0 for r, c in cells]
+                bil = ((o[0] * (1 - ax) + o[1] * ax) * (1 - ay) + (o[2] * (1 - ax) + o[3] * ax) * ay)
+                cands += [tuple(int(np.floor(v + 0.5)) for v in bil)]
+                cands += [tuple(int(np.floor(v + 0.5)) for v in oc) for oc in o]
             for cx, cy in cands:
                 if not (0 <= x + cx < sx and 0 <= y + cy < sy) or pk(cx, cy) == bo:
                     continue
@@ -191,8 +197,9 @@ def test_init_proposal_matches_loops(first):
     pal = rng.integers(0, 2 ** 32, (6, 2), dtype=np.uint64).astype(np.uint32)
     F1 = pal[rng.integers(0, 6, (sy, sx))]
     F2 = pal[rng.integers(0, 6, (sy, sx))]
-    coff = rng.integers(-12 * 256, 12 * 256, (H // 8, W // 8, 2)).astype(np.int32)
-    coff[0, 0] = [0, 0]
+    coff = rng.integers(-12 * 256, 12 * 256, (H // 8, W // 8, 4)).astype(np.int32)
+    coff[0, 0] = [0, 0, 0, 0]
+    coff[1, 1, 2:] = coff[1, 1, :2]                 # the two proposals agree in one cell
     prev = None
     if not first:
         dx = rng.integers(-6, 7, (sy, sx))
@@ -210,7 +217,7 @@ def test_oracle_plain_candidate_is_orc_init():
     sx, sy = 40, 30
     F1 = rng.integers(0, 2 ** 32, (sy, sx, 1), dtype=np.uint64).astype(np.uint32)
     F2 = F1.copy()                                  # zero offset is optimal (cost 0)
-    coff = rng.integers(-2000, 2000, (2, 3, 2)).astype(np.int32)
+    coff = rng.integers(-2000, 2000, (2, 3, 4)).astype(np.int32)
     off, cost = onet.init_proposal(F1, F2, coff, 24, 16)
     o2, c2 = oracle.init(F1, F2, first=True)
     assert np.array_equal(off, o2) and np.array_equal(cost, c2)
