@@ -13,6 +13,9 @@ Prints one JSON line.  Besides the driver's fields it carries:
                 launch time from HIP events on the context stream;
   stages        per-stage event times and their own roofline fractions;
   streaming     sequence mode (one BRIEF per frame, temporal init);
+  learned       the same pair through the learned-proposal path with half the
+                propagation iterations: throughput, per-stage times and EPE
+                next to the plain method's (see --no-learned);
   parity        the GPU match field against the CPU oracle on the same pair;
   cpu_baseline  the CPU oracle on the box's cores (see --no-cpu).
 
@@ -184,6 +187,41 @@ def bench_batch(a):
     print(json.dumps(res))
 
 
+def bench_learned(a, p, tuning, rgb1, rgb2, pr, W, H, plain_all, plain_noc):
+    """The pair through bm_flow_pair_learned with half the propagation
+    iterations (the committed network), timed like the plain path."""
+    import briefmatch_amd as bm
+    from briefmatch_amd import learned
+    if not os.path.exists(learned.DEFAULT_WEIGHTS):
+        return None
+    net = bm.Net(path=learned.DEFAULT_WEIGHTS)
+    pf = p.copy()
+    pf.iterations = max(1, p.iterations // 2)
+    ctx = bm.Context(pf, seed=1)
+    ctx.set_tuning(**tuning)
+    U, V = ctx.image_buffer(), ctx.image_buffer()
+    for _ in range(a.warmup):
+        ctx.flow_pair_learned(net, rgb1, rgb2, U, V)
+    ctx.sync()
+    ctx.profile_enable(True)
+    ctx.profile_read()
+    ctx.timer_start()
+    for _ in range(a.steps):
+        ctx.flow_pair_learned(net, rgb1, rgb2, U, V)
+    total_ms = ctx.timer_stop()
+    prof = ctx.profile_read()
+    ctx.profile_enable(False)
+    ms = total_ms / a.steps
+    e_all, e_noc = bm.epe(U.download((H, W), np.float32), V.download((H, W), np.float32), pr.gt, pr.occ)
+    return {"iterations": pf.iterations, "plain_iterations": p.iterations,
+            "ms_per_step": round(ms, 4), "value": round(W * H / (ms * 1e3), 3), "unit": "Mpix/s",
+            "epe": {"all": round(e_all, 4), "noc": round(e_noc, 4)},
+            "plain_epe": {"all": round(plain_all, 4), "noc": round(plain_noc, 4)},
+            "epe_ratio_vs_plain": round(e_all / plain_all, 4),
+            "stages_ms_per_step": {k: round(v[0] / a.steps, 4) for k, v in prof.items() if v[1]},
+            "weights": os.path.relpath(learned.DEFAULT_WEIGHTS, ROOT)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +234,7 @@ def main():
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--pairs", type=int, default=64, help="pairs of the batch workload")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (--gpus %d requested); the flow path shards by "
@@ -300,6 +339,8 @@ def main():
                       "unit": "Mpix/s"},
         "epe": {"all": round(e_all, 4), "noc": round(e_noc, 4)},
     }
+    if not a.no_learned:
+        out["learned"] = bench_learned(a, p, tuning, rgb1, rgb2, pr, W, H, e_all, e_noc)
     if not a.no_cpu:
         cb, parity = cpu_baseline(pr, p, ctx.pattern(), 1, off, cost, gU, gV, ctx.gauss_kernel(), a.band_rows)
         out["cpu_baseline"] = cb

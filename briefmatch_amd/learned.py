@@ -176,8 +176,10 @@ def validation_loss(net, ds, rows, batch=32):
 
 def evaluate(net, workload="c2_1920x1080_u2.9_n128", seeds=(0, 1, 2, 3), fewer=1, log=print):
     """EPE (all pixels) of plain(default iterations), plain(fewer),
-    learned(fewer) and learned(default iterations) on seeded pairs of a
-    benchmark workload."""
+    learned(fewer), learned(default iterations) and, as the method without
+    the trained network, the seeded path with an all-zero network (its
+    proposal is then the cost-volume argmin alone) with fewer iterations, on
+    seeded pairs of a benchmark workload."""
     wl = synth.WORKLOADS[workload]
     p = derive_params(wl["W"], wl["H"], wl["up"], wl["up"], N=wl["N"])
     default_it = p.iterations
@@ -185,20 +187,23 @@ def evaluate(net, workload="c2_1920x1080_u2.9_n128", seeds=(0, 1, 2, 3), fewer=1
     pf.iterations = fewer
     c_plain = Context(p, seed=0)
     c_few = Context(pf, seed=0)
-    res = {"plain": [], "plain_fewer": [], "learned_fewer": [], "learned": []}
+    zero = Net()
+    res = {"plain": [], "plain_fewer": [], "learned_fewer": [], "learned": [], "argmin_fewer": []}
     for s in seeds:
         pair = synth.workload_pair(workload, seed=s)
-        for key, ctx, learned in (("plain", c_plain, False), ("plain_fewer", c_few, False),
-                                  ("learned_fewer", c_few, True), ("learned", c_plain, True)):
-            if learned:
-                U, V = ctx.flow_numpy_learned(net, pair.frame1, pair.frame2)
+        for key, ctx, learned in (("plain", c_plain, None), ("plain_fewer", c_few, None),
+                                  ("learned_fewer", c_few, net), ("learned", c_plain, net),
+                                  ("argmin_fewer", c_few, zero)):
+            if learned is not None:
+                U, V = ctx.flow_numpy_learned(learned, pair.frame1, pair.frame2)
             else:
                 U, V = ctx.flow_numpy(pair.frame1, pair.frame2)
             res[key].append(epe(U, V, pair.gt)[0])
         if log:
-            log("[eval] seed %d: plain(%d) %.4f  plain(%d) %.4f  learned(%d) %.4f  learned(%d) %.4f"
+            log("[eval] seed %d: plain(%d) %.4f  plain(%d) %.4f  learned(%d) %.4f  learned(%d) %.4f  "
+                "argmin only(%d) %.4f"
                 % (s, default_it, res["plain"][-1], fewer, res["plain_fewer"][-1], fewer, res["learned_fewer"][-1],
-                   default_it, res["learned"][-1]))
+                   default_it, res["learned"][-1], fewer, res["argmin_fewer"][-1]))
     out = {k: float(np.mean(v)) for k, v in res.items()}
     out.update(iterations=default_it, fewer=fewer, seeds=list(seeds), workload=workload)
     return out

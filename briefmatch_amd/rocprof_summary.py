@@ -17,15 +17,16 @@ from collections import OrderedDict, defaultdict
 
 
 def short_name(name):
-    n = name.replace("bm::(anonymous namespace)::", "").replace("void ", "")
+    n = name.replace("bm::(anonymous namespace)::", "").replace("bm::net::", "").replace("void ", "")
     m = re.match(r"([A-Za-z_0-9]+(?:<[^>]*>)?)", n)
     return m.group(1) if m else n[:40]
 
 
 STAGE_OF = OrderedDict([("k_gray", "gray"), ("k_upsample", "upsample"), ("k_brief", "brief"),
-                        ("k_init", "init"), ("k_level<", "level"), ("k_random_search", "random_search"),
+                        ("k_init_proposal", "proposal"), ("k_init", "init"), ("k_level<", "level"), ("k_random_search", "random_search"),
                         ("k_downsample", "downsample"), ("k_median", "median"), ("k_trilateral", "trilateral"),
-                        ("k_copy", "copy")])
+                        ("k_copy", "copy"), ("k_features", "proposal"),
+                        ("k_block_sums", "proposal"), ("k_conv", "proposal"), ("k_proposal_offsets", "proposal")])
 
 
 def stage_of(short):
