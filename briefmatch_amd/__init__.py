@@ -93,6 +93,9 @@ def lib():
         "bm_memset": (I, [V, I, C.c_size_t]),
         "bm_gray": (I, [V, V, V, V]),
         "bm_upsample": (I, [V, V, V]),
+        "bm_upsample_lanczos": (I, [V, V, V]),
+        "bm_set_upsampler": (I, [V, I]),
+        "bm_get_upsampler": (I, [V, C.POINTER(I)]),
         "bm_brief": (I, [V, V, V]),
         "bm_brief_gray": (I, [V, V, V]),
         "bm_init": (I, [V, V, V, I, V, V, V]),
@@ -338,6 +341,18 @@ class Context:
     def upsample(self, g8, L):
         self._c(lib().bm_upsample(self.h, _ptr(g8), _ptr(L)))
 
+    def upsample_lanczos(self, g8, L):
+        self._c(lib().bm_upsample_lanczos(self.h, _ptr(g8), _ptr(L)))
+
+    def set_upsampler(self, mode):
+        """UPSAMPLE_BILINEAR (default) or UPSAMPLE_LANCZOS3 for the whole-path calls."""
+        self._c(lib().bm_set_upsampler(self.h, mode))
+
+    def upsampler(self):
+        m = C.c_int()
+        self._c(lib().bm_get_upsampler(self.h, C.byref(m)))
+        return m.value
+
     def brief(self, L, F):
         self._c(lib().bm_brief(self.h, _ptr(L), _ptr(F)))
 
@@ -492,6 +507,7 @@ class _BorrowedContext(Context):
 
 
 BATCH_PER_PAIR, BATCH_GRAPH = 0, 1
+UPSAMPLE_BILINEAR, UPSAMPLE_LANCZOS3 = 0, 1
 
 
 class Batch:

@@ -69,6 +69,11 @@ struct bm_ctx {
     uint16_t *d_cost[2] = {nullptr, nullptr};
     float *d_U0 = nullptr, *d_V0 = nullptr, *d_tU = nullptr, *d_tV = nullptr, *d_Um = nullptr, *d_Vm = nullptr;
     int field = 0;  // d_off[field] holds the last match field
+    // up-sampler of the whole-path calls (Lanczos tables and frame allocated on first use)
+    int upsampler = BM_UPSAMPLE_BILINEAR;
+    int *d_lz_x0 = nullptr, *d_lz_y0 = nullptr;
+    float *d_lz_wx = nullptr, *d_lz_wy = nullptr;
+    float *d_L = nullptr;  // up-sampled frame of the Lanczos mode
     // learned proposal (allocated on first use)
     uint16_t *d_feat = nullptr;            // [ch][cw][128] BF16
     uint16_t *d_nact[2] = {nullptr, nullptr};  // [ch][cw][64] BF16
@@ -310,6 +315,68 @@ int check(bm_ctx *ctx, const void *p, const char *name)
 {
     if (!ctx) return fail(nullptr, BM_E_PARAM, "null context");
     if (!p) return fail(ctx, BM_E_PARAM, std::string("null pointer: ") + name);
+    return BM_OK;
+}
+
+// Lanczos-3 weight table of one axis: first source index and six normalised
+// weights per output index (resizeLanczos, util_cuda.cu:133-157: xp = x *
+// float(in) / out, taps floor(xp) - 2 .. floor(xp) + 3, 3 sin(t) sin(t / 3) / t^2).
+void lanczos_table(int in, int out, std::vector<int> &first, std::vector<float> &w)
+{
+    const double pi = 3.14159265358979323846;
+    first.resize(out);
+    w.resize(6 * (size_t)out);
+    for (int x = 0; x < out; ++x) {
+        const float xp = (float)x * (float)in / (float)out;
+        const int x1 = (int)std::floor(xp);
+        double ws[6], sum = 0.0;
+        for (int k = 0; k < 6; ++k) {
+            const double t = pi * ((double)xp - (double)(x1 - 2 + k));
+            ws[k] = t == 0.0 ? 1.0 : 3.0 * std::sin(t) * std::sin(t / 3.0) / (t * t);
+            sum += ws[k];
+        }
+        first[x] = x1 - 2;
+        for (int k = 0; k < 6; ++k) w[6 * x + k] = (float)(ws[k] / sum);
+    }
+}
+
+int lanczos_ready(bm_ctx *ctx, bool frame)
+{
+    if (!ctx->d_lz_x0) {
+        std::vector<int> fx, fy;
+        std::vector<float> wx, wy;
+        lanczos_table(ctx->g.W, ctx->g.sx, fx, wx);
+        lanczos_table(ctx->g.H, ctx->g.sy, fy, wy);
+        int *x0 = nullptr, *y0 = nullptr;
+        float *dwx = nullptr, *dwy = nullptr;
+        int r = dev_alloc(ctx, &x0, fx.size());
+        if (!r) r = dev_alloc(ctx, &y0, fy.size());
+        if (!r) r = dev_alloc(ctx, &dwx, wx.size());
+        if (!r) r = dev_alloc(ctx, &dwy, wy.size());
+        if (!r) r = upload(ctx, x0, fx);
+        if (!r) r = upload(ctx, y0, fy);
+        if (!r) r = upload(ctx, dwx, wx);
+        if (!r) r = upload(ctx, dwy, wy);
+        if (r) return r;
+        ctx->g.lz_x0 = ctx->d_lz_x0 = x0;
+        ctx->g.lz_y0 = ctx->d_lz_y0 = y0;
+        ctx->g.lz_wx = ctx->d_lz_wx = dwx;
+        ctx->g.lz_wy = ctx->d_lz_wy = dwy;
+    }
+    if (frame && !ctx->d_L) return dev_alloc(ctx, &ctx->d_L, (size_t)ctx->g.sx * ctx->g.sy);
+    return BM_OK;
+}
+
+// Descriptors of one gray frame with the context's up-sampler: bilinear fused
+// into BRIEF, or Lanczos-3 into the context frame followed by BRIEF on it.
+int describe(bm_ctx *ctx, const uint8_t *g8, uint32_t *F)
+{
+    if (ctx->upsampler == BM_UPSAMPLE_LANCZOS3) {
+        BM_LAUNCH(ctx, BM_STAGE_UPSAMPLE, bm::launch_upsample_lanczos(ctx->g, g8, ctx->d_L, ctx->stream));
+        BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(ctx->g, ctx->d_L, nullptr, F, ctx->stream));
+    } else {
+        BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(ctx->g, nullptr, g8, F, ctx->stream));
+    }
     return BM_OK;
 }
 
@@ -715,6 +782,36 @@ int bm_upsample(bm_ctx *ctx, const uint8_t *g8, float *L)
     return BM_OK;
 }
 
+int bm_upsample_lanczos(bm_ctx *ctx, const uint8_t *g8, float *L)
+{
+    int r = check(ctx, g8, "gray8");
+    if (!r) r = check(ctx, L, "L");
+    if (!r) r = lanczos_ready(ctx, false);
+    if (r) return r;
+    BM_LAUNCH(ctx, BM_STAGE_UPSAMPLE, bm::launch_upsample_lanczos(ctx->g, g8, L, ctx->stream));
+    return BM_OK;
+}
+
+int bm_set_upsampler(bm_ctx *ctx, int mode)
+{
+    if (!ctx) return fail(nullptr, BM_E_PARAM, "null context");
+    if (mode != BM_UPSAMPLE_BILINEAR && mode != BM_UPSAMPLE_LANCZOS3)
+        return fail(ctx, BM_E_PARAM, "unknown up-sampler " + std::to_string(mode));
+    if (mode == BM_UPSAMPLE_LANCZOS3) {
+        int r = lanczos_ready(ctx, true);
+        if (r) return r;
+    }
+    ctx->upsampler = mode;
+    return BM_OK;
+}
+
+int bm_get_upsampler(const bm_ctx *ctx, int *mode)
+{
+    if (!ctx || !mode) return fail(nullptr, BM_E_PARAM, "null argument");
+    *mode = ctx->upsampler;
+    return BM_OK;
+}
+
 int bm_brief(bm_ctx *ctx, const float *L, uint32_t *F)
 {
     int r = check(ctx, L, "L");
@@ -876,8 +973,9 @@ int bm_flow_pair(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, uint32_t
     const bm::Geometry &g = ctx->g;
     BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb1, ctx->d_g8[0], ctx->d_g32[0], ctx->stream));
     BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb2, ctx->d_g8[1], nullptr, ctx->stream));
-    BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(g, nullptr, ctx->d_g8[0], ctx->d_F[0], ctx->stream));
-    BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(g, nullptr, ctx->d_g8[1], ctx->d_F[1], ctx->stream));
+    r = describe(ctx, ctx->d_g8[0], ctx->d_F[0]);
+    if (!r) r = describe(ctx, ctx->d_g8[1], ctx->d_F[1]);
+    if (r) return r;
     r = run_match(ctx, ctx->d_F[0], ctx->d_F[1], 1, nullptr, pair);
     if (r) return r;
     // an isolated pair breaks any running sequence
@@ -1004,8 +1102,9 @@ int bm_flow_pair_learned(bm_ctx *ctx, bm_net *net, const uint8_t *rgb1, const ui
     const int cw = g.W / bm::net::kCell, ch = g.H / bm::net::kCell;
     BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb1, ctx->d_g8[0], ctx->d_g32[0], ctx->stream));
     BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb2, ctx->d_g8[1], nullptr, ctx->stream));
-    BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(g, nullptr, ctx->d_g8[0], ctx->d_F[0], ctx->stream));
-    BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(g, nullptr, ctx->d_g8[1], ctx->d_F[1], ctx->stream));
+    r = describe(ctx, ctx->d_g8[0], ctx->d_F[0]);
+    if (!r) r = describe(ctx, ctx->d_g8[1], ctx->d_F[1]);
+    if (r) return r;
     BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, bm::net::launch_features(ctx->d_g8[0], ctx->d_g8[1], g.W, cw, ch, ctx->d_bsum,
                                                                  ctx->d_feat, ctx->stream));
     // the packed weights are written on the net's stream
@@ -1050,7 +1149,8 @@ int bm_stream_frame(bm_ctx *ctx, const uint8_t *rgb, float *U, float *V, int *ha
     const bm::Geometry &g = ctx->g;
     const int cur = ctx->cur, prev = 1 - cur;
     BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb, ctx->d_g8[cur], ctx->d_g32[cur], ctx->stream));
-    BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(g, nullptr, ctx->d_g8[cur], ctx->d_F[cur], ctx->stream));
+    r = describe(ctx, ctx->d_g8[cur], ctx->d_F[cur]);
+    if (r) return r;
     *has_flow = 0;
     if (ctx->have_prev) {
         if (!U || !V) return fail(ctx, BM_E_PARAM, "null flow output");

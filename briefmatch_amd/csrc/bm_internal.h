@@ -40,6 +40,8 @@ struct Geometry {
     const uint32_t *nn_x, *nn_y;  // nearest-neighbour source column / row per input pixel
     const float *gauss;           // P taps
     const float *tri_spatial;     // 2s entries: log2(e) i^2 / (2 sigmaS^2), i = -s..s-1
+    const int *lz_x0, *lz_y0;     // Lanczos-3: first source column / row per output column / row
+    const float *lz_wx, *lz_wy;   // Lanczos-3: six normalised weights per output column / row
     // tuning (results do not depend on it)
     int level_px;                 // pixels per thread of the level kernels: 1 or 2
     int strip_rows;               // strip height of the matching kernels' block order
@@ -51,6 +53,7 @@ int tri_max_band(int s);
 
 hipError_t launch_gray(const Geometry &g, const uint8_t *rgb, uint8_t *g8, float *g32, hipStream_t st);
 hipError_t launch_upsample(const Geometry &g, const uint8_t *g8, float *L, hipStream_t st);
+hipError_t launch_upsample_lanczos(const Geometry &g, const uint8_t *g8, float *L, hipStream_t st);
 hipError_t launch_brief(const Geometry &g, const float *L, const uint8_t *g8, uint32_t *F, hipStream_t st);
 hipError_t launch_init(const Geometry &g, const uint32_t *F1, const uint32_t *F2, int first,
                        const uint32_t *prev_off, uint32_t *off, uint16_t *cost, hipStream_t st);

@@ -175,6 +175,45 @@ __global__ void __launch_bounds__(kThreads) k_upsample(const uint8_t *__restrict
 }
 
 // ---------------------------------------------------------------------------
+// Lanczos-3 up-sampling (the reference's resizeLanczos, util_cuda.cu:126-158).
+// Its 6 x 6 weights are separable: per output column x the first source column
+// lx0[x] and six normalised weights lwx[6x..6x+5], likewise per row; the value
+// is sum_j wy_j * (sum_i wx_i * g[clamp(y0 + j)][clamp(x0 + i)]) with fused
+// multiply-adds in increasing i, j (the host oracle uses the same tables and
+// order, so the result is bit-identical).
+
+__global__ void __launch_bounds__(kThreads) k_upsample_lanczos(const uint8_t *__restrict__ g8,
+                                                               const int *__restrict__ lx0,
+                                                               const float *__restrict__ lwx,
+                                                               const int *__restrict__ ly0,
+                                                               const float *__restrict__ lwy, int W, int H,
+                                                               int sx, int sy, float *__restrict__ L)
+{
+    const int gx = (sx + kThreads - 1) / kThreads;
+    const int b = xcd_remap(blockIdx.x, gx * sy);
+    const int x = (b % gx) * kThreads + threadIdx.x, y = b / gx;
+    if (x >= sx) return;
+    const int x0 = lx0[x], y0 = ly0[y];
+    float wx[6];
+    int cx[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        wx[i] = lwx[6 * x + i];
+        cx[i] = clampi(x0 + i, 0, W - 1);
+    }
+    float v = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const uint8_t *row = g8 + (size_t)clampi(y0 + j, 0, H - 1) * W;
+        float h = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) h = __fmaf_rn(wx[i], (float)row[cx[i]], h);
+        v = __fmaf_rn(lwy[6 * y + j], h, v);
+    }
+    L[(size_t)y * sx + x] = v;
+}
+
+// ---------------------------------------------------------------------------
 // BRIEF.  A 256-thread work-group describes a 64 x 16 tile; thread (lx, ly)
 // owns pixels (lx, ly + 4k), k = 0..3, so each wave covers one image row per k.
 // The tile plus an R-pixel halo sits in LDS (clamped to the image, which is
@@ -854,6 +893,14 @@ hipError_t launch_gray(const Geometry &g, const uint8_t *rgb, uint8_t *g8, float
 {
     const int n = g.W * g.H;
     hipLaunchKernelGGL(k_gray, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, st, rgb, g8, g32, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_upsample_lanczos(const Geometry &g, const uint8_t *g8, float *L, hipStream_t st)
+{
+    if (!g.lz_x0 || !g.lz_wx || !g.lz_y0 || !g.lz_wy) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_upsample_lanczos, dim3(blocks_rows(g.sx, g.sy)), dim3(kThreads), 0, st, g8, g.lz_x0, g.lz_wx,
+                       g.lz_y0, g.lz_wy, g.W, g.H, g.sx, g.sy, L);
     return hipGetLastError();
 }
 

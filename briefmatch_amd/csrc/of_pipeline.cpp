@@ -32,7 +32,7 @@ bool has_extension(const std::string &f, const char *ext)
 OFPipeline::OFPipeline()
     : m_doWriteMOT(false), m_doWriteFLO(false), m_doWriteLDR(false), m_verbose(false), m_doPrintError(false),
       m_havePrev(false), m_endFrame(0), m_frame(0), m_stepFrame(1), m_startFrame(0), m_sx(0), m_sy(0), m_sx_in(0),
-      m_sy_in(0), m_c(0), m_seed(0), m_bm(nullptr), m_bmP(nullptr), m_timerCtx(nullptr), dev_rgb(nullptr),
+      m_sy_in(0), m_c(0), m_seed(0), m_upsampler("bilinear"), m_bm(nullptr), m_bmP(nullptr), m_timerCtx(nullptr), dev_rgb(nullptr),
       dev_g8(nullptr), m_cur(0)
 {
     dev_Lin[0] = dev_Lin[1] = dev_L[0] = dev_L[1] = nullptr;
@@ -124,6 +124,8 @@ bool OFPipeline::setParams(int argc, char *argv[])
         {"--range-spatial", "-rs", "Spatial sigma (percent of the diagonal)", Option::FLOAT, &sSRad, 0, 100},
         {"--flow-gt", "-gt", "Ground truth optical flow (.flo) for an EPE print", Option::STR, &m_flowGT, 0, 0},
         {"--seed", "-sd", "Seed of the BRIEF pattern and the random search", Option::UINT, nullptr, 0, 0},
+        {"--upsampler", "-us", "Up-sampling filter: bilinear (default) or lanczos3 (the reference's)", Option::STR,
+         &m_upsampler, 0, 0},
     };
     auto usage = [&]() {
         std::printf("briefmatch -- computes the optical flow of a sequence of frames on an AMD MI355X\n\n"
@@ -173,6 +175,8 @@ bool OFPipeline::setParams(int argc, char *argv[])
             *(float *)o->var = (float)x;
     }
     if (m_input.empty()) throw BriefMatchException("Required input option '--input' (-i) is missing.");
+    if (m_upsampler != "bilinear" && m_upsampler != "lanczos3")
+        throw BriefMatchException("Argument '--upsampler' expects bilinear or lanczos3, got '" + m_upsampler + "'");
     m_seed = seed;
     if (!m_output.empty()) {
         if (has_extension(m_output, ".flo"))
@@ -284,7 +288,10 @@ bool OFPipeline::processFrame()
     ok(bm_timer_start(ctx));
     const int cur = m_cur, prev = 1 - m_cur;
     ok(bm_gray(ctx, dev_rgb, dev_g8, dev_Lin[cur]));
-    ok(bm_upsample(ctx, dev_g8, dev_L[cur]));
+    if (m_upsampler == "lanczos3")
+        ok(bm_upsample_lanczos(ctx, dev_g8, dev_L[cur]));
+    else
+        ok(bm_upsample(ctx, dev_g8, dev_L[cur]));
     bool wrote = false;
     if (m_havePrev) {
         m_bm->match(dev_L[prev], dev_L[cur]);

@@ -121,6 +121,17 @@ int bm_gray(bm_ctx *ctx, const uint8_t *rgb_dev, uint8_t *gray8_dev, float *gray
 /* Up-sampling sx_in x sy_in gray8 -> sx x sy FP32, bilinear in fixed point
  * (replaces CUDA_resizeLanczos, util_cuda.cu:250-257). */
 int bm_upsample(bm_ctx *ctx, const uint8_t *gray8_dev, float *L_dev);
+/* Lanczos-3 up-sampling sx_in x sy_in gray8 -> sx x sy FP32, the reference's
+ * resizeLanczos (util_cuda.cu:126-158) with its 6 x 6 weights factored into
+ * per-column and per-row tables (normalised, FP32) and fused multiply-adds. */
+int bm_upsample_lanczos(bm_ctx *ctx, const uint8_t *gray8_dev, float *L_dev);
+/* Up-sampler of the whole-path calls (bm_flow_pair, bm_stream_frame,
+ * bm_flow_pair_learned, batches): bilinear fused into BRIEF (default) or
+ * Lanczos-3 into a context frame followed by BRIEF on it. */
+#define BM_UPSAMPLE_BILINEAR 0
+#define BM_UPSAMPLE_LANCZOS3 1
+int bm_set_upsampler(bm_ctx *ctx, int mode);
+int bm_get_upsampler(const bm_ctx *ctx, int *mode);
 /* BRIEF descriptors of an up-sampled FP32 frame (CUDA_binaryFeatures,
  * briefmatch_cuda.cu:311-319).  F has sx*sy*bm_descriptor_stride() words. */
 int bm_brief(bm_ctx *ctx, const float *L_dev, uint32_t *F_dev);

@@ -5,7 +5,8 @@
  *
  * Command-line options are those of OFPipeline::setParams (of_pipeline.cpp:
  * 106-126), all always available (the reference hides some behind a build
- * flag), plus --feature_length (the README's spelling) and --seed.
+ * flag), plus --feature_length (the README's spelling), --seed and
+ * --upsampler (bilinear by default; lanczos3 is the reference's filter).
  * Frames are PNG/PGM/PPM; the first frame of a range produces no flow file
  * (the reference wrote flow against uninitialised memory for it).
  */
@@ -35,6 +36,7 @@ private:
     bool m_doWriteMOT, m_doWriteFLO, m_doWriteLDR, m_verbose, m_doPrintError, m_havePrev;
     unsigned int m_endFrame, m_frame, m_stepFrame, m_startFrame, m_sx, m_sy, m_sx_in, m_sy_in, m_c;
     uint64_t m_seed;
+    std::string m_upsampler;  // "bilinear" or "lanczos3"
     std::string m_input, m_flowGT, m_output;
 
     BriefMatch *m_bm;

@@ -45,6 +45,9 @@ def lib():
         _lib.orc_philox4x32_10.argtypes = [V, V, V]
         _lib.orc_gray.argtypes = [V, C.c_int, C.c_int, V]
         _lib.orc_upsample.argtypes = [V, C.c_int, C.c_int, C.c_int, C.c_int, V]
+        _lib.orc_upsample_lanczos.argtypes = [V, C.c_int, C.c_int, C.c_int, C.c_int, V]
+        _lib.orc_lanczos_table.argtypes = [C.c_int, C.c_int, V, V]
+        _lib.orc_flow_pair_up.argtypes = [C.POINTER(Params), C.c_uint64, C.c_uint32, V, V, C.c_int, V, V, V, V]
         _lib.orc_brief.argtypes = [V, C.c_int, C.c_int, V, C.c_int, V]
         _lib.orc_init.argtypes = [V, V, C.c_int, C.c_int, C.c_int, C.c_int, V, V, V]
         _lib.orc_propagate_level.argtypes = [V, V, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -114,6 +117,22 @@ def upsample(g, sx, sy):
     out = np.zeros((sy, sx), np.float32)
     lib().orc_upsample(_p(g), W, H, sx, sy, _p(out))
     return out
+
+
+def upsample_lanczos(g, sx, sy):
+    g = np.ascontiguousarray(g, np.uint8)
+    H, W = g.shape
+    out = np.zeros((sy, sx), np.float32)
+    lib().orc_upsample_lanczos(_p(g), W, H, sx, sy, _p(out))
+    return out
+
+
+def lanczos_table(n_in, n_out):
+    """(first source index [n_out], weights [n_out, 6]) of one axis."""
+    first = np.zeros(n_out, np.int32)
+    w = np.zeros((n_out, 6), np.float32)
+    lib().orc_lanczos_table(n_in, n_out, _p(first), _p(w))
+    return first, w
 
 
 def brief(L, pattern, N):
@@ -196,15 +215,16 @@ def trilateral(Ui, Vi, Um, Vm, I, sigma_epe2, sigma_i2, sigma_s2, row0=0, row1=N
     return Uo, Vo
 
 
-def flow_pair(params, rgb1, rgb2, seed=0, pair=0):
+def flow_pair(params, rgb1, rgb2, seed=0, pair=0, lanczos=False):
     """Whole path for one isolated pair; returns (off, cost, U, V)."""
     sx, sy, W, H = params.sx, params.sy, params.sx_in, params.sy_in
     off = np.zeros((sy, sx), np.uint32)
     cost = np.zeros((sy, sx), np.uint16)
     U = np.zeros((H, W), np.float32)
     V = np.zeros((H, W), np.float32)
-    r = lib().orc_flow_pair(C.byref(params), seed, pair, _p(np.ascontiguousarray(rgb1, np.uint8)),
-                            _p(np.ascontiguousarray(rgb2, np.uint8)), _p(off), _p(cost), _p(U), _p(V))
+    r = lib().orc_flow_pair_up(C.byref(params), seed, pair, _p(np.ascontiguousarray(rgb1, np.uint8)),
+                               _p(np.ascontiguousarray(rgb2, np.uint8)), int(lanczos), _p(off), _p(cost), _p(U),
+                               _p(V))
     if r != 0:
         raise ValueError("oracle rejected the parameters")
     return off, cost, U, V

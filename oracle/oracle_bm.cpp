@@ -224,6 +224,54 @@ void orc_upsample(const uint8_t *gray, int W, int H, int sx, int sy, float *L)
     }
 }
 
+/* Lanczos-3 up-sampling, resizeLanczos (kernels/util_cuda.cu:126-158): output
+ * pixel x samples source coordinate xp = x * float(W) / sx (float, as :133-134),
+ * x1 = floor(xp), taps i = x1 - 2 .. x1 + 3 with weight 3 sin(t) sin(t / 3) / t^2,
+ * t = pi (xp - i) (1 at t = 0), sources clamped to the frame (:153), the sum
+ * divided by the sum of the weights (:157).  The weights are separable, so each
+ * axis gets a table of six weights (computed in double, normalised, then
+ * rounded to float) and the 36-tap sum is taken as
+ * sum_j wy_j * (sum_i wx_i * g[j][i]) with fused multiply-adds in increasing i
+ * and j; the device kernel uses the same tables and order. */
+void orc_lanczos_table(int in, int out, int *first, float *w)
+{
+    const double pi = 3.14159265358979323846;
+    for (int x = 0; x < out; ++x) {
+        const float xp = (float)x * (float)in / (float)out;
+        const int x1 = (int)std::floor(xp);
+        double ws[6], sum = 0.0;
+        for (int k = 0; k < 6; ++k) {
+            const double t = pi * ((double)xp - (double)(x1 - 2 + k));
+            ws[k] = t == 0.0 ? 1.0 : 3.0 * std::sin(t) * std::sin(t / 3.0) / (t * t);
+            sum += ws[k];
+        }
+        first[x] = x1 - 2;
+        for (int k = 0; k < 6; ++k) w[6 * x + k] = (float)(ws[k] / sum);
+    }
+}
+
+void orc_upsample_lanczos(const uint8_t *gray, int W, int H, int sx, int sy, float *L)
+{
+    std::vector<int> fx(sx), fy(sy);
+    std::vector<float> wx(6 * (size_t)sx), wy(6 * (size_t)sy);
+    orc_lanczos_table(W, sx, fx.data(), wx.data());
+    orc_lanczos_table(H, sy, fy.data(), wy.data());
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < sy; ++y) {
+        for (int x = 0; x < sx; ++x) {
+            float v = 0.0f;
+            for (int j = 0; j < 6; ++j) {
+                const uint8_t *row = gray + (size_t)std::min(std::max(fy[y] + j, 0), H - 1) * W;
+                float h = 0.0f;
+                for (int i = 0; i < 6; ++i)
+                    h = std::fmaf(wx[6 * (size_t)x + i], (float)row[std::min(std::max(fx[x] + i, 0), W - 1)], h);
+                v = std::fmaf(wy[6 * (size_t)y + j], h, v);
+            }
+            L[(size_t)y * sx + x] = v;
+        }
+    }
+}
+
 /* BRIEF descriptors, binaryFeatures (kernels/briefmatch_cuda.cu:82-124): bit
  * s % 32 of word s / 32 is I[p + o1] > I[p + o2] (strict), each offset clamped
  * per axis so the sample stays inside the image (:110-113).  Bits >= N are 0.
@@ -505,6 +553,13 @@ int orc_flow_pair(const orc_params *p, uint64_t seed, uint32_t pair,
                   const uint8_t *rgb1, const uint8_t *rgb2,
                   uint32_t *off_out, uint16_t *cost_out, float *U, float *V)
 {
+    return orc_flow_pair_up(p, seed, pair, rgb1, rgb2, 0, off_out, cost_out, U, V);
+}
+
+int orc_flow_pair_up(const orc_params *p, uint64_t seed, uint32_t pair,
+                     const uint8_t *rgb1, const uint8_t *rgb2, int lanczos,
+                     uint32_t *off_out, uint16_t *cost_out, float *U, float *V)
+{
     const int W = (int)p->sx_in, H = (int)p->sy_in, sx = (int)p->sx, sy = (int)p->sy;
     const int N = (int)p->N, NS = (N + 31) / 32;
     const size_t Win = (size_t)W * H, M = (size_t)sx * sy;
@@ -520,9 +575,9 @@ int orc_flow_pair(const orc_params *p, uint64_t seed, uint32_t pair,
     orc_gray(rgb2, W, H, g2.data());
     std::vector<float> L(M);
     std::vector<uint32_t> F1(M * NS), F2(M * NS);
-    orc_upsample(g1.data(), W, H, sx, sy, L.data());
+    (lanczos ? orc_upsample_lanczos : orc_upsample)(g1.data(), W, H, sx, sy, L.data());
     orc_brief(L.data(), sx, sy, pattern.data(), N, F1.data());
-    orc_upsample(g2.data(), W, H, sx, sy, L.data());
+    (lanczos ? orc_upsample_lanczos : orc_upsample)(g2.data(), W, H, sx, sy, L.data());
     orc_brief(L.data(), sx, sy, pattern.data(), N, F2.data());
     std::vector<float>().swap(L);
 

@@ -41,6 +41,10 @@ int64_t orc_rs_sigma_q16(unsigned sx);
 
 void orc_gray(const uint8_t *rgb, int W, int H, uint8_t *gray);
 void orc_upsample(const uint8_t *gray, int W, int H, int sx, int sy, float *L);
+/* Lanczos-3 (util_cuda.cu:126-158) with separable normalised float tables:
+ * first[x] = floor(x * float(in) / out) - 2, w[6x + k] the weight of tap k. */
+void orc_lanczos_table(int in, int out, int *first, float *w);
+void orc_upsample_lanczos(const uint8_t *gray, int W, int H, int sx, int sy, float *L);
 void orc_brief(const float *L, int sx, int sy, const int16_t *pattern, int N, uint32_t *F);
 void orc_init(const uint32_t *F1, const uint32_t *F2, int sx, int sy, int NS, int first,
               const uint32_t *prev_off, uint32_t *off, uint16_t *cost);
@@ -65,6 +69,10 @@ void orc_trilateral(const float *Ui, const float *Vi, const float *Um, const flo
 int  orc_flow_pair(const orc_params *p, uint64_t seed, uint32_t pair,
                    const uint8_t *rgb1, const uint8_t *rgb2,
                    uint32_t *off_out, uint16_t *cost_out, float *U, float *V);
+/* as orc_flow_pair; lanczos != 0 up-samples with orc_upsample_lanczos */
+int  orc_flow_pair_up(const orc_params *p, uint64_t seed, uint32_t pair,
+                      const uint8_t *rgb1, const uint8_t *rgb2, int lanczos,
+                      uint32_t *off_out, uint16_t *cost_out, float *U, float *V);
 void orc_epe(const float *U, const float *V, const float *gtU, const float *gtV,
              const uint8_t *occluded, int64_t n, double *epe_all, double *epe_noc);
 int  orc_num_threads(void);

@@ -61,3 +61,18 @@ def test_cli_sequence_outputs(tmp_path):
          cwd=tmp_path)
     d = np.fromfile(str(tmp_path / "raw11.bin"), np.float32)
     assert d.size == 2 * 584 * 388 and np.isfinite(d).all()
+
+
+def test_cli_lanczos_equals_flow_pair(tmp_path):
+    """--upsampler lanczos3 through OFPipeline equals bm_flow_pair with the
+    Lanczos-3 mode on the same frames."""
+    _run(["-i", os.path.join(GOLD, "frame%02d.png"), "-f", "10:11", "-ux", "2.0", "-uy", "2.0", "-fl", "64",
+          "--upsampler", "lanczos3", "-o", str(tmp_path / "flow%02d.flo")], cwd=tmp_path)
+    U, V = bio.read_flo(str(tmp_path / "flow11.flo"))
+    f10 = bio.read_image(os.path.join(GOLD, "frame10.png"))
+    f11 = bio.read_image(os.path.join(GOLD, "frame11.png"))
+    p = bm.derive_params(584, 388, 2.0, 2.0, N=64)
+    ctx = bm.Context(p, seed=0)
+    ctx.set_upsampler(bm.UPSAMPLE_LANCZOS3)
+    pU, pV = ctx.flow_numpy(f10, f11, pair=0)
+    assert np.array_equal(U, pU) and np.array_equal(V, pV)

@@ -311,3 +311,39 @@ def test_full_size_match_field(name, seed):
     e_all, e_noc = bm.epe(U, V, pr.gt, pr.occ)
     print("[epe]", name, e_all, e_noc, flush=True)
 
+
+
+@pytest.mark.parametrize("W,H,up", [(320, 240, 1.5), (61, 37, 2.3), (96, 64, 1.0), (40, 30, 4.9)])
+def test_upsample_lanczos_bit_exact(W, H, up):
+    """Lanczos-3 mode (the reference's up-sampler) against the oracle."""
+    say("lanczos", W, H, up)
+    p = bm.derive_params(W, H, up, up, N=64)
+    ctx = bm.Context(p, seed=1)
+    g = np.random.default_rng(W).integers(0, 256, (H, W)).astype(np.uint8)
+    g8 = bm.DeviceBuffer(array=g)
+    L = bm.DeviceBuffer(ctx.sx * ctx.sy * 4)
+    ctx.upsample_lanczos(g8, L)
+    ctx.sync()
+    assert np.array_equal(L.download((ctx.sy, ctx.sx), np.float32), oracle.upsample_lanczos(g, ctx.sx, ctx.sy))
+
+
+def test_flow_pair_lanczos_c1():
+    """The whole pair with the Lanczos-3 up-sampler: match field bit-exact,
+    flow within 1e-3 px of the oracle run with the same up-sampler."""
+    say("flow_pair c1 lanczos")
+    pr = synth.workload_pair("c1_320x240_u1.5_n32", seed=1)
+    p = bm.derive_params(320, 240, 1.5, 1.5, N=32)
+    ctx = bm.Context(p, seed=7)
+    ctx.set_upsampler(bm.UPSAMPLE_LANCZOS3)
+    assert ctx.upsampler() == bm.UPSAMPLE_LANCZOS3
+    U, V = ctx.flow_numpy(pr.frame1, pr.frame2, pair=3)
+    off, cost = ctx.field()
+    eo, ec, oU, oV = oracle.flow_pair(_oracle_params(p), pr.frame1, pr.frame2, seed=7, pair=3, lanczos=True)
+    assert np.array_equal(off, eo) and np.array_equal(cost, ec)
+    assert np.abs(U - oU).max() <= FLOW_TOL and np.abs(V - oV).max() <= FLOW_TOL
+    e_l, _ = bm.epe(U, V, pr.gt, pr.occ)
+    bctx = bm.Context(p, seed=7)
+    bU, bV = bctx.flow_numpy(pr.frame1, pr.frame2, pair=3)
+    e_b, _ = bm.epe(bU, bV, pr.gt, pr.occ)
+    print("[epe] c1 lanczos %.4f bilinear %.4f" % (e_l, e_b), flush=True)
+    assert e_l < 1.5 * e_b and e_b < 1.5 * e_l

@@ -114,6 +114,7 @@ def test_cli_parameter_errors(tmp_path):
         (["-i", frames, "-f", "10:11", "--feature-length", "600"], "out of range"),
         (["-i", frames, "-f", "10:11", "-fl"], "No value provided"),
         (["-i", str(tmp_path / "none%02d.png"), "-f", "1:2"], "Cannot read image file"),
+        (["-i", frames, "-f", "10:11", "--upsampler", "cubic"], "expects bilinear or lanczos3"),
     ]
     for args, msg in cases:
         r = _cli(*args)

@@ -112,6 +112,51 @@ def test_upsample_properties():
     assert np.array_equal(L[inner], ex[inner].astype(np.float32))
 
 
+def _lanczos_formula(g, sx, sy):
+    """The reference's 36-tap sum v / w (util_cuda.cu:126-158) in float64,
+    non-separable, with the source coordinates computed in float as there."""
+    H, W = g.shape
+    out = np.zeros((sy, sx))
+
+    def lz(t):
+        return 1.0 if t == 0 else 3.0 * np.sin(t) * np.sin(t / 3.0) / (t * t)
+    for y in range(sy):
+        yp = float(np.float32(y) * np.float32(H) / np.float32(sy))
+        y1 = int(np.floor(yp))
+        for x in range(sx):
+            xp = float(np.float32(x) * np.float32(W) / np.float32(sx))
+            x1 = int(np.floor(xp))
+            v = w = 0.0
+            for i in range(x1 - 2, x1 + 4):
+                Lx = lz(np.pi * (xp - i))
+                for j in range(y1 - 2, y1 + 4):
+                    Ly = lz(np.pi * (yp - j))
+                    w += Lx * Ly
+                    v += float(g[min(max(j, 0), H - 1), min(max(i, 0), W - 1)]) * Lx * Ly
+            out[y, x] = v / w
+    return out
+
+
+@pytest.mark.parametrize("W,H,sx,sy", [(9, 7, 23, 17), (12, 10, 12, 29), (6, 5, 35, 5)])
+def test_lanczos_matches_reference_formula(W, H, sx, sy):
+    g = np.random.default_rng(W).integers(0, 256, (H, W)).astype(np.uint8)
+    got = oracle.upsample_lanczos(g, sx, sy)
+    assert np.abs(got - _lanczos_formula(g, sx, sy)).max() <= 2e-4
+
+
+def test_lanczos_tables_and_properties():
+    first, w = oracle.lanczos_table(17, 49)
+    xp = np.arange(49, dtype=np.float32) * np.float32(17) / np.float32(49)
+    assert np.array_equal(first, np.floor(xp).astype(np.int32) - 2)
+    assert np.abs(w.astype(np.float64).sum(1) - 1.0).max() <= 1e-6
+    rng = np.random.default_rng(2)
+    g = np.full((13, 17), 77, np.uint8)
+    assert np.abs(oracle.upsample_lanczos(g, 40, 31) - 77.0).max() <= 1e-4
+    g = rng.integers(0, 256, (13, 17)).astype(np.uint8)
+    assert np.abs(oracle.upsample_lanczos(g, 17, 13) - g).max() <= 1e-4        # factor 1: interpolating
+    assert np.abs(oracle.upsample_lanczos(g, 34, 26)[::2, ::2] - g).max() <= 1e-4
+
+
 # --------------------------------------------------------------- BRIEF
 
 def _tiny_image(W=11, H=9, seed=0):
