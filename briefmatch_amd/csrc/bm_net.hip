@@ -118,17 +118,22 @@ __global__ void __launch_bounds__(128) k_features(const int32_t *__restrict__ S,
 // and B[4 (l / 16) + j][l % 16], j = 0..3, and receives D[4 (l / 16) + j][l % 16].
 // A = activations (pixel x channel), B = weights (channel x output channel).
 
-template <int COUT, int PT>
+// SPLIT: the four waves of a work-group share 16 * PT pixels and each takes a
+// quarter of the output-channel tiles (four times the waves for small maps);
+// otherwise each wave owns its own pixels and all output channels.
+template <int COUT, int PT, bool SPLIT>
 __global__ void __launch_bounds__(256) k_conv(const uint16_t *__restrict__ in, int cin,
                                               const uint16_t *__restrict__ w, const float *__restrict__ bias,
                                               int npix, int H, int W, int dil, int act,
                                               const uint16_t *__restrict__ ref, const uint16_t *__restrict__ skip,
                                               uint16_t *__restrict__ out_bf, float *__restrict__ out_f)
 {
-    constexpr int CT = COUT / 16;
+    static_assert(!SPLIT || COUT % 64 == 0, "split needs four output-channel tiles per wave group");
+    constexpr int CT = SPLIT ? COUT / 64 : COUT / 16;  // output-channel tiles per wave
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m = lane & 15, kq = lane >> 4;
-    const int p0 = (blockIdx.x * 4 + wave) * 16 * PT;
+    const int p0 = (SPLIT ? blockIdx.x : blockIdx.x * 4 + wave) * 16 * PT;
+    const int ct0 = SPLIT ? wave * CT : 0;  // first output-channel tile of this wave
     const int hw = H * W;
     int py[PT], px[PT], pb[PT];
 #pragma unroll
@@ -154,7 +159,7 @@ __global__ void __launch_bounds__(256) k_conv(const uint16_t *__restrict__ in, i
             inb[t] = ty >= 0 && ty < H && tx >= 0 && tx < W;
             src[t] = in + (size_t)(inb[t] ? pb[t] + ty * W + tx : 0) * cin + 4 * kq;
         }
-        const uint16_t *wt = w + (size_t)tap * COUT * cin + (size_t)m * cin + 4 * kq;
+        const uint16_t *wt = w + (size_t)tap * COUT * cin + (size_t)(16 * ct0 + m) * cin + 4 * kq;
         for (int c0 = 0; c0 < cin; c0 += 16) {
             v4s a[PT];
 #pragma unroll
@@ -177,7 +182,7 @@ __global__ void __launch_bounds__(256) k_conv(const uint16_t *__restrict__ in, i
         for (int u = 0; u < CT; ++u)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int q = p0 + 16 * t + 4 * kq + j, co = 16 * u + m;
+                const int q = p0 + 16 * t + 4 * kq + j, co = 16 * (ct0 + u) + m;
                 if (q >= npix) continue;
                 float v = acc[t][u][j] + (bias ? bias[co] : 0.0f);
                 if (skip && co < 2) v += 8.0f * bf2f(skip[(size_t)q * kFeatC + kSkipChannel + co]);
@@ -206,7 +211,8 @@ __global__ void __launch_bounds__(256) k_wgrad(const uint16_t *__restrict__ dz, 
     __shared__ __attribute__((aligned(16))) uint16_t sdz[COUT * kWgKP];
     __shared__ __attribute__((aligned(16))) uint16_t sxx[CIN * kWgKP];
     constexpr int TILES = (COUT / 16) * (CIN / 16);
-    constexpr int TPW = (TILES + 3) / 4;  // tiles per wave
+    constexpr int TPW = TILES / 4;  // tiles per wave
+    static_assert(TILES % 4 == 0 && (CIN / 16) % TPW == 0, "each wave's tiles lie in one output-channel block");
     const int chunk = blockIdx.x, tap = blockIdx.y;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m = lane & 15, kq = lane >> 4;
@@ -219,33 +225,40 @@ __global__ void __launch_bounds__(256) k_wgrad(const uint16_t *__restrict__ dz, 
     const int pbeg = chunk * kWgChunk, pend = min(npix, pbeg + kWgChunk);
     for (int s0 = pbeg; s0 < pend; s0 += kWgPix) {
         __syncthreads();
-        for (int i = threadIdx.x; i < kWgPix * COUT; i += 256) {
-            const int k = i / COUT, co = i % COUT, pp = s0 + k;
-            sdz[co * kWgKP + k] = pp < pend ? dz[(size_t)pp * COUT + co] : (uint16_t)0;
+        // four channels (8 bytes) per load, transposed into LDS
+#pragma unroll 2
+        for (int i = threadIdx.x; i < kWgPix * COUT / 4; i += 256) {
+            const int k = i / (COUT / 4), c4 = 4 * (i % (COUT / 4)), pp = s0 + k;
+            v4s v = {0, 0, 0, 0};
+            if (pp < pend) v = *reinterpret_cast<const v4s *>(dz + (size_t)pp * COUT + c4);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) sdz[(c4 + j) * kWgKP + k] = (uint16_t)v[j];
         }
-        for (int i = threadIdx.x; i < kWgPix * CIN; i += 256) {
-            const int k = i / CIN, ci = i % CIN, pp = s0 + k;
-            uint16_t v = 0;
+#pragma unroll 2
+        for (int i = threadIdx.x; i < kWgPix * CIN / 4; i += 256) {
+            const int k = i / (CIN / 4), c4 = 4 * (i % (CIN / 4)), pp = s0 + k;
+            v4s v = {0, 0, 0, 0};
             if (pp < pend) {
                 const int n = pp / hw, rem = pp - n * hw, yy = rem / W + oy, xx = rem % W + ox;
-                if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = x[((size_t)n * hw + yy * W + xx) * CIN + ci];
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                    v = *reinterpret_cast<const v4s *>(x + ((size_t)n * hw + yy * W + xx) * CIN + c4);
             }
-            sxx[ci * kWgKP + k] = v;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) sxx[(c4 + j) * kWgKP + k] = (uint16_t)v[j];
         }
         __syncthreads();
         if (tap == 0 && threadIdx.x < COUT)
             for (int k = 0; k < kWgPix; ++k) bsum += bf2f(sdz[threadIdx.x * kWgKP + k]);
-#pragma unroll
+        // a wave's tiles share one output-channel block: one A fragment per K step
+        const int ct = wave * TPW / (CIN / 16);
+#pragma unroll 1
         for (int ks = 0; ks < kWgPix; ks += 16) {
+            const v4s a = *reinterpret_cast<const v4s *>(&sdz[(16 * ct + m) * kWgKP + ks + 4 * kq]);
 #pragma unroll
             for (int i = 0; i < TPW; ++i) {
-                const int t = wave * TPW + i;
-                if (t < TILES) {
-                    const int ct = t / (CIN / 16), it = t % (CIN / 16);
-                    const v4s a = *reinterpret_cast<const v4s *>(&sdz[(16 * ct + m) * kWgKP + ks + 4 * kq]);
-                    const v4s b = *reinterpret_cast<const v4s *>(&sxx[(16 * it + m) * kWgKP + ks + 4 * kq]);
-                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i], 0, 0, 0);
-                }
+                const int it = (wave * TPW + i) % (CIN / 16);
+                const v4s b = *reinterpret_cast<const v4s *>(&sxx[(16 * it + m) * kWgKP + ks + 4 * kq]);
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i], 0, 0, 0);
             }
         }
     }
@@ -253,7 +266,6 @@ __global__ void __launch_bounds__(256) k_wgrad(const uint16_t *__restrict__ dz, 
 #pragma unroll
     for (int i = 0; i < TPW; ++i) {
         const int t = wave * TPW + i;
-        if (t >= TILES) break;
         const int ct = t / (CIN / 16), it = t % (CIN / 16);
 #pragma unroll
         for (int j = 0; j < 4; ++j) dst[(size_t)(16 * ct + 4 * kq + j) * CIN + 16 * it + m] = acc[i][j];
@@ -446,16 +458,26 @@ __global__ void __launch_bounds__(256) k_init_proposal(const uint32_t *__restric
             cand[5 * src + 3][d] = (int)((o10 + 128) >> 8);
             cand[5 * src + 4][d] = (int)((o11 + 128) >> 8);
         }
+    // a candidate equal to the plain one or to an earlier candidate has the same
+    // cost and cannot be strictly cheaper than the running best: no gather
+    uint32_t seen[10];
+    bool ok[10];
 #pragma unroll
     for (int k = 0; k < 10; ++k) {
         const int cx = cand[k][0], cy = cand[k][1];
-        if (x + cx < 0 || x + cx >= sx || y + cy < 0 || y + cy >= sy) continue;
-        const uint32_t po = packo(cx, cy);
-        if (po == bo) continue;
-        const int c = ham<NSP>(f1, F2 + (size_t)(i + cy * sx + cx) * NSP);
+        seen[k] = packo(cx, cy);
+        bool fresh = x + cx >= 0 && x + cx < sx && y + cy >= 0 && y + cy < sy && seen[k] != bo;
+#pragma unroll
+        for (int e = 0; e < k; ++e) fresh = fresh && !(ok[e] && seen[e] == seen[k]);
+        ok[k] = fresh;
+    }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+        if (!ok[k]) continue;
+        const int c = ham<NSP>(f1, F2 + (size_t)(i + cand[k][1] * sx + cand[k][0]) * NSP);
         if (c < best) {
             best = c;
-            bo = po;
+            bo = seen[k];
         }
     }
     off[i] = bo;
@@ -494,13 +516,13 @@ hipError_t launch_features(const uint8_t *g1, const uint8_t *g2, int W, int cw, 
     return hipGetLastError();
 }
 
-template <int COUT, int PT>
+template <int COUT, int PT, bool SPLIT>
 static void conv_t(const uint16_t *in, int cin, const uint16_t *w, const float *bias, int npix, int H, int W, int dil,
                    int act, const uint16_t *ref, const uint16_t *skip, uint16_t *out_bf, float *out_f, hipStream_t st)
 {
-    const int per_block = 4 * 16 * PT;
-    hipLaunchKernelGGL((k_conv<COUT, PT>), dim3((npix + per_block - 1) / per_block), dim3(256), 0, st, in, cin, w,
-                       bias, npix, H, W, dil, act, ref, skip, out_bf, out_f);
+    const int per_block = (SPLIT ? 1 : 4) * 16 * PT;
+    hipLaunchKernelGGL((k_conv<COUT, PT, SPLIT>), dim3((npix + per_block - 1) / per_block), dim3(256), 0, st, in, cin,
+                       w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f);
 }
 
 hipError_t launch_conv(int cout, const uint16_t *in, int cin, const uint16_t *w, const float *bias, int npix, int H,
@@ -508,16 +530,19 @@ hipError_t launch_conv(int cout, const uint16_t *in, int cin, const uint16_t *w,
                        float *out_f, hipStream_t st)
 {
     if (cin % 16 || npix <= 0) return hipErrorInvalidValue;
-    // enough work-groups to cover the chip before blocking pixels per wave
+    // enough waves to cover the chip before blocking pixels per wave; below
+    // 16384 pixels the four waves of a group split the output channels
+    // (measured: 4,800 pixels 31 -> 23 us, 32,400 pixels 55 -> 83 us)
     const int pt = npix >= 4 * 64 * 1024 ? 4 : (npix >= 2 * 64 * 512 ? 2 : 1);
     if (cout == 64) {
-        if (pt == 4) conv_t<64, 4>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
-        else if (pt == 2) conv_t<64, 2>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
-        else conv_t<64, 1>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
+        if (pt == 4) conv_t<64, 4, false>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
+        else if (pt == 2) conv_t<64, 2, false>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
+        else if (npix >= 16384) conv_t<64, 1, false>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
+        else conv_t<64, 1, true>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
     } else if (cout == 16) {
-        if (pt == 4) conv_t<16, 4>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
-        else if (pt == 2) conv_t<16, 2>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
-        else conv_t<16, 1>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
+        if (pt == 4) conv_t<16, 4, false>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
+        else if (pt == 2) conv_t<16, 2, false>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
+        else conv_t<16, 1, false>(in, cin, w, bias, npix, H, W, dil, act, ref, skip, out_bf, out_f, st);
     } else {
         return hipErrorInvalidValue;
     }
