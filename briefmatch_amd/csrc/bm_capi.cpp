@@ -1268,8 +1268,9 @@ namespace {
 
 int tuning_valid(const bm_ctx *ctx, const bm_tuning &t, std::string &why)
 {
-    if (t.level_px != 1 && t.level_px != 2) return why = "level_px must be 1 or 2", 0;
+    if (t.level_px != 1 && t.level_px != 2 && t.level_px != 4) return why = "level_px must be 1, 2 or 4", 0;
     if (t.level_px == 2 && ctx->g.NSP > 8) return why = "level_px 2 needs at most 256 feature bits", 0;
+    if (t.level_px == 4 && ctx->g.NSP > 4) return why = "level_px 4 needs at most 128 feature bits", 0;
     if (t.strip_rows < 16 || t.strip_rows > 4096) return why = "strip_rows outside [16, 4096]", 0;
     if (!(t.brief_near == 0 || t.brief_near == 8 || t.brief_near == 16 || t.brief_near == 24 || t.brief_near == 32))
         return why = "brief_near must be 0, 8, 16, 24 or 32", 0;
@@ -1401,7 +1402,7 @@ int bm_autotune(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, const cha
     for (int v : {4, 8, 12, 16})
         if (v <= maxb) bands.push_back(v);
     const Knob knobs[] = {{&bm_tuning::brief_near, {16, 24, 32}, br},
-                          {&bm_tuning::level_px, {1, 2}, lv},
+                          {&bm_tuning::level_px, {1, 2, 4}, lv},
                           {&bm_tuning::strip_rows, {64, 128, 256, 512, 1024}, lv},
                           {&bm_tuning::tri_band, bands, tr}};
     int e = BM_OK;

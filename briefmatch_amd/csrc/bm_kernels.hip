@@ -769,6 +769,9 @@ __global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict
 {
     extern __shared__ float4 band[];
     const int pitch = kTriTW + 2 * s;
+    // the column part of the spatial weight (2s entries) after the band
+    float *spl = reinterpret_cast<float *>(band + (size_t)BR * pitch);
+    for (int i = threadIdx.x; i < 2 * s; i += kThreads) spl[i] = spt[i];
     const int gxb = (W + kTriTW - 1) / kTriTW, gyb = (H + kTriTH - 1) / kTriTH;
     const int blk = xcd_remap(blockIdx.x, gxb * gyb);
     const int tx0 = (blk % gxb) * kTriTW, ty0 = (blk / gxb) * kTriTH;
@@ -806,20 +809,32 @@ __global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict
             float ru[kTriPY], rv[kTriPY], rw[kTriPY];
 #pragma unroll
             for (int k = 0; k < kTriPY; ++k) ru[k] = rv[k] = rw[k] = 0.0f;
+            // two taps per step: their sources and spatial terms are read from
+            // LDS together (2s is even), one wait per pair of taps
             const float4 *src = band + r * pitch + lx;
-            for (int i = 0; i < 2 * s; ++i) {
-                const float4 q = src[i];
-                const float sp = spt[i];
+            for (int i = 0; i < 2 * s; i += 2) {
+                const float4 qa = src[i], qb = src[i + 1];
+                const float2 sp = *reinterpret_cast<const float2 *>(spl + i);
 #pragma unroll
                 for (int k = 0; k < kTriPY; ++k) {
-                    const float du = q.x - um[k], dv = q.y - vm[k], dg = q.z - g0[k];
-                    float t = fmaf(du, du, sp);
+                    float du = qa.x - um[k], dv = qa.y - vm[k], dg = qa.z - g0[k];
+                    float t = fmaf(du, du, sp.x);
                     t = fmaf(dv, dv, t);
                     t = fmaf(dg, dg, t);
-                    const float w = __builtin_amdgcn_exp2f(-t);
+                    float w = __builtin_amdgcn_exp2f(-t);
                     rw[k] += w;
-                    ru[k] = fmaf(w, q.x, ru[k]);
-                    rv[k] = fmaf(w, q.y, rv[k]);
+                    ru[k] = fmaf(w, qa.x, ru[k]);
+                    rv[k] = fmaf(w, qa.y, rv[k]);
+                    du = qb.x - um[k];
+                    dv = qb.y - vm[k];
+                    dg = qb.z - g0[k];
+                    t = fmaf(du, du, sp.y);
+                    t = fmaf(dv, dv, t);
+                    t = fmaf(dg, dg, t);
+                    w = __builtin_amdgcn_exp2f(-t);
+                    rw[k] += w;
+                    ru[k] = fmaf(w, qb.x, ru[k]);
+                    rv[k] = fmaf(w, qb.y, rv[k]);
                 }
             }
 #pragma unroll
@@ -877,7 +892,7 @@ int brief_tile_rows(int NSP) { return NSP <= 4 ? 32 : 16; }
 
 int tri_max_band(int s)
 {
-    const int BR = kTriLdsBytes / ((kTriTW + 2 * s) * (int)sizeof(float4));
+    const int BR = (kTriLdsBytes - 2 * s * (int)sizeof(float)) / ((kTriTW + 2 * s) * (int)sizeof(float4));
     return BR < 16 ? BR : 16;
 }
 
@@ -1012,15 +1027,30 @@ static void launch_level_t(const Geometry &g, const uint32_t *F1, const uint32_t
                        cost_out, g.sx, g.sy, p, pd, g.doubleProp, g.strip_rows, rs);
 }
 
+// pixels per thread: 4 only for descriptors of at most 128 bits (registers)
+template <int NSP, bool RSV>
+static void level_px_t(const Geometry &g, const uint32_t *F1, const uint32_t *F2, const uint32_t *off_in,
+                       const uint16_t *cost_in, uint32_t *off_out, uint16_t *cost_out, int p, int pd, const RSArgs &rs,
+                       hipStream_t st)
+{
+    if constexpr (NSP <= 4) {
+        if (g.level_px == 4) {
+            launch_level_t<NSP, RSV, 4>(g, F1, F2, off_in, cost_in, off_out, cost_out, p, pd, rs, st);
+            return;
+        }
+    }
+    if (g.level_px == 2)
+        launch_level_t<NSP, RSV, 2>(g, F1, F2, off_in, cost_in, off_out, cost_out, p, pd, rs, st);
+    else
+        launch_level_t<NSP, RSV, 1>(g, F1, F2, off_in, cost_in, off_out, cost_out, p, pd, rs, st);
+}
+
 hipError_t launch_level(const Geometry &g, const uint32_t *F1, const uint32_t *F2, const uint32_t *off_in,
                         const uint16_t *cost_in, uint32_t *off_out, uint16_t *cost_out, int p, int pd,
                         int rs_iteration, uint32_t pair, hipStream_t st)
 {
     const RSArgs rs = make_rs(g, rs_iteration >= 0, (uint32_t)(rs_iteration < 0 ? 0 : rs_iteration), pair);
-    const bool two = g.level_px == 2;
-#define BM_LEVEL_CALL(RSV)                                                                                   \
-    (two ? launch_level_t<NSP, RSV, 2>(g, F1, F2, off_in, cost_in, off_out, cost_out, p, pd, rs, st)         \
-         : launch_level_t<NSP, RSV, 1>(g, F1, F2, off_in, cost_in, off_out, cost_out, p, pd, rs, st))
+#define BM_LEVEL_CALL(RSV) level_px_t<NSP, RSV>(g, F1, F2, off_in, cost_in, off_out, cost_out, p, pd, rs, st)
     if (rs.enabled) {
         BM_NSP_DISPATCH(g.NSP, BM_LEVEL_CALL(true));
     } else {
@@ -1076,10 +1106,10 @@ hipError_t launch_trilateral(const Geometry &g, const float *Ui, const float *Vi
 {
     const int s = g.tri_s;
     const int pitch = kTriTW + 2 * s;
-    int BR = kTriLdsBytes / (pitch * (int)sizeof(float4));
+    int BR = (kTriLdsBytes - 2 * s * (int)sizeof(float)) / (pitch * (int)sizeof(float4));
     if (BR < 1) return hipErrorInvalidValue;
     if (BR > g.tri_band) BR = g.tri_band;
-    const size_t lds = (size_t)BR * pitch * sizeof(float4);
+    const size_t lds = (size_t)BR * pitch * sizeof(float4) + (size_t)2 * s * sizeof(float);
     const int gxb = (g.W + kTriTW - 1) / kTriTW, gyb = (g.H + kTriTH - 1) / kTriTH;
     hipLaunchKernelGGL(k_trilateral, dim3(gxb * gyb), dim3(kThreads), lds, st, Ui, Vi, Um, Vm, I, Uo, Vo, g.W, g.H,
                        s, BR, g.tri_a, g.tri_b, g.tri_spatial);

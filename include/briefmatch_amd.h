@@ -197,7 +197,7 @@ int bm_stream_reset(bm_ctx *ctx);
 /* Launch-shape choices of one context.  Every valid tuning computes identical
  * results; only the speed changes. */
 typedef struct bm_tuning {
-    int level_px;     /* pixels per thread of the jump-flooding kernels: 1 or 2 (2 needs N <= 256) */
+    int level_px;     /* pixels per thread of the jump-flooding kernels: 1, 2 (N <= 256) or 4 (N <= 128) */
     int strip_rows;   /* rows per strip of the matching kernels' work-group order: 16..4096 */
     int brief_near;   /* LDS near halo of the fused BRIEF: 8, 16, 24, 32; 0 = kernel with global far taps */
     int tri_band;     /* source rows per LDS band of the trilateral filter: 1..(LDS limit, <= 16) */

@@ -30,7 +30,7 @@ def test_tuning_does_not_change_results(N):
     ctx = bm.Context(p, seed=3)
     ref = _flow(ctx, pr)
     t0 = ctx.tuning()
-    px = (1, 2) if N <= 256 else (1,)
+    px = (1, 2, 4) if N <= 128 else ((1, 2) if N <= 256 else (1,))
     for lp, strip, near, band in itertools.product(px, (16, 256), (0, 8, 32), (1, t0.tri_band)):
         ctx.set_tuning(level_px=lp, strip_rows=strip, brief_near=near, tri_band=band)
         got = _flow(ctx, pr)
@@ -41,7 +41,8 @@ def test_tuning_does_not_change_results(N):
 def test_tuning_rejects_invalid():
     p = bm.derive_params(64, 48, 2.0, 2.0, N=512)
     ctx = bm.Context(p)
-    for kw in ({"level_px": 3}, {"level_px": 2}, {"strip_rows": 8}, {"brief_near": 12}, {"tri_band": 0}):
+    for kw in ({"level_px": 3}, {"level_px": 2}, {"level_px": 4}, {"strip_rows": 8}, {"brief_near": 12},
+               {"tri_band": 0}):
         with pytest.raises(bm.BriefMatchError):
             ctx.set_tuning(**kw)
 
@@ -55,7 +56,7 @@ def test_autotune_and_cache(tmp_path):
     r1, r2 = bm.DeviceBuffer(array=pr.frame1), bm.DeviceBuffer(array=pr.frame2)
     cache = str(tmp_path / "tune.txt")
     t = ctx.autotune(r1, r2, cache)
-    assert t.level_px in (1, 2) and 16 <= t.strip_rows <= 4096
+    assert t.level_px in (1, 2, 4) and 16 <= t.strip_rows <= 4096
     assert len(open(cache).read().splitlines()) == 1
     ctx2 = bm.Context(p, seed=1)
     t2 = ctx2.autotune(r1, r2, cache)                  # served from the cache
