@@ -215,6 +215,22 @@ void make_pattern(uint64_t seed, unsigned N, float patchArea, std::vector<int16_
 // the smallest of 8/16/24/32 covering min(far, 32); the gray tile bound is the
 // widest input-resolution extent of any tile.  The kernel is used when one
 // work-group's LDS fits in 64 KiB.
+// LDS bytes one work-group may use on the current device (160 KiB on gfx950),
+// queried once; 64 KiB if the query fails.
+size_t device_lds_limit()
+{
+    static size_t limit = 0;
+    if (limit == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            limit = (size_t)v;
+        else
+            limit = 64 * 1024;
+    }
+    return limit;
+}
+
 bool plan_brief(bm_ctx *ctx, int rmax, int rn_req)
 {
     bm::Geometry &g = ctx->g;
@@ -242,7 +258,7 @@ bool plan_brief(bm_ctx *ctx, int rmax, int rn_req)
     g.brief_gw[1 - which] = g.brief_gh[1 - which] = 0;
     if (rn_req == 0) return true;  // stand-alone kernel requested
     if (gw >= 4096 || (size_t)gh * gw >= 32768) return false;
-    if (bm::brief_lds_bytes(g, rn, TH) > 64 * 1024) return false;
+    if (bm::brief_lds_bytes(g, rn, TH) > device_lds_limit()) return false;
     g.brief_near = rn;
     return true;
 }
@@ -1272,8 +1288,8 @@ int tuning_valid(const bm_ctx *ctx, const bm_tuning &t, std::string &why)
     if (t.level_px == 2 && ctx->g.NSP > 8) return why = "level_px 2 needs at most 256 feature bits", 0;
     if (t.level_px == 4 && ctx->g.NSP > 4) return why = "level_px 4 needs at most 128 feature bits", 0;
     if (t.strip_rows < 16 || t.strip_rows > 4096) return why = "strip_rows outside [16, 4096]", 0;
-    if (!(t.brief_near == 0 || t.brief_near == 8 || t.brief_near == 16 || t.brief_near == 24 || t.brief_near == 32))
-        return why = "brief_near must be 0, 8, 16, 24 or 32", 0;
+    if (!(t.brief_near == 0 || (t.brief_near % 8 == 0 && t.brief_near >= 8 && t.brief_near <= 48)))
+        return why = "brief_near must be 0, 8, 16, 24, 32, 40 or 48", 0;
     const int maxb = ctx->p.filter ? bm::tri_max_band(ctx->g.tri_s) : 16;
     if (t.tri_band < 1 || t.tri_band > std::max(1, maxb)) return why = "tri_band outside the LDS limit", 0;
     return 1;
@@ -1401,7 +1417,7 @@ int bm_autotune(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, const cha
     const int maxb = ctx->p.filter ? bm::tri_max_band(ctx->g.tri_s) : 0;
     for (int v : {4, 8, 12, 16})
         if (v <= maxb) bands.push_back(v);
-    const Knob knobs[] = {{&bm_tuning::brief_near, {16, 24, 32}, br},
+    const Knob knobs[] = {{&bm_tuning::brief_near, {16, 24, 32, 40, 48}, br},
                           {&bm_tuning::level_px, {1, 2, 4}, lv},
                           {&bm_tuning::strip_rows, {64, 128, 256, 512, 1024}, lv},
                           {&bm_tuning::tri_band, bands, tr}};

@@ -958,6 +958,8 @@ static hipError_t launch_brief_lds_n(const Geometry &g, const uint8_t *g8, uint3
     case 16: return launch_brief_lds_t<NSP, 16>(g, g8, F, st);
     case 24: return launch_brief_lds_t<NSP, 24>(g, g8, F, st);
     case 32: return launch_brief_lds_t<NSP, 32>(g, g8, F, st);
+    case 40: return launch_brief_lds_t<NSP, 40>(g, g8, F, st);
+    case 48: return launch_brief_lds_t<NSP, 48>(g, g8, F, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -199,7 +199,8 @@ int bm_stream_reset(bm_ctx *ctx);
 typedef struct bm_tuning {
     int level_px;     /* pixels per thread of the jump-flooding kernels: 1, 2 (N <= 256) or 4 (N <= 128) */
     int strip_rows;   /* rows per strip of the matching kernels' work-group order: 16..4096 */
-    int brief_near;   /* LDS near halo of the fused BRIEF: 8, 16, 24, 32; 0 = kernel with global far taps */
+    int brief_near;   /* LDS near halo of the fused BRIEF: 8 .. 48 in steps of 8 (40, 48 need > 64 KiB of LDS
+                         per work-group, as on gfx950); 0 = kernel with global far taps */
     int tri_band;     /* source rows per LDS band of the trilateral filter: 1..(LDS limit, <= 16) */
 } bm_tuning;
 

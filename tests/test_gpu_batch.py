@@ -31,7 +31,7 @@ def test_tuning_does_not_change_results(N):
     ref = _flow(ctx, pr)
     t0 = ctx.tuning()
     px = (1, 2, 4) if N <= 128 else ((1, 2) if N <= 256 else (1,))
-    for lp, strip, near, band in itertools.product(px, (16, 256), (0, 8, 32), (1, t0.tri_band)):
+    for lp, strip, near, band in itertools.product(px, (16, 256), (0, 8, 32, 48), (1, t0.tri_band)):
         ctx.set_tuning(level_px=lp, strip_rows=strip, brief_near=near, tri_band=band)
         got = _flow(ctx, pr)
         for a, b in zip(ref, got):
