@@ -4,6 +4,11 @@
     python -m briefmatch_amd.rocprof_summary traffic <fetch_counter_collection.csv> <write_counter_collection.csv>
            <copy_fetch.csv> <copy_write.csv> <copy_bytes> <out.txt> <traffic.json> --cmd "<commands>"
 
+`counters` averages every counter per kernel over its dispatches and adds, from
+the kernel statistics of a trace of the same command, the instruction rates
+(VALU and INT32 VALU lane-operations per second against the 256 CU x 64 lanes
+x 2.4 GHz issue peak of one MI355X).
+
 `traffic` scales each kernel's mean FETCH_SIZE and WRITE_SIZE (KiB per dispatch)
 by what the same counter reads for the plain copy kernel (k_copy) of known
 size, and writes the HBM bytes per launch per kernel to traffic.json, which
@@ -46,6 +51,36 @@ def stats(path, out, cmd):
         lines.append("%-34s %7d %12.3f %12.2f %12.2f %7.2f" % (
             short_name(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"]) / 1e6,
             float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3, float(r["Percentage"])))
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+PEAK_LANE_OPS = 256 * 64 * 2.4e9   # one VALU lane-operation per lane per cycle
+
+
+def counters(path, stats_path, out, cmd):
+    acc = defaultdict(lambda: defaultdict(list))
+    for r in csv.DictReader(open(path)):
+        acc[short_name(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    dur = {}
+    for r in csv.DictReader(open(stats_path)):
+        dur[short_name(r["Name"])] = float(r["AverageNs"]) * 1e-9
+    names = sorted({c for k in acc.values() for c in k})
+    lines = ["# rocprofv3 counters, mean per dispatch", "# command: " + cmd,
+             "# rates use the average duration of the same kernel in the kernel-trace run;",
+             "# VALU lane-op peak = 256 CU x 64 lanes x 2.4 GHz = %.1f T/s" % (PEAK_LANE_OPS / 1e12), ""]
+    head = "%-30s" % "kernel" + "".join(" %14s" % n[:14] for n in names)
+    extra = "int32_Tops/s valu_frac_peak" if any("SQ_INSTS_VALU" in n for n in names) else ""
+    lines.append(head + "  " + extra)
+    for k in sorted(acc, key=lambda k: -dur.get(k, 0.0)):
+        vals = {n: sum(v) / len(v) for n, v in acc[k].items()}
+        row = "%-30s" % k + "".join(" %14.4g" % vals[n] if n in vals else " %14s" % "-" for n in names)
+        d = dur.get(k)
+        if extra and d:
+            i32 = vals.get("SQ_INSTS_VALU_INT32", 0.0) * 64 / d
+            va = vals.get("SQ_INSTS_VALU", 0.0) * 64 / d
+            row += "  %12.2f %14.3f" % (i32 / 1e12, va / PEAK_LANE_OPS)
+        lines.append(row)
     with open(out, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -101,9 +136,16 @@ def main():
     b.add_argument("out")
     b.add_argument("out_json")
     b.add_argument("--cmd", required=True)
+    c = sub.add_parser("counters")
+    c.add_argument("csv")
+    c.add_argument("stats_csv")
+    c.add_argument("out")
+    c.add_argument("--cmd", required=True)
     args = ap.parse_args()
     if args.mode == "stats":
         stats(args.csv, args.out, args.cmd)
+    elif args.mode == "counters":
+        counters(args.csv, args.stats_csv, args.out, args.cmd)
     else:
         traffic(args.fetch, args.write, args.copy_fetch, args.copy_write, args.copy_bytes, args.out, args.out_json,
                 args.cmd)
