@@ -8,6 +8,7 @@ Every HIP translation unit is compiled with packed FP32 instructions disabled
 (-packed-fp32-ops); host code is compiled without FP contraction.
 """
 import argparse
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -43,13 +44,17 @@ def _run(cmd, verbose):
 
 
 def build_lib(verbose=False):
-    objs = []
     os.makedirs(os.path.join(PKG, "_build"), exist_ok=True)
+    cmds, objs = [], []
     for src in ("bm_kernels.hip", "bm_net.hip", "bm_capi.cpp", "bm_net_capi.cpp"):
         obj = os.path.join(PKG, "_build", src + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        _run([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        cmds.append([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # the translation units compile independently
+    with concurrent.futures.ThreadPoolExecutor(max_workers=len(cmds)) as pool:
+        for f in [pool.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB)
