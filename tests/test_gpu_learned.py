@@ -228,3 +228,41 @@ def test_trained_proposal_gain():
     r = learned.evaluate(net, seeds=(0, 1), fewer=1, log=lambda m: print(m, flush=True))
     assert r["learned_fewer"] <= 1.02 * r["plain"]
     assert r["learned_fewer"] < r["plain_fewer"]
+
+
+def test_net_api_errors_and_roundtrip(tmp_path):
+    """Invalid calls fail with a status before any launch; weights survive a
+    save / load round trip bit for bit."""
+    w = _random_weights(4)
+    net = bm.Net(weights=w)
+    path = str(tmp_path / "w.bnet")
+    net.save(path)
+    assert np.array_equal(bm.Net(path=path).weights(), w)
+    raw = open(path, "rb").read()
+    (tmp_path / "short.bnet").write_bytes(raw[:-4])
+    (tmp_path / "magic.bnet").write_bytes(b"XXXXXXXX" + raw[8:])
+    for bad in ("short.bnet", "magic.bnet", "missing.bnet"):
+        with pytest.raises(bm.BriefMatchError):
+            net.load(str(tmp_path / bad))
+    with pytest.raises(bm.BriefMatchError):
+        net.set_weights(w[:-1])
+    bad_w = w.copy()
+    bad_w[7] = np.nan
+    with pytest.raises(bm.BriefMatchError):
+        net.set_weights(bad_w)
+    assert np.array_equal(net.weights(), w)                 # rejected calls leave the weights alone
+    feat = bm.DeviceBuffer(2 * 4 * 5 * 256)
+    lab = bm.DeviceBuffer(2 * 4 * 5 * 8)
+    feat.zero()
+    lab.zero()
+    for idx in ([0, 2], [-1], list(range(257))):            # row outside the set / batch over 256
+        with pytest.raises(bm.BriefMatchError):
+            net.train_step(feat, lab, 2, idx, 4, 5, 1e-3)
+    with pytest.raises(bm.BriefMatchError):
+        net.train_step(feat, lab, 2, [0, 1], 4, 5, float("nan"))
+    assert np.array_equal(net.weights(), w)
+    tiny = bm.Context(bm.derive_params(7, 7, 2.0, 2.0, N=32))
+    rgb = bm.DeviceBuffer(array=np.zeros((7, 7, 3), np.uint8))
+    U, V = tiny.image_buffer(), tiny.image_buffer()
+    with pytest.raises(bm.BriefMatchError):
+        tiny.flow_pair_learned(net, rgb, rgb, U, V)
