@@ -10,7 +10,8 @@ flow.  value = output flow pixels (1920*1080) per second / 1e6.
 Prints one JSON line.  Besides the driver's fields it carries:
   roofline      the dominant kernel (largest summed device time over the timed
                 steps): algorithmic bytes (or flops) per launch / its average
-                launch time from HIP events on the context stream;
+                launch time from HIP events on the context stream; "pipeline":
+                the sum over stages of work / peak against the measured step;
   stages        per-stage event times and their own roofline fractions;
   streaming     sequence mode (one BRIEF per frame, temporal init);
   learned       the same pair through the learned-proposal path with half the
@@ -122,8 +123,8 @@ def cpu_baseline(pr, p, pattern, seed, gpu_off, gpu_cost, gpu_U, gpu_V, kernel, 
     parity = {"match_field_bit_exact": bool(np.array_equal(off, gpu_off) and np.array_equal(cost, gpu_cost)),
               "flow_max_abs_px_on_bands": dmax, "flow_tolerance_px": 1e-3, "rows_checked": rows}
     cb = {"value": W * H / total / 1e6, "unit": "Mpix/s", "cores": oracle.num_threads(), "kind": "port",
-          "sample": "same 1920x1080 pair: gray, up-sampling, BRIEF, match, down-sampling, median on the full "
-                    "frame; trilateral on %d of %d rows (3 bands), its time scaled x%.2f" % (rows, H, H / rows),
+          "sample": "same %dx%d pair: gray, up-sampling, BRIEF, match, down-sampling, median on the full "
+                    "frame; trilateral on %d of %d rows (3 bands), its time scaled x%.2f" % (W, H, rows, H, H / rows),
           "seconds_measured": sum(v for k, v in t.items() if k != "trilateral") + t_tri_band,
           "stage_seconds": {k: round(v, 3) for k, v in t.items()}}
     return cb, parity
@@ -305,6 +306,14 @@ def main():
     roof["kernel"] = dom
     roof["work_per_launch"] = w
     roof["traffic"] = load_traffic(dom)
+    # whole pair: sum over stages of algorithmic work / peak against the step time
+    t_roof = 0.0
+    for name, st in stages.items():
+        if name in work:
+            k, wk = work[name]
+            t_roof += wk * st["launches_per_step"] / (HBM_PEAK_GBS * 1e9 if k == "hbm" else FP32_PEAK_TFLOPS * 1e12)
+    roof["pipeline"] = {"t_roof_ms": round(t_roof * 1e3, 4), "t_measured_ms": round(ms_per_step, 4),
+                        "frac": round(t_roof * 1e3 / ms_per_step, 4)}
 
     # streaming: alternate the two frames through the sequence entry point
     has = ctx.stream_frame(rgb1, U, V)
