@@ -186,7 +186,8 @@ int bm_net_create(bm_net **out)
 void bm_net_destroy(bm_net *net)
 {
     if (!net) return;
-    if (net->stream) (void)hipStreamSynchronize(net->stream);
+    // contexts may still have queued learned-path work reading the packed weights
+    (void)hipDeviceSynchronize();
     free_training(net);
     for (void *p : net->bufs) (void)hipFree(p);
     if (net->ready) (void)hipEventDestroy(net->ready);

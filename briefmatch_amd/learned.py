@@ -225,6 +225,7 @@ def main(argv=None):
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
     e.add_argument("--seeds", type=int, default=4)
+    e.add_argument("--workload", default="c2_1920x1080_u2.9_n128", choices=sorted(synth.WORKLOADS))
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     if a.cmd == "train":
@@ -241,7 +242,7 @@ def main(argv=None):
             log(str(evaluate(net, log=log)))
     else:
         net = Net(path=a.weights)
-        log(str(evaluate(net, seeds=tuple(range(a.seeds)), log=log)))
+        log(str(evaluate(net, workload=a.workload, seeds=tuple(range(a.seeds)), log=log)))
     return 0
 
 

@@ -255,7 +255,7 @@ const char *bm_batch_last_error(const bm_batch *b);
 #define BM_NET_PARAMS 193808
 typedef struct bm_net bm_net;
 int bm_net_create(bm_net **out);            /* all weights zero */
-void bm_net_destroy(bm_net *net);           /* waits for the net's work */
+void bm_net_destroy(bm_net *net);           /* waits for all device work (contexts may read its weights) */
 const char *bm_net_last_error(const bm_net *net);
 int bm_net_set_weights(bm_net *net, const float *w_host, size_t count);  /* count = BM_NET_PARAMS; resets Adam */
 int bm_net_get_weights(bm_net *net, float *w_host, size_t count);
