@@ -1103,8 +1103,8 @@ int bm_init_proposal(bm_ctx *ctx, const uint32_t *F1, const uint32_t *F2, int fi
     const bm::Geometry &g = ctx->g;
     const int cw = g.W / bm::net::kCell, ch = g.H / bm::net::kCell;
     if (cw < 1 || ch < 1) return fail(ctx, BM_E_PARAM, "the learned proposal needs frames of at least 8x8 pixels");
-    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, bm::net::launch_init_proposal(g.NSP, F1, F2, first, prev, coff, cw, ch, g.W, g.H,
-                                                                      g.sx, g.sy, off, cost, ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL,
+              bm::launch_init_proposal(g, F1, F2, first, prev, coff, cw, ch, -1, 0, off, cost, ctx->stream));
     return BM_OK;
 }
 
@@ -1133,14 +1133,11 @@ int bm_flow_pair_learned(bm_ctx *ctx, bm_net *net, const uint8_t *rgb1, const ui
                                                          ctx->d_nout, ctx->stream));
     BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, launch_offsets(ctx, ctx->d_nout, ctx->d_feat, ctx->d_coff));
     const int dst = ctx->field;
-    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, bm::net::launch_init_proposal(g.NSP, ctx->d_F[0], ctx->d_F[1], 1, nullptr,
-                                                                      ctx->d_coff, cw, ch, g.W, g.H, g.sx, g.sy,
-                                                                      ctx->d_off[dst], ctx->d_cost[dst], ctx->stream));
-    // one random search around the seeded field, on its own stream index
-    // (the propagation iterations use 0 .. iterations - 1)
-    BM_LAUNCH(ctx, BM_STAGE_RANDOM_SEARCH, bm::launch_random_search(g, ctx->d_F[0], ctx->d_F[1], ctx->d_off[dst],
-                                                                    ctx->d_cost[dst], (uint32_t)g.iterations, pair,
-                                                                    ctx->stream));
+    // the seeded field, then one random search around it on its own stream
+    // index (the propagation iterations use 0 .. iterations - 1), in one kernel
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL,
+              bm::launch_init_proposal(g, ctx->d_F[0], ctx->d_F[1], 1, nullptr, ctx->d_coff, cw, ch, g.iterations,
+                                       pair, ctx->d_off[dst], ctx->d_cost[dst], ctx->stream));
     int pd = -1;
     for (int i = 0; i < g.iterations; ++i) {
         r = run_iteration(ctx, ctx->d_F[0], ctx->d_F[1], pd, i, pair, true);

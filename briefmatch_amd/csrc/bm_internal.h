@@ -66,6 +66,11 @@ hipError_t launch_level(const Geometry &g, const uint32_t *F1, const uint32_t *F
 hipError_t launch_random_search(const Geometry &g, const uint32_t *F1, const uint32_t *F2,
                                 uint32_t *off, uint16_t *cost, uint32_t iteration, uint32_t pair,
                                 hipStream_t st);
+// Proposal-seeded initialisation from coff [ch * cw][4] (network and argmin
+// offsets, Q8 matched pixels); rs_iteration >= 0 fuses that random search.
+hipError_t launch_init_proposal(const Geometry &g, const uint32_t *F1, const uint32_t *F2, int first,
+                                const uint32_t *prev, const int32_t *coff, int cw, int ch, int rs_iteration,
+                                uint32_t pair, uint32_t *off, uint16_t *cost, hipStream_t st);
 hipError_t launch_downsample(const Geometry &g, const uint32_t *off, float *U, float *V, hipStream_t st);
 hipError_t launch_median(const Geometry &g, const float *in0, const float *in1, float *out0, float *out1,
                          int dir, hipStream_t st);

@@ -17,6 +17,7 @@
 #include <algorithm>
 
 #include "bm_internal.h"
+#include "bm_device.h"
 
 namespace bm {
 namespace {
@@ -26,38 +27,6 @@ __device__ const int32_t d_ndtri_q16[4096] = {
 };
 
 constexpr int kThreads = 256;
-
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
-
-__device__ __forceinline__ uint32_t pack_off(int dx, int dy)
-{
-    return (uint32_t)(uint16_t)dx | ((uint32_t)(uint16_t)dy << 16);
-}
-__device__ __forceinline__ int off_dx(uint32_t o) { return (int)(int16_t)(o & 0xffffu); }
-__device__ __forceinline__ int off_dy(uint32_t o) { return (int)(int16_t)(o >> 16); }
-
-// Work-groups are handed to the 8 XCDs round-robin in dispatch order.  Give
-// XCD k the k-th contiguous slice of the tile order instead, so that the tiles
-// sharing one L2 are neighbours in the image.
-__device__ __forceinline__ int xcd_remap(int b, int B)
-{
-    const int q = B >> 3, r = B & 7, k = b & 7;
-    return k * q + min(k, r) + (b >> 3);
-}
-
-// Row-segment blocks (kThreads pixels of one row) visited strip by strip: within
-// a strip of strip_rows rows the order runs down a column of blocks first, so the
-// blocks resident on one XCD at any time cover a compact 2-D patch and the
-// data-dependent descriptor gathers around it stay in that XCD's L2.
-__device__ __forceinline__ void strip_block(int b, int gx, int rows_total, int strip_rows, int &bx, int &by)
-{
-    const int per = gx * strip_rows;
-    const int strip = b / per;
-    const int rows = min(strip_rows, rows_total - strip * strip_rows);
-    const int rem = b - strip * per;
-    bx = rem / rows;
-    by = strip * strip_rows + (rem - bx * rows);
-}
 
 __device__ __forceinline__ void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3,
                                               uint32_t k0, uint32_t k1)
@@ -82,52 +51,6 @@ __device__ __forceinline__ int round_q32(int64_t v)
     const int64_t a = v < 0 ? -v : v;
     const int64_t r = (a + (int64_t(1) << 31)) >> 32;
     return (int)(v < 0 ? -r : r);
-}
-
-template <int NSP>
-__device__ __forceinline__ void load_desc(const uint32_t *__restrict__ F, uint32_t idx, uint32_t (&w)[NSP])
-{
-    const uint32_t *p = F + (size_t)idx * NSP;
-    if constexpr (NSP == 1) {
-        w[0] = p[0];
-    } else if constexpr (NSP == 2) {
-        const uint2 v = *reinterpret_cast<const uint2 *>(p);
-        w[0] = v.x;
-        w[1] = v.y;
-    } else {
-#pragma unroll
-        for (int k = 0; k < NSP / 4; ++k) {
-            const uint4 v = reinterpret_cast<const uint4 *>(p)[k];
-            w[4 * k + 0] = v.x;
-            w[4 * k + 1] = v.y;
-            w[4 * k + 2] = v.z;
-            w[4 * k + 3] = v.w;
-        }
-    }
-}
-
-template <int NSP>
-__device__ __forceinline__ void store_desc(uint32_t *__restrict__ F, uint32_t idx, const uint32_t (&w)[NSP])
-{
-    uint32_t *p = F + (size_t)idx * NSP;
-    if constexpr (NSP == 1) {
-        p[0] = w[0];
-    } else if constexpr (NSP == 2) {
-        *reinterpret_cast<uint2 *>(p) = make_uint2(w[0], w[1]);
-    } else {
-#pragma unroll
-        for (int k = 0; k < NSP / 4; ++k)
-            reinterpret_cast<uint4 *>(p)[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
-    }
-}
-
-template <int NSP>
-__device__ __forceinline__ int hamming(const uint32_t (&a)[NSP], const uint32_t (&b)[NSP])
-{
-    int d = 0;
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) d += __popc(a[k] ^ b[k]);
-    return d;
 }
 
 // Fixed-point bilinear sample (Q16 gray levels) at the packed (x0 << 8 | frac)
@@ -513,15 +436,15 @@ __device__ __forceinline__ void random_candidate(const uint32_t (&f1)[NSP], cons
     nx = clampi(nx, -x, sx - 1 - x);
     ny = clampi(ny, -y, sy - 1 - y);
     const uint32_t cand = pack_off(nx, ny);
-    // insert() skips a candidate equal to the current offset (briefmatch_cuda.cu:131-132)
-    if (cand != best) {
-        uint32_t f2[NSP];
-        load_desc<NSP>(F2, idx + ny * sx + nx, f2);
-        const int c = hamming<NSP>(f1, f2);
-        if (c < bcost) {
-            bcost = c;
-            best = cand;
-        }
+    // the gather is unconditional (the target is in the frame) so it can be
+    // issued with the caller's loads; insert() skips a candidate equal to the
+    // current offset (briefmatch_cuda.cu:131-132)
+    uint32_t f2[NSP];
+    load_desc<NSP>(F2, idx + ny * sx + nx, f2);
+    const int c = hamming<NSP>(f1, f2);
+    if (cand != best && c < bcost) {
+        bcost = c;
+        best = cand;
     }
 }
 
@@ -581,11 +504,10 @@ __global__ void __launch_bounds__(kThreads) k_level(const uint32_t *__restrict__
     }
     uint32_t cand[PX][4];
     int tgt[PX][4];
-    bool need[PX][4], any[PX];
+    bool need[PX][4];
 #pragma unroll
     for (int q = 0; q < PX; ++q) {
         const int x = min(xq[q], sx - 1);
-        any[q] = false;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const int nx = clampi(off_dx(o[q][c]), -x, sx - 1 - x);
@@ -598,38 +520,41 @@ __global__ void __launch_bounds__(kThreads) k_level(const uint32_t *__restrict__
 #pragma unroll
             for (int e = 0; e < c; ++e) fresh = fresh && !(valid[q][e] && cand[q][e] == cand[q][c]);
             need[q][c] = fresh;
-            any[q] = any[q] || fresh;
         }
     }
     // round 2: own descriptor, candidate gathers and the random-search gather,
-    // all issued before any cost is formed
+    // all issued before any cost is formed.  The loads are unconditional (every
+    // target is in the frame: offsets are clamped and idx is clamped), so none of
+    // them waits behind a branch; a candidate whose cost is known reads a
+    // descriptor that is then ignored.
     uint32_t f1[PX][NSP], f2[PX][4][NSP], fr[PX][NSP];
     bool rs_need[PX];
 #pragma unroll
     for (int q = 0; q < PX; ++q) {
         rs_need[q] = RS && live[q] && rcand[q] != own[q];
-        if (any[q] || rs_need[q]) load_desc<NSP>(F1, (uint32_t)idx[q], f1[q]);
+        load_desc<NSP>(F1, (uint32_t)idx[q], f1[q]);
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
-            if (need[q][c]) load_desc<NSP>(F2, (uint32_t)tgt[q][c], f2[q][c]);
-        if (rs_need[q]) load_desc<NSP>(F2, (uint32_t)rtgt[q], fr[q]);
+        for (int c = 0; c < 4; ++c) load_desc<NSP>(F2, (uint32_t)tgt[q][c], f2[q][c]);
+        if constexpr (RS) load_desc<NSP>(F2, (uint32_t)rtgt[q], fr[q]);
     }
 #pragma unroll
     for (int q = 0; q < PX; ++q) {
+        // every Hamming distance is formed and the known costs are selected, with
+        // no branch the compiler could sink a gather into
         int cst[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            cst[c] = 0x7fffffff;
-            if (need[q][c]) {
-                cst[c] = hamming<NSP>(f1[q], f2[q][c]);
-            } else if (valid[q][c]) {
-                cst[c] = own_cost[q];
+            const int h = hamming<NSP>(f1[q], f2[q][c]);
+            int k = valid[q][c] ? own_cost[q] : 0x7fffffff;
 #pragma unroll
-                for (int e = c - 1; e >= 0; --e)
-                    if (valid[q][e] && cand[q][e] == cand[q][c]) cst[c] = cst[e];
-            }
+            for (int e = c - 1; e >= 0; --e) k = (valid[q][e] && cand[q][e] == cand[q][c]) ? cst[e] : k;
+            cst[c] = need[q][c] ? h : k;
         }
-        const int rcost = rs_need[q] ? hamming<NSP>(f1[q], fr[q]) : 0x7fffffff;
+        int rcost = 0x7fffffff;
+        if constexpr (RS) {
+            const int h = hamming<NSP>(f1[q], fr[q]);
+            rcost = rs_need[q] ? h : 0x7fffffff;
+        }
         // sequential insert() semantics: skip a candidate equal to the running
         // best, accept a strictly lower cost; the random candidate comes last
         uint32_t best = own[q];
@@ -672,6 +597,120 @@ __global__ void __launch_bounds__(kThreads) k_random_search(const uint32_t *__re
     random_candidate<NSP>(f1, F2, idx, x, y, sx, sy, rs, best, bcost);
     off[idx] = best;
     cost[idx] = (uint16_t)bcost;
+}
+
+// Proposal-seeded initialisation.  The plain candidate (zero offset, or the
+// temporal chain through prev) is evaluated first; then, for the network's
+// proposal and then the cost-volume argmin, the bilinear interpolation of the
+// four surrounding coarse cells' offsets and the four cells' own offsets.  A
+// candidate is skipped when it points outside the frame or equals the running
+// best, and taken only when strictly cheaper.  coff [cell][4] holds the network
+// (u, v) and argmin (u, v) offsets in Q8 matched pixels.  With RS the random-
+// search candidate of rs.iteration is inserted last, as k_random_search would.
+template <int NSP, bool RS>
+__global__ void __launch_bounds__(kThreads) k_init_proposal(const uint32_t *__restrict__ F1,
+                                                            const uint32_t *__restrict__ F2, int first,
+                                                            const uint32_t *__restrict__ prev,
+                                                            const int32_t *__restrict__ coff, int cw, int ch, int W,
+                                                            int H, int sx, int sy, int strip_rows, RSArgs rs,
+                                                            uint32_t *__restrict__ off, uint16_t *__restrict__ cost)
+{
+    // row segments of 256 pixels, strip-major and XCD-contiguous, so the
+    // candidate gathers of the segments sharing an L2 overlap
+    const int gx = (sx + kThreads - 1) / kThreads;
+    int bx, y;
+    strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, strip_rows, bx, y);
+    const int x = bx * kThreads + threadIdx.x;
+    if (x >= sx) return;
+    const int i = y * sx + x;
+    int nx = 0, ny = 0;
+    if (!first) {
+        const uint32_t o0 = prev[i];
+        const int dx0 = clampi(off_dx(o0), -x, sx - 1 - x), dy0 = clampi(off_dy(o0), -y, sy - 1 - y);
+        const uint32_t o1 = prev[i + dy0 * sx + dx0];
+        nx = clampi(off_dx(o1), -x, sx - 1 - x);
+        ny = clampi(off_dy(o1), -y, sy - 1 - y);
+    }
+    const int plain = i + ny * sx + nx;
+    uint32_t f1[NSP], f0[NSP], fr[NSP];
+    load_desc<NSP>(F1, (uint32_t)i, f1);
+    load_desc<NSP>(F2, (uint32_t)plain, f0);
+    // the random-search candidate depends only on (pixel, iteration, pair)
+    uint32_t rcand = 0u;
+    if constexpr (RS) {
+        uint32_t c0 = (uint32_t)i, c1 = rs.iteration, c2 = rs.pair, c3 = 0u;
+        philox4x32_10(c0, c1, c2, c3, rs.key0, rs.key1);
+        const int rx = clampi(round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c0 >> 20]), -x, sx - 1 - x);
+        const int ry = clampi(round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c1 >> 20]), -y, sy - 1 - y);
+        rcand = pack_off(rx, ry);
+        load_desc<NSP>(F2, (uint32_t)(i + ry * sx + rx), fr);
+    }
+
+    // coarse coordinate in Q8: cell c's centre is input pixel 8c + 3.5
+    const int64_t qx = min(max((int64_t)x * W * 32 / sx - 112, (int64_t)0), (int64_t)(cw - 1) * 256);
+    const int64_t qy = min(max((int64_t)y * H * 32 / sy - 112, (int64_t)0), (int64_t)(ch - 1) * 256);
+    const int x0 = (int)(qx >> 8), y0 = (int)(qy >> 8), x1 = min(x0 + 1, cw - 1), y1 = min(y0 + 1, ch - 1);
+    const int64_t ax = qx & 255, ay = qy & 255;
+    const int c00 = (y0 * cw + x0) * 4, c01 = (y0 * cw + x1) * 4, c10 = (y1 * cw + x0) * 4, c11 = (y1 * cw + x1) * 4;
+    int cand[10][2];
+#pragma unroll
+    for (int src = 0; src < 2; ++src)
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const int e = 2 * src + d;
+            const int64_t o00 = coff[c00 + e], o01 = coff[c01 + e], o10 = coff[c10 + e], o11 = coff[c11 + e];
+            const int64_t v = (o00 * (256 - ax) + o01 * ax) * (256 - ay) + (o10 * (256 - ax) + o11 * ax) * ay;
+            cand[5 * src][d] = (int)((v + (1 << 23)) >> 24);
+            cand[5 * src + 1][d] = (int)((o00 + 128) >> 8);
+            cand[5 * src + 2][d] = (int)((o01 + 128) >> 8);
+            cand[5 * src + 3][d] = (int)((o10 + 128) >> 8);
+            cand[5 * src + 4][d] = (int)((o11 + 128) >> 8);
+        }
+    // a candidate equal to the plain one or to an earlier candidate has the same
+    // cost and cannot be strictly cheaper than the running best: its cost is not
+    // needed.  Every candidate's gather is still issued (a skipped one re-reads
+    // the plain candidate's descriptor, always in the frame), so that the loads
+    // of a group are in flight together instead of one round trip each.
+    uint32_t bo = pack_off(nx, ny);
+    uint32_t seen[10];
+    bool ok[10];
+    int tgt[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+        const int cx = cand[k][0], cy = cand[k][1];
+        seen[k] = pack_off(cx, cy);
+        bool fresh = x + cx >= 0 && x + cx < sx && y + cy >= 0 && y + cy < sy && seen[k] != bo;
+#pragma unroll
+        for (int e = 0; e < k; ++e) fresh = fresh && !(ok[e] && seen[e] == seen[k]);
+        ok[k] = fresh;
+        tgt[k] = fresh ? i + cy * sx + cx : plain;
+    }
+    int best = hamming<NSP>(f1, f0);
+    // groups of G gathers (10 for 128-bit descriptors: 40 registers)
+    constexpr int G = NSP <= 4 ? 10 : (NSP <= 8 ? 5 : 2);
+#pragma unroll
+    for (int k0 = 0; k0 < 10; k0 += G) {
+        uint32_t f2[G][NSP];
+#pragma unroll
+        for (int g = 0; g < G; ++g) load_desc<NSP>(F2, (uint32_t)tgt[k0 + g], f2[g]);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int c = hamming<NSP>(f1, f2[g]);
+            if (ok[k0 + g] && c < best) {
+                best = c;
+                bo = seen[k0 + g];
+            }
+        }
+    }
+    if constexpr (RS) {
+        const int c = hamming<NSP>(f1, fr);
+        if (rcand != bo && c < best) {
+            best = c;
+            bo = rcand;
+        }
+    }
+    off[i] = bo;
+    cost[i] = (uint16_t)best;
 }
 
 // ---------------------------------------------------------------------------
@@ -1069,6 +1108,24 @@ hipError_t launch_random_search(const Geometry &g, const uint32_t *F1, const uin
     const RSArgs rs = make_rs(g, 1, iteration, pair);
     BM_NSP_DISPATCH(g.NSP, hipLaunchKernelGGL((k_random_search<NSP>), grid, dim3(kThreads), 0, st, F1, F2, off,
                                               cost, g.sx, g.sy, g.strip_rows, rs));
+    return hipGetLastError();
+}
+
+hipError_t launch_init_proposal(const Geometry &g, const uint32_t *F1, const uint32_t *F2, int first,
+                                const uint32_t *prev, const int32_t *coff, int cw, int ch, int rs_iteration,
+                                uint32_t pair, uint32_t *off, uint16_t *cost, hipStream_t st)
+{
+    const dim3 grid(blocks_rows(g.sx, g.sy));
+    const RSArgs rs = make_rs(g, rs_iteration >= 0, (uint32_t)(rs_iteration < 0 ? 0 : rs_iteration), pair);
+#define BM_INIT_PROPOSAL(RSV)                                                                                       \
+    hipLaunchKernelGGL((k_init_proposal<NSP, RSV>), grid, dim3(kThreads), 0, st, F1, F2, first, prev, coff, cw, ch, \
+                       g.W, g.H, g.sx, g.sy, g.strip_rows, rs, off, cost)
+    if (rs.enabled) {
+        BM_NSP_DISPATCH(g.NSP, BM_INIT_PROPOSAL(true));
+    } else {
+        BM_NSP_DISPATCH(g.NSP, BM_INIT_PROPOSAL(false));
+    }
+#undef BM_INIT_PROPOSAL
     return hipGetLastError();
 }
 

@@ -72,9 +72,6 @@ hipError_t launch_coarse_flow(const float *U, const float *V, int W, int cw, int
 // coff [n][4]: network (u, v) and cost-volume argmin (u, v), Q8 matched pixels
 hipError_t launch_proposal_offsets(const float *net_out, const uint16_t *feat, int n, float scale_x, float scale_y,
                                    int lim_x, int lim_y, int32_t *coff, hipStream_t st);
-hipError_t launch_init_proposal(int NSP, const uint32_t *F1, const uint32_t *F2, int first, const uint32_t *prev,
-                                const int32_t *coff, int cw, int ch, int W, int H, int sx, int sy, uint32_t *off,
-                                uint16_t *cost, hipStream_t st);
 // Row indices of a mini-batch, passed by value as a kernel argument.
 constexpr int kMaxBatch = 256;
 struct RowList {

@@ -39,15 +39,6 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-template <int NSP>
-__device__ __forceinline__ int ham(const uint32_t *__restrict__ a, const uint32_t *__restrict__ b)
-{
-    int d = 0;
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) d += __popc(a[k] ^ b[k]);
-    return d;
-}
-
 // num / den rounded half away from zero (den > 0)
 __device__ __forceinline__ int rdiv_sym(int64_t num, int64_t den)
 {
@@ -403,87 +394,6 @@ __global__ void k_proposal_offsets(const float *__restrict__ out, const uint16_t
     }
 }
 
-__device__ __forceinline__ int offx(uint32_t o) { return (int)(int16_t)(o & 0xffffu); }
-__device__ __forceinline__ int offy(uint32_t o) { return (int)(int16_t)(o >> 16); }
-__device__ __forceinline__ uint32_t packo(int dx, int dy) { return (uint32_t)(uint16_t)dx | ((uint32_t)(uint16_t)dy << 16); }
-
-// Proposal-seeded initialisation.  The plain candidate (zero offset, or the
-// temporal chain through prev) is evaluated first; then, for the network's
-// proposal and then the cost-volume argmin, the bilinear interpolation of the
-// four surrounding coarse cells' offsets and the four cells' own offsets.  A
-// candidate is skipped when it points outside the frame or equals the running
-// best, and taken only when strictly cheaper.
-template <int NSP>
-__global__ void __launch_bounds__(256) k_init_proposal(const uint32_t *__restrict__ F1,
-                                                       const uint32_t *__restrict__ F2, int first,
-                                                       const uint32_t *__restrict__ prev,
-                                                       const int32_t *__restrict__ coff, int cw, int ch, int W, int H,
-                                                       int sx, int sy, uint32_t *__restrict__ off,
-                                                       uint16_t *__restrict__ cost)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= sx * sy) return;
-    const int x = i % sx, y = i / sx;
-    int nx = 0, ny = 0;
-    if (!first) {
-        const uint32_t o0 = prev[i];
-        const int dx0 = min(max(offx(o0), -x), sx - 1 - x), dy0 = min(max(offy(o0), -y), sy - 1 - y);
-        const uint32_t o1 = prev[i + dy0 * sx + dx0];
-        nx = min(max(offx(o1), -x), sx - 1 - x);
-        ny = min(max(offy(o1), -y), sy - 1 - y);
-    }
-    uint32_t f1[NSP];
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) f1[k] = F1[(size_t)i * NSP + k];
-    int best = ham<NSP>(f1, F2 + (size_t)(i + ny * sx + nx) * NSP);
-    uint32_t bo = packo(nx, ny);
-
-    // coarse coordinate in Q8: cell c's centre is input pixel 8c + 3.5
-    const int64_t qx = min(max((int64_t)x * W * 32 / sx - 112, (int64_t)0), (int64_t)(cw - 1) * 256);
-    const int64_t qy = min(max((int64_t)y * H * 32 / sy - 112, (int64_t)0), (int64_t)(ch - 1) * 256);
-    const int x0 = (int)(qx >> 8), y0 = (int)(qy >> 8), x1 = min(x0 + 1, cw - 1), y1 = min(y0 + 1, ch - 1);
-    const int64_t ax = qx & 255, ay = qy & 255;
-    const int c00 = (y0 * cw + x0) * 4, c01 = (y0 * cw + x1) * 4, c10 = (y1 * cw + x0) * 4, c11 = (y1 * cw + x1) * 4;
-    int cand[10][2];
-#pragma unroll
-    for (int src = 0; src < 2; ++src)
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-            const int e = 2 * src + d;
-            const int64_t o00 = coff[c00 + e], o01 = coff[c01 + e], o10 = coff[c10 + e], o11 = coff[c11 + e];
-            const int64_t v = (o00 * (256 - ax) + o01 * ax) * (256 - ay) + (o10 * (256 - ax) + o11 * ax) * ay;
-            cand[5 * src][d] = (int)((v + (1 << 23)) >> 24);
-            cand[5 * src + 1][d] = (int)((o00 + 128) >> 8);
-            cand[5 * src + 2][d] = (int)((o01 + 128) >> 8);
-            cand[5 * src + 3][d] = (int)((o10 + 128) >> 8);
-            cand[5 * src + 4][d] = (int)((o11 + 128) >> 8);
-        }
-    // a candidate equal to the plain one or to an earlier candidate has the same
-    // cost and cannot be strictly cheaper than the running best: no gather
-    uint32_t seen[10];
-    bool ok[10];
-#pragma unroll
-    for (int k = 0; k < 10; ++k) {
-        const int cx = cand[k][0], cy = cand[k][1];
-        seen[k] = packo(cx, cy);
-        bool fresh = x + cx >= 0 && x + cx < sx && y + cy >= 0 && y + cy < sy && seen[k] != bo;
-#pragma unroll
-        for (int e = 0; e < k; ++e) fresh = fresh && !(ok[e] && seen[e] == seen[k]);
-        ok[k] = fresh;
-    }
-#pragma unroll
-    for (int k = 0; k < 10; ++k) {
-        if (!ok[k]) continue;
-        const int c = ham<NSP>(f1, F2 + (size_t)(i + cand[k][1] * sx + cand[k][0]) * NSP);
-        if (c < best) {
-            best = c;
-            bo = seen[k];
-        }
-    }
-    off[i] = bo;
-    cost[i] = (uint16_t)best;
-}
-
 // dst[r] = src[rl.idx[r]] for rows of row_words 32-bit words
 __global__ void k_gather_rows(const uint32_t *__restrict__ src, const RowList rl, size_t row_words, int rows,
                               uint32_t *__restrict__ dst)
@@ -496,16 +406,6 @@ __global__ void k_gather_rows(const uint32_t *__restrict__ src, const RowList rl
 
 // ---------------------------------------------------------------------------
 // launchers
-
-#define BM_NET_NSP(NSP_, CALL)                        \
-    switch (NSP_) {                                   \
-    case 1: { constexpr int NSP = 1; CALL; } break;   \
-    case 2: { constexpr int NSP = 2; CALL; } break;   \
-    case 4: { constexpr int NSP = 4; CALL; } break;   \
-    case 8: { constexpr int NSP = 8; CALL; } break;   \
-    case 16: { constexpr int NSP = 16; CALL; } break; \
-    default: return hipErrorInvalidValue;             \
-    }
 
 hipError_t launch_features(const uint8_t *g1, const uint8_t *g2, int W, int cw, int ch, int32_t *S, uint16_t *feat,
                            hipStream_t st)
@@ -609,16 +509,6 @@ hipError_t launch_proposal_offsets(const float *net_out, const uint16_t *feat, i
 {
     hipLaunchKernelGGL(k_proposal_offsets, dim3((n + 255) / 256), dim3(256), 0, st, net_out, feat, n, scale_x, scale_y,
                        lim_x, lim_y, coff);
-    return hipGetLastError();
-}
-
-hipError_t launch_init_proposal(int NSP_, const uint32_t *F1, const uint32_t *F2, int first, const uint32_t *prev,
-                                const int32_t *coff, int cw, int ch, int W, int H, int sx, int sy, uint32_t *off,
-                                uint16_t *cost, hipStream_t st)
-{
-    const dim3 grid((unsigned)(((size_t)sx * sy + 255) / 256));
-    BM_NET_NSP(NSP_, hipLaunchKernelGGL(k_init_proposal<NSP>, grid, dim3(256), 0, st, F1, F2, first, prev, coff, cw,
-                                        ch, W, H, sx, sy, off, cost));
     return hipGetLastError();
 }
 
