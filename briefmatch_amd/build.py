@@ -29,6 +29,12 @@ HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=" + ARCH, "-fPIC", "-Wall",
              "-Xarch_host", "-ffp-contract=off", "-I" + INCLUDE]
 
 
+# The trilateral filter's MFMA accumulators stay in VGPRs: with the default
+# AGPR form the register allocator swaps the row groups' accumulators through
+# accvgpr moves on every tap.
+EXTRA_FLAGS = {"bm_kernels.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -49,7 +55,7 @@ def build_lib(verbose=False):
     for src in ("bm_kernels.hip", "bm_net.hip", "bm_capi.cpp", "bm_net_capi.cpp"):
         obj = os.path.join(PKG, "_build", src + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmds.append([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC] + HIP_FLAGS + lang + EXTRA_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     # the translation units compile independently
     with concurrent.futures.ThreadPoolExecutor(max_workers=len(cmds)) as pool:

@@ -594,6 +594,10 @@ int bm_ctx_create(const bm_params *params, uint64_t seed, bm_ctx **out)
         return fail(nullptr, BM_E_PARAM, "trilateral window outside [1, 2000]");
     }
     g.tri_band = p.filter ? bm::tri_max_band(g.tri_s) : 1;
+    // the MFMA kernel's 64 x 8 tiles fill the device where the VALU kernel's
+    // 64 x 16 tiles cannot (measured: 2x faster at 320x240, 75 tiles; 3 % at
+    // 1280x720, 900 tiles; 2 % slower at 1920x1080, 2040 tiles)
+    g.tri_groups = ((p.sx_in + 63) / 64) * ((p.sy_in + 15) / 16) <= 1024 ? 2 : 0;
     std::vector<float> spt(std::max(2 * g.tri_s, 1));
     for (int m_ = 0; m_ < 2 * g.tri_s; ++m_) {
         const double i = m_ - g.tri_s;
@@ -1293,6 +1297,7 @@ int tuning_valid(const bm_ctx *ctx, const bm_tuning &t, std::string &why)
         return why = "brief_near must be 0, 8, 16, 24, 32, 40 or 48", 0;
     const int maxb = ctx->p.filter ? bm::tri_max_band(ctx->g.tri_s) : 16;
     if (t.tri_band < 1 || t.tri_band > std::max(1, maxb)) return why = "tri_band outside the LDS limit", 0;
+    if (t.tri_groups != 0 && t.tri_groups != 2 && t.tri_groups != 4) return why = "tri_groups must be 0, 2 or 4", 0;
     return 1;
 }
 
@@ -1309,6 +1314,7 @@ int apply_tuning(bm_ctx *ctx, const bm_tuning &t)
     ctx->g.level_px = t.level_px;
     ctx->g.strip_rows = t.strip_rows;
     ctx->g.tri_band = t.tri_band;
+    ctx->g.tri_groups = t.tri_groups;
     return BM_OK;
 }
 
@@ -1319,6 +1325,7 @@ bm_tuning current_tuning(const bm_ctx *ctx)
     t.strip_rows = ctx->g.strip_rows;
     t.brief_near = ctx->g.brief_near;
     t.tri_band = ctx->g.tri_band;
+    t.tri_groups = ctx->g.tri_groups;
     return t;
 }
 
@@ -1371,7 +1378,7 @@ int bm_autotune(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, const cha
             std::istringstream ls(line);
             std::string k;
             bm_tuning t;
-            if (ls >> k >> t.level_px >> t.strip_rows >> t.brief_near >> t.tri_band && k == key) {
+            if (ls >> k >> t.level_px >> t.strip_rows >> t.brief_near >> t.tri_band >> t.tri_groups && k == key) {
                 std::string why;
                 if (tuning_valid(ctx, t, why) && apply_tuning(ctx, t) == BM_OK) {
                     if (out) *out = t;
@@ -1421,6 +1428,7 @@ int bm_autotune(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, const cha
     const Knob knobs[] = {{&bm_tuning::brief_near, {16, 24, 32, 40, 48}, br},
                           {&bm_tuning::level_px, {1, 2, 4}, lv},
                           {&bm_tuning::strip_rows, {64, 128, 256, 512, 1024}, lv},
+                          {&bm_tuning::tri_groups, {0, 2, 4}, tr},
                           {&bm_tuning::tri_band, bands, tr}};
     int e = BM_OK;
     for (const Knob &kn : knobs) {
@@ -1458,7 +1466,7 @@ int bm_autotune(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, const cha
     if (cache_path) {
         std::ofstream o(cache_path, std::ios::app);
         o << key << " " << best.level_px << " " << best.strip_rows << " " << best.brief_near << " " << best.tri_band
-          << "\n";
+          << " " << best.tri_groups << "\n";
     }
     return BM_OK;
 }

@@ -46,6 +46,7 @@ struct Geometry {
     int level_px;                 // pixels per thread of the level kernels: 1 or 2
     int strip_rows;               // strip height of the matching kernels' block order
     int tri_band;                 // max source rows per LDS band of the trilateral filter
+    int tri_groups;               // trilateral kernel: 0 VALU sums, 2 / 4 MFMA sums with that many 4-row groups
 };
 
 // Largest trilateral band (rows) that fits the LDS budget for half window s.

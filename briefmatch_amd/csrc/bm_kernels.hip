@@ -436,15 +436,15 @@ __device__ __forceinline__ void random_candidate(const uint32_t (&f1)[NSP], cons
     nx = clampi(nx, -x, sx - 1 - x);
     ny = clampi(ny, -y, sy - 1 - y);
     const uint32_t cand = pack_off(nx, ny);
-    // the gather is unconditional (the target is in the frame) so it can be
-    // issued with the caller's loads; insert() skips a candidate equal to the
-    // current offset (briefmatch_cuda.cu:131-132)
-    uint32_t f2[NSP];
-    load_desc<NSP>(F2, idx + ny * sx + nx, f2);
-    const int c = hamming<NSP>(f1, f2);
-    if (cand != best && c < bcost) {
-        bcost = c;
-        best = cand;
+    // insert() skips a candidate equal to the current offset (briefmatch_cuda.cu:131-132)
+    if (cand != best) {
+        uint32_t f2[NSP];
+        load_desc<NSP>(F2, idx + ny * sx + nx, f2);
+        const int c = hamming<NSP>(f1, f2);
+        if (c < bcost) {
+            bcost = c;
+            best = cand;
+        }
     }
 }
 
@@ -504,10 +504,11 @@ __global__ void __launch_bounds__(kThreads) k_level(const uint32_t *__restrict__
     }
     uint32_t cand[PX][4];
     int tgt[PX][4];
-    bool need[PX][4];
+    bool need[PX][4], any[PX];
 #pragma unroll
     for (int q = 0; q < PX; ++q) {
         const int x = min(xq[q], sx - 1);
+        any[q] = false;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const int nx = clampi(off_dx(o[q][c]), -x, sx - 1 - x);
@@ -520,41 +521,38 @@ __global__ void __launch_bounds__(kThreads) k_level(const uint32_t *__restrict__
 #pragma unroll
             for (int e = 0; e < c; ++e) fresh = fresh && !(valid[q][e] && cand[q][e] == cand[q][c]);
             need[q][c] = fresh;
+            any[q] = any[q] || fresh;
         }
     }
     // round 2: own descriptor, candidate gathers and the random-search gather,
-    // all issued before any cost is formed.  The loads are unconditional (every
-    // target is in the frame: offsets are clamped and idx is clamped), so none of
-    // them waits behind a branch; a candidate whose cost is known reads a
-    // descriptor that is then ignored.
+    // all issued before any cost is formed
     uint32_t f1[PX][NSP], f2[PX][4][NSP], fr[PX][NSP];
     bool rs_need[PX];
 #pragma unroll
     for (int q = 0; q < PX; ++q) {
         rs_need[q] = RS && live[q] && rcand[q] != own[q];
-        load_desc<NSP>(F1, (uint32_t)idx[q], f1[q]);
+        if (any[q] || rs_need[q]) load_desc<NSP>(F1, (uint32_t)idx[q], f1[q]);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) load_desc<NSP>(F2, (uint32_t)tgt[q][c], f2[q][c]);
-        if constexpr (RS) load_desc<NSP>(F2, (uint32_t)rtgt[q], fr[q]);
+        for (int c = 0; c < 4; ++c)
+            if (need[q][c]) load_desc<NSP>(F2, (uint32_t)tgt[q][c], f2[q][c]);
+        if (rs_need[q]) load_desc<NSP>(F2, (uint32_t)rtgt[q], fr[q]);
     }
 #pragma unroll
     for (int q = 0; q < PX; ++q) {
-        // every Hamming distance is formed and the known costs are selected, with
-        // no branch the compiler could sink a gather into
         int cst[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const int h = hamming<NSP>(f1[q], f2[q][c]);
-            int k = valid[q][c] ? own_cost[q] : 0x7fffffff;
+            cst[c] = 0x7fffffff;
+            if (need[q][c]) {
+                cst[c] = hamming<NSP>(f1[q], f2[q][c]);
+            } else if (valid[q][c]) {
+                cst[c] = own_cost[q];
 #pragma unroll
-            for (int e = c - 1; e >= 0; --e) k = (valid[q][e] && cand[q][e] == cand[q][c]) ? cst[e] : k;
-            cst[c] = need[q][c] ? h : k;
+                for (int e = c - 1; e >= 0; --e)
+                    if (valid[q][e] && cand[q][e] == cand[q][c]) cst[c] = cst[e];
+            }
         }
-        int rcost = 0x7fffffff;
-        if constexpr (RS) {
-            const int h = hamming<NSP>(f1[q], fr[q]);
-            rcost = rs_need[q] ? h : 0x7fffffff;
-        }
+        const int rcost = rs_need[q] ? hamming<NSP>(f1[q], fr[q]) : 0x7fffffff;
         // sequential insert() semantics: skip a candidate equal to the running
         // best, accept a strictly lower cost; the random candidate comes last
         uint32_t best = own[q];
@@ -857,9 +855,9 @@ __global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict
 #pragma unroll
                 for (int k = 0; k < kTriPY; ++k) {
                     float du = qa.x - um[k], dv = qa.y - vm[k], dg = qa.z - g0[k];
-                    float t = fmaf(du, du, sp.x);
+                    float t = fmaf(dg, dg, sp.x);
                     t = fmaf(dv, dv, t);
-                    t = fmaf(dg, dg, t);
+                    t = fmaf(du, du, t);
                     float w = __builtin_amdgcn_exp2f(-t);
                     rw[k] += w;
                     ru[k] = fmaf(w, qa.x, ru[k]);
@@ -867,9 +865,9 @@ __global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict
                     du = qb.x - um[k];
                     dv = qb.y - vm[k];
                     dg = qb.z - g0[k];
-                    t = fmaf(du, du, sp.y);
+                    t = fmaf(dg, dg, sp.y);
                     t = fmaf(dv, dv, t);
-                    t = fmaf(dg, dg, t);
+                    t = fmaf(du, du, t);
                     w = __builtin_amdgcn_exp2f(-t);
                     rw[k] += w;
                     ru[k] = fmaf(w, qb.x, ru[k]);
@@ -896,6 +894,138 @@ __global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict
                 const float den = fmaxf(1e-5f, aw[k]);
                 Uo[(size_t)y * W + x] = (au[k] * ia) / den;
                 Vo[(size_t)y * W + x] = (av[k] * ia) / den;
+            }
+        }
+    }
+}
+
+// Cross-trilateral filter with the three weighted sums on the matrix cores.
+// v_mfma_f32_4x4x1f32 runs 16 independent 4x4 outer products per wave; block b
+// is lanes 4b..4b+3 and is one output column.  Lane 4b+k computes the weight w
+// of output row k for the current source q (the A operand, w of the four rows
+// of a column share q); lane 4b+j supplies component j of (1, u, v, I) of q
+// times the column factor exp2(-spatial_i) (the B operand); D[i][j], held by
+// lane 4b+j in register i, accumulates row i's sum over q of w * B_j.  The row
+// part of the spatial term is the first addend of the exponent (+inf outside
+// the window), so a tap costs 3 sub, 3 FMA and one exp2 per pixel on the VALU;
+// the per-source B scaling is shared by the R groups of four rows a wave owns.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <int R, int MASK>
+__device__ __forceinline__ void tri_taps(const float4 *__restrict__ src, const float *__restrict__ srcf,
+                                         const float *__restrict__ es, int n, const float (&um)[R],
+                                         const float (&vm)[R], const float (&g0)[R], const float (&ck)[R],
+                                         v4f (&acc)[R])
+{
+#pragma unroll 2
+    for (int i = 0; i < n; ++i) {
+        const float4 q = src[i];
+        const float bq = srcf[4 * i] * es[i];
+#pragma unroll
+        for (int g = 0; g < R; ++g) {
+            if (!((MASK >> g) & 1)) continue;
+            const float du = q.x - um[g], dv = q.y - vm[g], dg = q.z - g0[g];
+            float t = fmaf(dg, dg, ck[g]);
+            t = fmaf(dv, dv, t);
+            t = fmaf(du, du, t);
+            const float w = __builtin_amdgcn_exp2f(-t);
+            acc[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(w, bq, acc[g], 0, 0, 0);
+        }
+    }
+}
+
+// groups whose rows see source row ys form a contiguous range; one loop per mask
+template <int R, int M>
+__device__ __forceinline__ void tri_taps_mask(int mask, const float4 *__restrict__ src,
+                                              const float *__restrict__ srcf, const float *__restrict__ es, int n,
+                                              const float (&um)[R], const float (&vm)[R], const float (&g0)[R],
+                                              const float (&ck)[R], v4f (&acc)[R])
+{
+    if constexpr (M > 0) {
+        if (mask == M)
+            tri_taps<R, M>(src, srcf, es, n, um, vm, g0, ck, acc);
+        else
+            tri_taps_mask<R, M - 1>(mask, src, srcf, es, n, um, vm, g0, ck, acc);
+    }
+}
+
+template <int R>
+__global__ void __launch_bounds__(kThreads) k_trilateral_mfma(const float *__restrict__ Ui,
+                                                              const float *__restrict__ Vi,
+                                                              const float *__restrict__ Um,
+                                                              const float *__restrict__ Vm,
+                                                              const float *__restrict__ I, float *__restrict__ Uo,
+                                                              float *__restrict__ Vo, int W, int H, int s, int BR,
+                                                              float a, float bI, const float *__restrict__ spt)
+{
+    constexpr int TH = 4 * R;
+    extern __shared__ float4 band[];
+    const int pitch = kTriTW + 2 * s;
+    // exp2 of minus the column part of the spatial term (2s entries) after the band
+    float *es = reinterpret_cast<float *>(band + (size_t)BR * pitch);
+    for (int i = threadIdx.x; i < 2 * s; i += kThreads) es[i] = __builtin_amdgcn_exp2f(-spt[i]);
+    const int gxb = (W + kTriTW - 1) / kTriTW, gyb = (H + TH - 1) / TH;
+    const int blk = xcd_remap(blockIdx.x, gxb * gyb);
+    const int tx0 = (blk % gxb) * kTriTW, ty0 = (blk / gxb) * TH;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = wave * 16 + (lane >> 2);  // output column within the tile
+    const int k = lane & 3;                   // row of the group (A), component (B)
+    const int comp = (k + 3) & 3;             // B: 1 (the .w), u, v, I
+    const int x = tx0 + col, xc = min(x, W - 1);
+
+    float um[R], vm[R], g0[R];
+    v4f acc[R];
+#pragma unroll
+    for (int g = 0; g < R; ++g) {
+        const size_t o = (size_t)min(ty0 + 4 * g + k, H - 1) * W + xc;
+        um[g] = Um[o] * a;
+        vm[g] = Vm[o] * a;
+        g0[g] = I[o] * bI;
+        acc[g] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+
+    const int src0 = ty0 - s, src1 = ty0 + TH + s;  // source rows of the tile
+    for (int band0 = src0; band0 < src1; band0 += BR) {
+        __syncthreads();
+        for (int r = wave; r < BR; r += 4) {
+            const int gy = clampi(band0 + r, 0, H - 1);
+            const float *ru = Ui + (size_t)gy * W, *rv = Vi + (size_t)gy * W, *ri = I + (size_t)gy * W;
+            for (int c = lane; c < pitch; c += 64) {
+                const int gx = clampi(tx0 - s + c, 0, W - 1);
+                band[r * pitch + c] = make_float4(ru[gx] * a, rv[gx] * a, ri[gx] * bI, 1.0f);
+            }
+        }
+        __syncthreads();
+        const int rend = min(BR, src1 - band0);
+        for (int r = 0; r < rend; ++r) {
+            const int ys = band0 + r;
+            float ck[R];
+            int mask = 0;  // wave-uniform: groups with a row inside the window of ys
+#pragma unroll
+            for (int g = 0; g < R; ++g) {
+                const int j = ys - (ty0 + 4 * g + k);
+                ck[g] = (j >= -s && j < s) ? spt[j + s] : __builtin_inff();
+                const int y0g = ty0 + 4 * g;
+                if (ys >= y0g - s && ys < y0g + 3 + s) mask |= 1 << g;
+            }
+            const float4 *src = band + r * pitch + col;
+            const float *srcf = reinterpret_cast<const float *>(src) + comp;
+            tri_taps_mask<R, (1 << R) - 1>(mask, src, srcf, es, 2 * s, um, vm, g0, ck, acc);
+        }
+    }
+    // lane 4b+j holds, in register i, row i's sum of w * (1, u, v, I)_j * es
+    const float ia = 1.0f / a;
+#pragma unroll
+    for (int g = 0; g < R; ++g) {
+        float den[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) den[i] = fmaxf(1e-5f, __shfl(acc[g][i], lane & ~3));
+        if (x < W && (k == 1 || k == 2)) {
+            float *out = k == 1 ? Uo : Vo;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int y = ty0 + 4 * g + i;
+                if (y < H) out[(size_t)y * W + x] = (acc[g][i] * ia) / den[i];
             }
         }
     }
@@ -1169,9 +1299,22 @@ hipError_t launch_trilateral(const Geometry &g, const float *Ui, const float *Vi
     if (BR < 1) return hipErrorInvalidValue;
     if (BR > g.tri_band) BR = g.tri_band;
     const size_t lds = (size_t)BR * pitch * sizeof(float4) + (size_t)2 * s * sizeof(float);
-    const int gxb = (g.W + kTriTW - 1) / kTriTW, gyb = (g.H + kTriTH - 1) / kTriTH;
-    hipLaunchKernelGGL(k_trilateral, dim3(gxb * gyb), dim3(kThreads), lds, st, Ui, Vi, Um, Vm, I, Uo, Vo, g.W, g.H,
-                       s, BR, g.tri_a, g.tri_b, g.tri_spatial);
+    const int gxb = (g.W + kTriTW - 1) / kTriTW;
+    switch (g.tri_groups) {
+    case 0:
+        hipLaunchKernelGGL(k_trilateral, dim3(gxb * ((g.H + kTriTH - 1) / kTriTH)), dim3(kThreads), lds, st, Ui, Vi,
+                           Um, Vm, I, Uo, Vo, g.W, g.H, s, BR, g.tri_a, g.tri_b, g.tri_spatial);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_trilateral_mfma<2>, dim3(gxb * ((g.H + 7) / 8)), dim3(kThreads), lds, st, Ui, Vi, Um, Vm,
+                           I, Uo, Vo, g.W, g.H, s, BR, g.tri_a, g.tri_b, g.tri_spatial);
+        break;
+    case 4:
+        hipLaunchKernelGGL(k_trilateral_mfma<4>, dim3(gxb * ((g.H + 15) / 16)), dim3(kThreads), lds, st, Ui, Vi, Um,
+                           Vm, I, Uo, Vo, g.W, g.H, s, BR, g.tri_a, g.tri_b, g.tri_spatial);
+        break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -202,6 +202,8 @@ typedef struct bm_tuning {
     int brief_near;   /* LDS near halo of the fused BRIEF: 8 .. 48 in steps of 8 (40, 48 need > 64 KiB of LDS
                          per work-group, as on gfx950); 0 = kernel with global far taps */
     int tri_band;     /* source rows per LDS band of the trilateral filter: 1..(LDS limit, <= 16) */
+    int tri_groups;   /* trilateral kernel: 0 = weighted sums on the VALU, 4 output rows per thread;
+                         2 or 4 = weighted sums on MFMA (4x4x1 f32), that many 4-row groups per wave */
 } bm_tuning;
 
 int bm_get_tuning(const bm_ctx *ctx, bm_tuning *out);

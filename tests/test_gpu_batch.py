@@ -36,13 +36,21 @@ def test_tuning_does_not_change_results(N):
         got = _flow(ctx, pr)
         for a, b in zip(ref, got):
             assert np.array_equal(a, b), (lp, strip, near, band)
+    # the trilateral kernels sum in different orders: equal within the flow tolerance
+    for groups in (0, 2, 4):
+        ctx.set_tuning(tri_groups=groups)
+        got = _flow(ctx, pr)
+        for a, b in zip(ref[:2], got[:2]):
+            assert np.array_equal(a, b), groups
+        for a, b in zip(ref[2:], got[2:]):
+            assert np.abs(a - b).max() <= 1e-3, groups
 
 
 def test_tuning_rejects_invalid():
     p = bm.derive_params(64, 48, 2.0, 2.0, N=512)
     ctx = bm.Context(p)
     for kw in ({"level_px": 3}, {"level_px": 2}, {"level_px": 4}, {"strip_rows": 8}, {"brief_near": 12},
-               {"tri_band": 0}):
+               {"tri_band": 0}, {"tri_groups": 1}, {"tri_groups": 8}):
         with pytest.raises(bm.BriefMatchError):
             ctx.set_tuning(**kw)
 

@@ -205,6 +205,39 @@ def test_refinement_stages():
     assert np.abs(Uo.download((H, W), np.float32) - tU).max() <= FLOW_TOL
 
 
+@pytest.mark.parametrize("W,H,sS", [(150, 100, 6.0), (67, 45, 12.0), (40, 30, 20.0), (129, 17, 3.0)])
+def test_trilateral_kernels(W, H, sS):
+    """Every trilateral kernel (VALU sums; MFMA sums with 2 or 4 row groups) against
+    the oracle on ragged frames, windows up to wider than the frame."""
+    say("trilateral kernels", W, H, sS)
+    rng = np.random.default_rng(W * 1000 + H)
+    p = bm.derive_params(W, H, 1.0, 1.0, N=32)
+    p.sS = sS
+    ctx = bm.Context(p, seed=1)
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float32)
+    # piecewise-smooth flow with a motion boundary, noise, and a textured guide
+    U = (np.where(xx > W / 2, 6.0, -3.0) + 0.05 * yy + rng.normal(0, 0.7, (H, W))).astype(np.float32)
+    V = (np.where(yy > H / 3, -4.0, 2.5) + rng.normal(0, 0.7, (H, W))).astype(np.float32)
+    Um = (U + rng.normal(0, 0.3, (H, W))).astype(np.float32)
+    Vm = (V + rng.normal(0, 0.3, (H, W))).astype(np.float32)
+    guide = rng.integers(0, 256, (H, W)).astype(np.float32)
+    tU, tV = oracle.trilateral(U, V, Um, Vm, guide, p.sEPE ** 2, p.sI ** 2, p.sS ** 2)
+    ins = [bm.DeviceBuffer(array=a) for a in (U, V, Um, Vm, guide)]
+    got = {}
+    for groups in (0, 2, 4):
+        ctx.set_tuning(tri_groups=groups)
+        Uo, Vo = ctx.image_buffer(), ctx.image_buffer()
+        ctx.trilateral(*ins, Uo, Vo)
+        ctx.sync()
+        gU, gV = Uo.download((H, W), np.float32), Vo.download((H, W), np.float32)
+        eu, ev = np.abs(gU - tU).max(), np.abs(gV - tV).max()
+        print("[tri]", W, H, sS, groups, eu, ev, flush=True)
+        assert eu <= FLOW_TOL and ev <= FLOW_TOL, (groups, eu, ev)
+        got[groups] = (gU, gV)
+    for groups in (2, 4):
+        assert np.abs(got[groups][0] - got[0][0]).max() <= FLOW_TOL
+
+
 @pytest.mark.parametrize("filt", [1, 0])
 def test_flow_pair_c1(filt):
     """The 320x240 / u1.5 / 32-bit workload, whole path, against the oracle."""
