@@ -60,23 +60,24 @@ def _offset(buf, nbytes):
     return C.c_void_p(buf.ptr.value + int(nbytes))
 
 
-def train_pair(seed, W=640, H=480):
+def train_pair(seed, W=640, H=480, with_gt=False):
     """Training pair `seed`: moving rectangles over an affine background
     (the family of the 1920x1080 benchmark pair), motion up to 16..40 px, the
     frames a window of a scene 1 to 3 times their size (so texture and
-    rectangle scales span those of 640x480 and 1920x1080 renderings)."""
+    rectangle scales span those of 640x480 and 1920x1080 renderings).
+    Returns (frame1, frame2) or, with_gt, (frame1, frame2, gt flow (H, W, 2))."""
     rng = np.random.default_rng([seed, 7])
     n_rects = int(rng.integers(3, 11))
     max_motion = float(rng.uniform(16.0, 40.0))
     k = float(rng.uniform(1.0, 3.0))
     p = synth.make_pair(W, H, motion="rects", n_rects=n_rects, max_motion=max_motion, seed=seed,
                         scene=(int(W * k), int(H * k)))
-    return p.frame1, p.frame2
+    return (p.frame1, p.frame2, p.gt) if with_gt else (p.frame1, p.frame2)
 
 
 def _gen(args):
-    seed, W, H = args
-    return train_pair(seed, W, H)
+    seed, W, H, with_gt = args
+    return train_pair(seed, W, H, with_gt)
 
 
 class Dataset:
@@ -94,10 +95,15 @@ class Dataset:
         return self.label.download((self.rows, self.ch, self.cw, 2), np.float32)
 
 
-def build_dataset(pairs, seed0=1000, W=640, H=480, up=1.0, N=128, iterations=4, workers=16, log=print):
-    """Generate `pairs` seeded pairs (seeds seed0..), label them with the plain
-    method (`iterations` propagation iterations) and store features and labels
-    on the device."""
+def build_dataset(pairs, seed0=1000, W=640, H=480, up=1.0, N=128, iterations=4, workers=16, labels="plain",
+                  log=print):
+    """Generate `pairs` seeded pairs (seeds seed0..), label them and store
+    features and labels on the device.  labels="plain": the plain method's
+    flow (`iterations` propagation iterations); labels="gt": the pairs' known
+    flow.  Either is reduced to 8x8 block means in coarse pixels."""
+    if labels not in ("plain", "gt"):
+        raise ValueError("labels must be 'plain' or 'gt'")
+    with_gt = labels == "gt"
     params = derive_params(W, H, up, up, N=N)
     params.iterations = iterations
     ctx = Context(params, seed=0)
@@ -107,15 +113,21 @@ def build_dataset(pairs, seed0=1000, W=640, H=480, up=1.0, N=128, iterations=4, 
     U, V = ctx.image_buffer(), ctx.image_buffer()
     g8, F = ctx.frames()
     t0 = time.time()
-    jobs = [(seed0 + i, W, H) for i in range(pairs)]
+    jobs = [(seed0 + i, W, H, with_gt) for i in range(pairs)]
     pool = mp.get_context("spawn").Pool(workers) if workers > 1 else None
     it = pool.imap(_gen, jobs, chunksize=4) if pool else map(_gen, jobs)
     try:
-        for i, (f1, f2) in enumerate(it):
+        for i, job in enumerate(it):
             ctx.sync()
-            r1.upload(f1)
-            r2.upload(f2)
-            ctx.flow_pair(r1, r2, U, V, pair=0)
+            r1.upload(job[0])
+            r2.upload(job[1])
+            if with_gt:
+                ctx.gray(r1, g8[0])
+                ctx.gray(r2, g8[1])
+                U.upload(np.ascontiguousarray(job[2][..., 0]))
+                V.upload(np.ascontiguousarray(job[2][..., 1]))
+            else:
+                ctx.flow_pair(r1, r2, U, V, pair=0)
             ctx.coarse_flow(U, V, _offset(ds.label, i * ds.label_row))
             ctx.proposal_features(g8[0], g8[1], _offset(ds.feat, i * ds.feat_row))
             if log and ((i + 1) % 1000 == 0 or i + 1 == pairs):
@@ -220,6 +232,8 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--workers", type=int, default=16)
     t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--labels", choices=["plain", "gt"], default="plain",
+                   help="label flow: the plain method's output or the pairs' known flow")
     t.add_argument("--out", required=True)
     t.add_argument("--eval", action="store_true", help="evaluate on the 1920x1080 workload afterwards")
     e = sub.add_parser("eval")
@@ -229,7 +243,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     if a.cmd == "train":
-        ds = build_dataset(a.pairs, workers=a.workers, log=log)
+        ds = build_dataset(a.pairs, workers=a.workers, labels=a.labels, log=log)
         net = Net(weights=init_weights(a.seed))
         rows = np.arange(a.pairs - a.val)
         val = np.arange(a.pairs - a.val, a.pairs)
