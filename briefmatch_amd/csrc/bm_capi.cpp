@@ -80,6 +80,9 @@ struct bm_ctx {
     float *d_nout = nullptr;               // [ch][cw][16]
     int32_t *d_coff = nullptr;             // [ch][cw][4] Q8
     int32_t *d_bsum = nullptr;             // [2][ch][cw] block sums
+    // the proposal network runs on a second stream beside the descriptor kernels
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     // sequence mode
     int have_prev = 0, cur = 0, field_valid = 0;
     uint32_t frame_counter = 0;
@@ -131,23 +134,25 @@ hipEvent_t take_event(bm_ctx *ctx)
 }
 
 // Enqueue one kernel; while profiling, bracket it with events on the stream.
-#define BM_LAUNCH(ctx, stage, call)                                                  \
+// a launch on stream st, bracketed by timing events when profiling is on
+#define BM_LAUNCH_ON(ctx, stage, st, call)                                           \
     do {                                                                             \
         hipEvent_t a_ = nullptr;                                                     \
         if ((ctx)->prof && (a_ = take_event(ctx)) != nullptr)                        \
-            (void)hipEventRecord(a_, (ctx)->stream);                                 \
+            (void)hipEventRecord(a_, (st));                                          \
         hipError_t e_ = (call);                                                      \
         if (e_ != hipSuccess) return hip_fail((ctx), e_, #call);                     \
         if (a_) {                                                                    \
             hipEvent_t b_ = take_event(ctx);                                         \
             if (b_) {                                                                \
-                (void)hipEventRecord(b_, (ctx)->stream);                             \
+                (void)hipEventRecord(b_, (st));                                      \
                 (ctx)->pending.push_back({(stage), a_, b_});                         \
             } else {                                                                 \
                 (ctx)->pool.push_back(a_);                                           \
             }                                                                        \
         }                                                                            \
     } while (0)
+#define BM_LAUNCH(ctx, stage, call) BM_LAUNCH_ON(ctx, stage, (ctx)->stream, call)
 
 template <class T>
 int dev_alloc(bm_ctx *ctx, T **ptr, size_t count)
@@ -684,6 +689,12 @@ void bm_ctx_destroy(bm_ctx *ctx)
         (void)hipEventDestroy(pe.b);
     }
     for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
+    if (ctx->side) {
+        (void)hipStreamSynchronize(ctx->side);
+        (void)hipStreamDestroy(ctx->side);
+    }
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+    if (ctx->join) (void)hipEventDestroy(ctx->join);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1040,18 +1051,36 @@ int proposal_ready(bm_ctx *ctx)
     if (!r) r = dev_alloc(ctx, &ctx->d_nout, cells * bm::net::kOutC);
     if (!r) r = dev_alloc(ctx, &ctx->d_coff, cells * 4);
     if (!r) r = dev_alloc(ctx, &ctx->d_bsum, cells * 2);
-    if (r) ctx->d_feat = nullptr;
-    return r;
+    if (r) {
+        ctx->d_feat = nullptr;
+        return r;
+    }
+    if (!ctx->side) {
+        // A stream with a CU mask (here every CU) gets a hardware queue of its
+        // own; plain streams beyond the runtime's queue count share queues, and
+        // work on a shared queue runs in submission order, never side by side.
+        int dev = 0, cus = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        std::vector<uint32_t> mask((size_t)std::max(1, (cus + 31) / 32), 0xffffffffu);
+        if (e == hipSuccess) e = hipExtStreamCreateWithCUMask(&ctx->side, (uint32_t)mask.size(), mask.data());
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            ctx->d_feat = nullptr;
+            return hip_fail(ctx, e, "proposal stream");
+        }
+    }
+    return BM_OK;
 }
 
-hipError_t launch_offsets(bm_ctx *ctx, const float *nout, const uint16_t *feat, int32_t *coff)
+hipError_t launch_offsets(bm_ctx *ctx, const float *nout, const uint16_t *feat, int32_t *coff, hipStream_t st)
 {
     const bm::Geometry &g = ctx->g;
     const int cw = g.W / bm::net::kCell, ch = g.H / bm::net::kCell;
     const float scx = (float)(256 * bm::net::kCell * g.sx) / (float)g.W;
     const float scy = (float)(256 * bm::net::kCell * g.sy) / (float)g.H;
-    return bm::net::launch_proposal_offsets(nout, feat, cw * ch, scx, scy, 256 * g.sx, 256 * g.sy, coff,
-                                            ctx->stream);
+    return bm::net::launch_proposal_offsets(nout, feat, cw * ch, scx, scy, 256 * g.sx, 256 * g.sy, coff, st);
 }
 
 }  // namespace
@@ -1089,7 +1118,7 @@ int bm_proposal_offsets(bm_ctx *ctx, const float *nout, const uint16_t *feat, in
     if (r) return r;
     if (ctx->g.W < bm::net::kCell || ctx->g.H < bm::net::kCell)
         return fail(ctx, BM_E_PARAM, "the learned proposal needs frames of at least 8x8 pixels");
-    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, launch_offsets(ctx, nout, feat, coff));
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, launch_offsets(ctx, nout, feat, coff, ctx->stream));
     return BM_OK;
 }
 
@@ -1126,16 +1155,23 @@ int bm_flow_pair_learned(bm_ctx *ctx, bm_net *net, const uint8_t *rgb1, const ui
     const int cw = g.W / bm::net::kCell, ch = g.H / bm::net::kCell;
     BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb1, ctx->d_g8[0], ctx->d_g32[0], ctx->stream));
     BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb2, ctx->d_g8[1], nullptr, ctx->stream));
+    // the proposal (features, network, offsets) needs only the gray frames: it
+    // runs on the side stream while the descriptor kernels run on the main one
+    hipStream_t side = ctx->side;
+    BM_HIP(ctx, hipEventRecord(ctx->fork, ctx->stream));
+    BM_HIP(ctx, hipStreamWaitEvent(side, ctx->fork, 0));
+    BM_LAUNCH_ON(ctx, BM_STAGE_PROPOSAL, side,
+                 bm::net::launch_features(ctx->d_g8[0], ctx->d_g8[1], g.W, cw, ch, ctx->d_bsum, ctx->d_feat, side));
+    // the packed weights are written on the net's stream
+    BM_HIP(ctx, hipStreamWaitEvent(side, net->ready, 0));
+    BM_LAUNCH_ON(ctx, BM_STAGE_PROPOSAL, side,
+                 bm::net::forward(net->pk, ctx->d_feat, 1, ch, cw, ctx->d_nact[0], ctx->d_nact[1], ctx->d_nout, side));
+    BM_LAUNCH_ON(ctx, BM_STAGE_PROPOSAL, side, launch_offsets(ctx, ctx->d_nout, ctx->d_feat, ctx->d_coff, side));
+    BM_HIP(ctx, hipEventRecord(ctx->join, side));
     r = describe(ctx, ctx->d_g8[0], ctx->d_F[0]);
     if (!r) r = describe(ctx, ctx->d_g8[1], ctx->d_F[1]);
     if (r) return r;
-    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, bm::net::launch_features(ctx->d_g8[0], ctx->d_g8[1], g.W, cw, ch, ctx->d_bsum,
-                                                                 ctx->d_feat, ctx->stream));
-    // the packed weights are written on the net's stream
-    BM_HIP(ctx, hipStreamWaitEvent(ctx->stream, net->ready, 0));
-    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, bm::net::forward(net->pk, ctx->d_feat, 1, ch, cw, ctx->d_nact[0], ctx->d_nact[1],
-                                                         ctx->d_nout, ctx->stream));
-    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, launch_offsets(ctx, ctx->d_nout, ctx->d_feat, ctx->d_coff));
+    BM_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->join, 0));
     const int dst = ctx->field;
     // the seeded field, then one random search around it on its own stream
     // index (the propagation iterations use 0 .. iterations - 1), in one kernel
