@@ -13,7 +13,8 @@ Prints one JSON line.  Besides the driver's fields it carries:
                 launch time from HIP events on the context stream; "pipeline":
                 the sum over stages of work / peak against the measured step;
   stages        per-stage event times and their own roofline fractions;
-  streaming     sequence mode (one BRIEF per frame, temporal init);
+  streaming     sequence mode (one BRIEF per frame, temporal init), serial and
+                pipelined (each frame's refinement tail beside the next frame);
   learned       the same pair through the learned-proposal path with half the
                 propagation iterations: throughput, per-stage times and EPE
                 next to the plain method's (see --no-learned);
@@ -326,6 +327,18 @@ def main():
     for i in range(ns):
         ctx.stream_frame(rgb2 if i % 2 == 0 else rgb1, U, V)
     st_ms = ctx.timer_stop() / ns
+    # the same sequence pipelined: each frame's medians and trilateral filter
+    # overlap the next frame's descriptors and matching (identical results)
+    ctx.set_pipelining(True)
+    for _ in range(2):
+        ctx.stream_frame(rgb2, U, V)
+        ctx.stream_frame(rgb1, U, V)
+    ctx.sync()
+    ctx.timer_start()
+    for i in range(ns):
+        ctx.stream_frame(rgb2 if i % 2 == 0 else rgb1, U, V)
+    sp_ms = ctx.timer_stop() / ns
+    ctx.set_pipelining(False)
 
     # final flow of the pair for EPE and parity
     ctx.flow_pair(rgb1, rgb2, U, V)
@@ -345,7 +358,8 @@ def main():
         "roofline": roof,
         "stages": stages,
         "streaming": {"ms_per_frame": round(st_ms, 4), "value": round(W * H / (st_ms * 1e3), 3),
-                      "unit": "Mpix/s"},
+                      "unit": "Mpix/s",
+                      "pipelined": {"ms_per_frame": round(sp_ms, 4), "value": round(W * H / (sp_ms * 1e3), 3)}},
         "epe": {"all": round(e_all, 4), "noc": round(e_noc, 4)},
     }
     if not a.no_learned:

@@ -110,6 +110,7 @@ def lib():
         "bm_flow_pair": (I, [V, V, V, U32, V, V]),
         "bm_stream_frame": (I, [V, V, V, V, C.POINTER(I)]),
         "bm_stream_reset": (I, [V]),
+        "bm_set_pipelining": (I, [V, I]),
         "bm_timer_start": (I, [V]),
         "bm_timer_stop": (I, [V, C.POINTER(F)]),
         "bm_epe": (I, [V, V, V, V, V, C.c_size_t, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -398,6 +399,11 @@ class Context:
 
     def stream_reset(self):
         self._c(lib().bm_stream_reset(self.h))
+
+    def set_pipelining(self, on=True):
+        """bm_set_pipelining: the refinement tail of stream_frame on a second
+        stream, overlapping the next frame; results complete after sync()."""
+        self._c(lib().bm_set_pipelining(self.h, int(on)))
 
     def profile_enable(self, on=True):
         self._c(lib().bm_profile_enable(self.h, int(on)))

@@ -80,9 +80,13 @@ struct bm_ctx {
     float *d_nout = nullptr;               // [ch][cw][16]
     int32_t *d_coff = nullptr;             // [ch][cw][4] Q8
     int32_t *d_bsum = nullptr;             // [2][ch][cw] block sums
-    // the proposal network runs on a second stream beside the descriptor kernels
+    // second stream (own hardware queue): the proposal network beside the
+    // descriptor kernels, and the refinement tail of a pipelined sequence
     hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, tail_done = nullptr;
+    int pipelined = 0;     // bm_set_pipelining
+    int tail_pending = 0;  // a pipelined tail may still read d_U0, d_V0, d_guide
+    float *d_guide = nullptr;  // guide frame of the pipelined tail
     // sequence mode
     int have_prev = 0, cur = 0, field_valid = 0;
     uint32_t frame_counter = 0;
@@ -443,18 +447,38 @@ int run_match(bm_ctx *ctx, const uint32_t *F1, const uint32_t *F2, int first, co
     return BM_OK;
 }
 
-int run_refine(bm_ctx *ctx, const float *U, const float *V, const float *I, float *Uo, float *Vo)
+// The context stream waits for a pipelined refinement tail still in flight
+// (it reads d_U0, d_V0 and d_guide and writes the median buffers).
+int settle_tail(bm_ctx *ctx)
+{
+    if (!ctx->tail_pending) return BM_OK;
+    BM_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->tail_done, 0));
+    ctx->tail_pending = 0;
+    return BM_OK;
+}
+
+// medians x then y, then the trilateral filter, on stream st
+int run_refine_on(bm_ctx *ctx, const float *U, const float *V, const float *I, float *Uo, float *Vo, hipStream_t st)
 {
     const bm::Geometry &g = ctx->g;
-    BM_LAUNCH(ctx, BM_STAGE_MEDIAN, bm::launch_median(g, U, V, ctx->d_tU, ctx->d_tV, 1, ctx->stream));
-    BM_LAUNCH(ctx, BM_STAGE_MEDIAN, bm::launch_median(g, ctx->d_tU, ctx->d_tV, ctx->d_Um, ctx->d_Vm, 0, ctx->stream));
-    BM_LAUNCH(ctx, BM_STAGE_TRILATERAL, bm::launch_trilateral(g, U, V, ctx->d_Um, ctx->d_Vm, I, Uo, Vo, ctx->stream));
+    BM_LAUNCH_ON(ctx, BM_STAGE_MEDIAN, st, bm::launch_median(g, U, V, ctx->d_tU, ctx->d_tV, 1, st));
+    BM_LAUNCH_ON(ctx, BM_STAGE_MEDIAN, st, bm::launch_median(g, ctx->d_tU, ctx->d_tV, ctx->d_Um, ctx->d_Vm, 0, st));
+    BM_LAUNCH_ON(ctx, BM_STAGE_TRILATERAL, st, bm::launch_trilateral(g, U, V, ctx->d_Um, ctx->d_Vm, I, Uo, Vo, st));
     return BM_OK;
+}
+
+int run_refine(bm_ctx *ctx, const float *U, const float *V, const float *I, float *Uo, float *Vo)
+{
+    const int r = settle_tail(ctx);
+    if (r) return r;
+    return run_refine_on(ctx, U, V, I, Uo, Vo, ctx->stream);
 }
 
 int run_flow_tail(bm_ctx *ctx, const float *guide, float *U, float *V)
 {
     const bm::Geometry &g = ctx->g;
+    const int r = settle_tail(ctx);
+    if (r) return r;
     if (!ctx->p.filter) {
         BM_LAUNCH(ctx, BM_STAGE_DOWNSAMPLE, bm::launch_downsample(g, ctx->d_off[ctx->field], U, V, ctx->stream));
         return BM_OK;
@@ -683,6 +707,7 @@ void bm_ctx_destroy(bm_ctx *ctx)
 {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     for (void *p : ctx->allocs) (void)hipFree(p);
     for (auto &pe : ctx->pending) {
         (void)hipEventDestroy(pe.a);
@@ -695,6 +720,7 @@ void bm_ctx_destroy(bm_ctx *ctx)
     }
     if (ctx->fork) (void)hipEventDestroy(ctx->fork);
     if (ctx->join) (void)hipEventDestroy(ctx->join);
+    if (ctx->tail_done) (void)hipEventDestroy(ctx->tail_done);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -714,6 +740,7 @@ int bm_sync(bm_ctx *ctx)
 {
     if (!ctx) return fail(nullptr, BM_E_PARAM, "null context");
     BM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->side) BM_HIP(ctx, hipStreamSynchronize(ctx->side));
     return BM_OK;
 }
 
@@ -1039,6 +1066,30 @@ int bm_ctx_frames(bm_ctx *ctx, uint8_t **g8, uint32_t **F)
 
 namespace {
 
+// The second stream and its events, created on first use.  A stream with a CU
+// mask (here every CU) gets a hardware queue of its own; plain streams beyond
+// the runtime's queue count share queues, and work on a shared queue runs in
+// submission order, never side by side.
+int side_ready(bm_ctx *ctx)
+{
+    if (ctx->side) return BM_OK;
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    std::vector<uint32_t> mask((size_t)std::max(1, (cus + 31) / 32), 0xffffffffu);
+    hipStream_t s = nullptr;
+    if (e == hipSuccess) e = hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data());
+    if (e == hipSuccess && !ctx->fork) e = hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming);
+    if (e == hipSuccess && !ctx->join) e = hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming);
+    if (e == hipSuccess && !ctx->tail_done) e = hipEventCreateWithFlags(&ctx->tail_done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        if (s) (void)hipStreamDestroy(s);
+        return hip_fail(ctx, e, "second stream");
+    }
+    ctx->side = s;
+    return BM_OK;
+}
+
 int proposal_ready(bm_ctx *ctx)
 {
     const int cw = ctx->g.W / bm::net::kCell, ch = ctx->g.H / bm::net::kCell;
@@ -1051,27 +1102,9 @@ int proposal_ready(bm_ctx *ctx)
     if (!r) r = dev_alloc(ctx, &ctx->d_nout, cells * bm::net::kOutC);
     if (!r) r = dev_alloc(ctx, &ctx->d_coff, cells * 4);
     if (!r) r = dev_alloc(ctx, &ctx->d_bsum, cells * 2);
-    if (r) {
-        ctx->d_feat = nullptr;
-        return r;
-    }
-    if (!ctx->side) {
-        // A stream with a CU mask (here every CU) gets a hardware queue of its
-        // own; plain streams beyond the runtime's queue count share queues, and
-        // work on a shared queue runs in submission order, never side by side.
-        int dev = 0, cus = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        std::vector<uint32_t> mask((size_t)std::max(1, (cus + 31) / 32), 0xffffffffu);
-        if (e == hipSuccess) e = hipExtStreamCreateWithCUMask(&ctx->side, (uint32_t)mask.size(), mask.data());
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming);
-        if (e != hipSuccess) {
-            ctx->d_feat = nullptr;
-            return hip_fail(ctx, e, "proposal stream");
-        }
-    }
-    return BM_OK;
+    if (!r) r = side_ready(ctx);
+    if (r) ctx->d_feat = nullptr;
+    return r;
 }
 
 hipError_t launch_offsets(bm_ctx *ctx, const float *nout, const uint16_t *feat, int32_t *coff, hipStream_t st)
@@ -1189,6 +1222,22 @@ int bm_flow_pair_learned(bm_ctx *ctx, bm_net *net, const uint8_t *rgb1, const ui
     return run_flow_tail(ctx, ctx->d_g32[0], U, V);
 }
 
+int bm_set_pipelining(bm_ctx *ctx, int on)
+{
+    if (!ctx) return fail(nullptr, BM_E_PARAM, "null context");
+    if (!on) {
+        const int r = settle_tail(ctx);
+        if (r) return r;
+        ctx->pipelined = 0;
+        return BM_OK;
+    }
+    int r = side_ready(ctx);
+    if (!r && !ctx->d_guide) r = dev_alloc(ctx, &ctx->d_guide, (size_t)ctx->g.W * ctx->g.H);
+    if (r) return r;
+    ctx->pipelined = 1;
+    return BM_OK;
+}
+
 int bm_stream_reset(bm_ctx *ctx)
 {
     if (!ctx) return fail(nullptr, BM_E_PARAM, "null context");
@@ -1215,8 +1264,27 @@ int bm_stream_frame(bm_ctx *ctx, const uint8_t *rgb, float *U, float *V, int *ha
         r = run_match(ctx, ctx->d_F[prev], ctx->d_F[cur], first, first ? nullptr : ctx->d_off[ctx->field],
                       ctx->frame_counter);
         if (r) return r;
-        r = run_flow_tail(ctx, ctx->d_g32[prev], U, V);
-        if (r) return r;
+        if (ctx->pipelined && ctx->p.filter) {
+            // down-sample here, then medians and trilateral on the second stream:
+            // the next frame's descriptors and matching run beside them and wait
+            // for them only before its own down-sampling (settle_tail)
+            r = settle_tail(ctx);
+            if (r) return r;
+            BM_LAUNCH(ctx, BM_STAGE_DOWNSAMPLE,
+                      bm::launch_downsample(g, ctx->d_off[ctx->field], ctx->d_U0, ctx->d_V0, ctx->stream));
+            // the guide is the previous frame, whose gray buffer the next call overwrites
+            BM_HIP(ctx, hipMemcpyAsync(ctx->d_guide, ctx->d_g32[prev], (size_t)g.W * g.H * sizeof(float),
+                                       hipMemcpyDeviceToDevice, ctx->stream));
+            BM_HIP(ctx, hipEventRecord(ctx->fork, ctx->stream));
+            BM_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+            r = run_refine_on(ctx, ctx->d_U0, ctx->d_V0, ctx->d_guide, U, V, ctx->side);
+            if (r) return r;
+            BM_HIP(ctx, hipEventRecord(ctx->tail_done, ctx->side));
+            ctx->tail_pending = 1;
+        } else {
+            r = run_flow_tail(ctx, ctx->d_g32[prev], U, V);
+            if (r) return r;
+        }
         ctx->field_valid = 1;
         ctx->frame_counter++;
         *has_flow = 1;
@@ -1236,6 +1304,8 @@ int bm_timer_start(bm_ctx *ctx)
 int bm_timer_stop(bm_ctx *ctx, float *ms)
 {
     if (!ctx || !ms) return fail(nullptr, BM_E_PARAM, "null argument");
+    const int r = settle_tail(ctx);
+    if (r) return r;
     BM_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     BM_HIP(ctx, hipEventSynchronize(ctx->ev1));
     BM_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
@@ -1253,6 +1323,7 @@ int bm_profile_read(bm_ctx *ctx, double *ms, unsigned *launches)
 {
     if (!ctx) return fail(nullptr, BM_E_PARAM, "null context");
     BM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->side) BM_HIP(ctx, hipStreamSynchronize(ctx->side));
     for (auto &pe : ctx->pending) {
         float t = 0.0f;
         BM_HIP(ctx, hipEventElapsedTime(&t, pe.a, pe.b));

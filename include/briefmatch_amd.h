@@ -191,6 +191,15 @@ int bm_flow_pair(bm_ctx *ctx, const uint8_t *rgb1_dev, const uint8_t *rgb2_dev, 
 int bm_stream_frame(bm_ctx *ctx, const uint8_t *rgb_dev, float *U_dev, float *V_dev, int *has_flow);
 /* Forget the previous frame (the next bm_stream_frame starts a new sequence). */
 int bm_stream_reset(bm_ctx *ctx);
+/* Pipelined sequence mode (off by default; no counterpart in the reference,
+ * whose OFPipeline runs each frame to completion).  When on, bm_stream_frame
+ * runs the medians and the trilateral filter of its frame on a second stream of
+ * the context, so that the next call's descriptors and matching run beside
+ * them; results are identical.  U_dev, V_dev of a frame are complete after
+ * bm_sync(ctx) or bm_timer_stop(ctx), not in order with the stream returned by
+ * bm_ctx_stream; calls on the context that use its refinement buffers wait for
+ * them on the device. */
+int bm_set_pipelining(bm_ctx *ctx, int on);
 
 /* ---- tuning ------------------------------------------------------------------ */
 

@@ -295,6 +295,40 @@ def test_sequence_temporal_init():
         prev = eo
 
 
+@pytest.mark.parametrize("W,H,up,N", [(160, 120, 2.0, 64), (97, 61, 2.9, 128)])
+def test_sequence_pipelined(W, H, up, N):
+    """Pipelined sequence mode (refinement tail on the second stream) gives the
+    serial mode's flow bit for bit, frame by frame, with one output pair per
+    frame and a single synchronisation at the end; a whole-pair call after it
+    sees a settled context."""
+    say("stream pipelined", W, H, up, N)
+    frames = [_pair(W, H, seed=40 + k, max_motion=5.0).frame1 for k in range(5)]
+    p = bm.derive_params(W, H, up, up, N=N)
+    keep = [bm.DeviceBuffer(array=f) for f in frames]
+    res = {}
+    for piped in (False, True):
+        ctx = bm.Context(p, seed=5)
+        if piped:
+            ctx.set_pipelining(True)
+        outs = [(ctx.image_buffer(), ctx.image_buffer()) for _ in frames]
+        for k in range(len(frames)):
+            ctx.stream_frame(keep[k], *outs[k])
+        ctx.sync()
+        res[piped] = [(U.download((H, W), np.float32), V.download((H, W), np.float32)) for U, V in outs[1:]]
+        # a whole-pair call right after pipelined frames (no sync in between)
+        for k in range(len(frames)):
+            ctx.stream_frame(keep[k], *outs[k])
+        U, V = ctx.image_buffer(), ctx.image_buffer()
+        ctx.flow_pair(keep[0], keep[1], U, V)
+        ctx.sync()
+        res[(piped, "pair")] = (U.download((H, W), np.float32), V.download((H, W), np.float32))
+        ctx.close()
+    for a, b in zip(res[False], res[True]):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.array_equal(res[(False, "pair")][0], res[(True, "pair")][0])
+    assert np.array_equal(res[(False, "pair")][1], res[(True, "pair")][1])
+
+
 def _field_invariants(off, cost, F1, F2, NS, row0=0, row_step=1):
     """Size-independent checks of a final field on the rows row0::row_step:
     in-image targets, and the stored cost is the Hamming distance between the
