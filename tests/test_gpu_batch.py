@@ -22,7 +22,7 @@ def _flow(ctx, pr, pair=0):
     return off, cost, U, V
 
 
-@pytest.mark.parametrize("N", [32, 128, 512])
+@pytest.mark.parametrize("N", [32, 128, 256, 512])
 def test_tuning_does_not_change_results(N):
     say("tuning invariance", N)
     pr = synth.make_pair(150, 100, motion="rects", max_motion=8.0, seed=N)

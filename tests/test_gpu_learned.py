@@ -63,7 +63,8 @@ def test_features_and_labels_bit_exact(W, H, up, N):
     assert np.array_equal(lab.download((ch, cw, 2), np.float32), onet.coarse_flow(Uh, Vh))
 
 
-@pytest.mark.parametrize("W,H,up,N", [(320, 240, 1.5, 32), (200, 136, 2.9, 128), (97, 43, 2.2, 512)])
+@pytest.mark.parametrize("W,H,up,N", [(320, 240, 1.5, 32), (200, 136, 2.9, 128), (97, 43, 2.2, 512),
+                                      (120, 64, 2.0, 256)])
 def test_offsets_and_seeded_init_bit_exact(W, H, up, N):
     say("seeded init", W, H, up, N)
     ctx, pr, _ = _flow_ctx(W, H, up, N, seed=5)

@@ -61,7 +61,7 @@ def _pad(F, NSP):
 
 
 @pytest.mark.parametrize("W,H,up,N", [(320, 240, 1.5, 32), (48, 32, 2.0, 512), (61, 37, 2.3, 72),
-                                      (96, 64, 1.0, 128), (80, 50, 3.1, 100)])
+                                      (96, 64, 1.0, 128), (80, 50, 3.1, 100), (70, 44, 2.4, 200)])
 def test_gray_upsample_brief(W, H, up, N):
     say("gray/upsample/brief", W, H, up, N)
     pr = _pair(W, H, seed=W + N)
@@ -88,7 +88,7 @@ def test_gray_upsample_brief(W, H, up, N):
     assert np.array_equal(ctx.download_desc(Ff), oF)
 
 
-@pytest.mark.parametrize("N,double_prop", [(32, True), (128, True), (128, False), (512, True)])
+@pytest.mark.parametrize("N,double_prop", [(32, True), (128, True), (128, False), (256, True), (512, True)])
 def test_levels_random_search_init(N, double_prop):
     """Level / random-search / init parity from a random (non-trivial) snapshot."""
     say("levels", N, double_prop)
