@@ -641,6 +641,16 @@ int bm_ctx_create(const bm_params *params, uint64_t seed, bm_ctx **out)
         nnx[x] = (uint32_t)std::min<uint64_t>((2ull * x * p.sx + p.sx_in) / (2ull * p.sx_in), p.sx - 1);
     for (unsigned y = 0; y < p.sy_in; ++y)
         nny[y] = (uint32_t)std::min<uint64_t>((2ull * y * p.sy + p.sy_in) / (2ull * p.sy_in), p.sy - 1);
+    // largest field window of a down-sampling output tile (NN samples +- P/2)
+    {
+        unsigned cols = 0, rows = 0;
+        for (unsigned x0 = 0; x0 < p.sx_in; x0 += bm::kDsTileW)
+            cols = std::max(cols, nnx[std::min(x0 + bm::kDsTileW, p.sx_in) - 1] - nnx[x0] + 1);
+        for (unsigned y0 = 0; y0 < p.sy_in; y0 += bm::kDsTileH)
+            rows = std::max(rows, nny[std::min(y0 + bm::kDsTileH, p.sy_in) - 1] - nny[y0] + 1);
+        const size_t bytes = (size_t)(cols + g.P - 1) * (rows + g.P - 1) * sizeof(uint32_t);
+        g.ds_lds = bytes <= (size_t)bm::kDsLdsMax ? (int)bytes : 0;
+    }
 
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);

@@ -38,6 +38,7 @@ struct Geometry {
     const int16_t *pattern;       // 4N
     const uint32_t *up_x, *up_y;  // bilinear (x0 << 8 | frac) per matched column / row
     const uint32_t *nn_x, *nn_y;  // nearest-neighbour source column / row per input pixel
+    int ds_lds;                   // LDS bytes of the tiled down-sampling (largest tile window), 0 = direct kernel
     const float *gauss;           // P taps
     const float *tri_spatial;     // 2s entries: log2(e) i^2 / (2 sigmaS^2), i = -s..s-1
     const int *lz_x0, *lz_y0;     // Lanczos-3: first source column / row per output column / row
@@ -73,6 +74,8 @@ hipError_t launch_init_proposal(const Geometry &g, const uint32_t *F1, const uin
                                 const uint32_t *prev, const int32_t *coff, int cw, int ch, int rs_iteration,
                                 uint32_t pair, uint32_t *off, uint16_t *cost, hipStream_t st);
 hipError_t launch_downsample(const Geometry &g, const uint32_t *off, float *U, float *V, hipStream_t st);
+// Output tile of the LDS-tiled down-sampling (columns x rows of input pixels).
+constexpr int kDsTileW = 64, kDsTileH = 4, kDsLdsMax = 64 * 1024;
 hipError_t launch_median(const Geometry &g, const float *in0, const float *in1, float *out0, float *out1,
                          int dir, hipStream_t st);
 hipError_t launch_trilateral(const Geometry &g, const float *Ui, const float *Vi, const float *Um,

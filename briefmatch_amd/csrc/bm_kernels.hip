@@ -750,6 +750,52 @@ __global__ void __launch_bounds__(kThreads) k_downsample(const uint32_t *__restr
     V[o] = gv * scy;
 }
 
+// The same down-sampling per output tile of kDsTileW x kDsTileH input pixels:
+// the field window the tile's NN samples need (clamped to the frame, as the
+// taps are) is staged in LDS once, so the P x P taps of neighbouring outputs,
+// whose windows overlap, read LDS instead of memory.  Same products in the
+// same order as k_downsample.
+__global__ void __launch_bounds__(kThreads) k_downsample_tile(const uint32_t *__restrict__ off,
+                                                              const uint32_t *__restrict__ nn_x,
+                                                              const uint32_t *__restrict__ nn_y,
+                                                              const float *__restrict__ gauss, int P, int sx,
+                                                              int sy, int W, int H, float scx, float scy,
+                                                              float *__restrict__ U, float *__restrict__ V)
+{
+    extern __shared__ uint32_t win[];
+    const int gxb = (W + kDsTileW - 1) / kDsTileW, gyb = (H + kDsTileH - 1) / kDsTileH;
+    const int blk = xcd_remap(blockIdx.x, gxb * gyb);
+    const int X0 = (blk % gxb) * kDsTileW, Y0 = (blk / gxb) * kDsTileH;
+    const int P2 = P / 2;
+    const int cx0 = (int)nn_x[X0] - P2, ry0 = (int)nn_y[Y0] - P2;
+    const int cols = (int)nn_x[min(X0 + kDsTileW, W) - 1] + P2 - cx0 + 1;
+    const int rows = (int)nn_y[min(Y0 + kDsTileH, H) - 1] + P2 - ry0 + 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int r = wave; r < rows; r += kThreads / 64) {
+        const uint32_t *row = off + (size_t)clampi(ry0 + r, 0, sy - 1) * sx;
+        for (int c = lane; c < cols; c += 64) win[r * cols + c] = row[clampi(cx0 + c, 0, sx - 1)];
+    }
+    __syncthreads();
+    const int x = X0 + (threadIdx.x % kDsTileW), y = Y0 + (threadIdx.x / kDsTileW);
+    if (x >= W || y >= H) return;
+    const int wx = (int)nn_x[x] - P2 - cx0, wy = (int)nn_y[y] - P2 - ry0;
+    float gu = 0.0f, gv = 0.0f;
+    for (int j = 0; j < P; ++j) {
+        const uint32_t *row = win + (wy + j) * cols + wx;
+        float hu = 0.0f, hv = 0.0f;
+        for (int i = 0; i < P; ++i) {
+            const uint32_t v = row[i];
+            hu += gauss[i] * (float)off_dx(v);
+            hv += gauss[i] * (float)off_dy(v);
+        }
+        gu += gauss[j] * hu;
+        gv += gauss[j] * hv;
+    }
+    const size_t o = (size_t)y * W + x;
+    U[o] = gu * scx;
+    V[o] = gv * scy;
+}
+
 // ---------------------------------------------------------------------------
 // 1-D median: the fsize window (clamped) in registers, partially sorted with
 // min/max exchanges exactly as far as the middle element.
@@ -1263,8 +1309,15 @@ hipError_t launch_downsample(const Geometry &g, const uint32_t *off, float *U, f
 {
     const int identity = (g.sx == g.W && g.sy == g.H) ? 1 : 0;
     const float scx = (float)g.W / (float)g.sx, scy = (float)g.H / (float)g.sy;
-    hipLaunchKernelGGL(k_downsample, dim3(blocks_rows(g.W, g.H)), dim3(kThreads), 0, st, off, g.nn_x, g.nn_y,
-                       g.gauss, g.P, g.sx, g.sy, g.W, g.H, scx, scy, identity, U, V);
+    static_assert(kDsTileW * kDsTileH == kThreads, "one output pixel per thread");
+    if (!identity && g.ds_lds > 0) {
+        const int tiles = ((g.W + kDsTileW - 1) / kDsTileW) * ((g.H + kDsTileH - 1) / kDsTileH);
+        hipLaunchKernelGGL(k_downsample_tile, dim3(tiles), dim3(kThreads), (size_t)g.ds_lds, st, off, g.nn_x, g.nn_y,
+                           g.gauss, g.P, g.sx, g.sy, g.W, g.H, scx, scy, U, V);
+    } else {
+        hipLaunchKernelGGL(k_downsample, dim3(blocks_rows(g.W, g.H)), dim3(kThreads), 0, st, off, g.nn_x, g.nn_y,
+                           g.gauss, g.P, g.sx, g.sy, g.W, g.H, scx, scy, identity, U, V);
+    }
     return hipGetLastError();
 }
 

@@ -205,6 +205,28 @@ def test_refinement_stages():
     assert np.abs(Uo.download((H, W), np.float32) - tU).max() <= FLOW_TOL
 
 
+@pytest.mark.parametrize("W,H,up,sigma_rad", [(97, 61, 4.9, 0.025), (130, 70, 2.9, 0.4), (64, 48, 12.0, 0.025),
+                                               (300, 9, 1.7, 0.025)])
+def test_downsample_paths(W, H, up, sigma_rad):
+    """Down-sampling on random fields: the LDS-tiled kernel (windows up to
+    64 KB) and the direct kernel (larger windows, here x12 up-sampling) against
+    the oracle, on ragged tiles and wide Gaussians."""
+    say("downsample", W, H, up, sigma_rad)
+    p = bm.derive_params(W, H, up, up, N=32, sigma_ds=sigma_rad)
+    ctx = bm.Context(p, seed=1)
+    rng = np.random.default_rng(W + H)
+    dx = rng.integers(-300, 301, (p.sy, p.sx)).astype(np.int32)
+    dy = rng.integers(-300, 301, (p.sy, p.sx)).astype(np.int32)
+    off = ((dx & 0xffff) | ((dy & 0xffff) << 16)).astype(np.uint32)
+    U, V = ctx.image_buffer(), ctx.image_buffer()
+    ctx.downsample(bm.DeviceBuffer(array=off), U, V)
+    ctx.sync()
+    oU, oV = oracle.downsample(off, W, H, ctx.gauss_kernel())
+    gU, gV = U.download((H, W), np.float32), V.download((H, W), np.float32)
+    print("[ds]", W, H, up, len(ctx.gauss_kernel()), np.abs(gU - oU).max(), np.abs(gV - oV).max(), flush=True)
+    assert np.abs(gU - oU).max() <= 1e-4 and np.abs(gV - oV).max() <= 1e-4
+
+
 @pytest.mark.parametrize("W,H,sS", [(150, 100, 6.0), (67, 45, 12.0), (40, 30, 20.0), (129, 17, 3.0)])
 def test_trilateral_kernels(W, H, sS):
     """Every trilateral kernel (VALU sums; MFMA sums with 2 or 4 row groups) against
