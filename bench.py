@@ -232,7 +232,8 @@ def main():
     ap.add_argument("--workload", default="c2_1920x1080_u2.9_n128")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and parity leg")
-    ap.add_argument("--band-rows", type=int, default=135, help="trilateral rows of the CPU sample")
+    ap.add_argument("--band-rows", type=int, default=540,
+                    help="trilateral rows of the CPU sample (about 10 s of oracle work at 1080p)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--pairs", type=int, default=64, help="pairs of the batch workload")
