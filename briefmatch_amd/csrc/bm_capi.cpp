@@ -407,16 +407,12 @@ int describe(bm_ctx *ctx, const uint8_t *g8, uint32_t *F)
 
 // Ping-pong propagation of one iteration over the context's field buffers,
 // starting from ctx->field; the random search of `iteration` is fused into the
-// last level when fuse_rs.  zero_field: the field holds offset (0, 0) at every
-// pixel (the first iteration after a zero initialisation); then every level is
-// the identity (each candidate equals the pixel's own offset and is skipped,
-// the cost is kept), so only the last level, carrying the random search, runs.
+// last level when fuse_rs.
 int run_iteration(bm_ctx *ctx, const uint32_t *F1, const uint32_t *F2, int pd, int iteration, uint32_t pair,
-                  bool fuse_rs, bool zero_field = false)
+                  bool fuse_rs)
 {
     const bm::Geometry &g = ctx->g;
     for (int p = g.jfMax; p > 0; p /= 2) {
-        if (zero_field && p > 1) continue;
         const int rs = (fuse_rs && p == 1) ? iteration : -1;
         const int a = ctx->field, b = 1 - a;
         BM_LAUNCH(ctx, (rs >= 0 ? BM_STAGE_LEVEL_RS : BM_STAGE_LEVEL), bm::launch_level(g, F1, F2, ctx->d_off[a], ctx->d_cost[a], ctx->d_off[b], ctx->d_cost[b], p, pd,
@@ -436,12 +432,19 @@ int run_match(bm_ctx *ctx, const uint32_t *F1, const uint32_t *F2, int first, co
     const bm::Geometry &g = ctx->g;
     int dst = ctx->field;
     if (!first && prev == ctx->d_off[dst]) dst = 1 - dst;
-    BM_LAUNCH(ctx, BM_STAGE_INIT, bm::launch_init(g, F1, F2, first, prev, ctx->d_off[dst], ctx->d_cost[dst], ctx->stream));
+    // After a zero initialisation the first iteration is the identity except for
+    // its random search (every propagation candidate equals the pixel's own
+    // (0, 0)): the initialisation kernel inserts that random search itself.
+    const bool fused = first && g.iterations > 0;
+    BM_LAUNCH(ctx, BM_STAGE_INIT, bm::launch_init(g, F1, F2, first, prev, ctx->d_off[dst], ctx->d_cost[dst],
+                                                  fused ? 0 : -1, pair, ctx->stream));
     ctx->field = dst;
     int pd = -1;
     for (int i = 0; i < g.iterations; ++i) {
-        int r = run_iteration(ctx, F1, F2, pd, i, pair, true, i == 0 && first);
-        if (r != BM_OK) return r;
+        if (!(i == 0 && fused)) {
+            const int r = run_iteration(ctx, F1, F2, pd, i, pair, true);
+            if (r != BM_OK) return r;
+        }
         pd = -pd;
     }
     return BM_OK;
@@ -911,7 +914,7 @@ int bm_init(bm_ctx *ctx, const uint32_t *F1, const uint32_t *F2, int first, cons
     if (!r) r = check(ctx, cost, "cost");
     if (r) return r;
     if (!first && (!prev || prev == off)) return fail(ctx, BM_E_PARAM, "temporal init needs a separate prev_off");
-    BM_LAUNCH(ctx, BM_STAGE_INIT, bm::launch_init(ctx->g, F1, F2, first, prev, off, cost, ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_INIT, bm::launch_init(ctx->g, F1, F2, first, prev, off, cost, -1, 0, ctx->stream));
     return BM_OK;
 }
 

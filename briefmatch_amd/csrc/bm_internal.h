@@ -57,8 +57,10 @@ hipError_t launch_gray(const Geometry &g, const uint8_t *rgb, uint8_t *g8, float
 hipError_t launch_upsample(const Geometry &g, const uint8_t *g8, float *L, hipStream_t st);
 hipError_t launch_upsample_lanczos(const Geometry &g, const uint8_t *g8, float *L, hipStream_t st);
 hipError_t launch_brief(const Geometry &g, const float *L, const uint8_t *g8, uint32_t *F, hipStream_t st);
+// rs_iteration >= 0 inserts that iteration's random search after the initial candidate.
 hipError_t launch_init(const Geometry &g, const uint32_t *F1, const uint32_t *F2, int first,
-                       const uint32_t *prev_off, uint32_t *off, uint16_t *cost, hipStream_t st);
+                       const uint32_t *prev_off, uint32_t *off, uint16_t *cost, int rs_iteration, uint32_t pair,
+                       hipStream_t st);
 // One propagation level; rs_iteration >= 0 fuses the random search of that
 // iteration after the level.
 hipError_t launch_level(const Geometry &g, const uint32_t *F1, const uint32_t *F2,

@@ -388,36 +388,6 @@ __global__ void __launch_bounds__(kThreads) k_brief_lds(BriefArgs a)
 // ---------------------------------------------------------------------------
 // field initialisation
 
-template <int NSP>
-__global__ void __launch_bounds__(kThreads) k_init(const uint32_t *__restrict__ F1, const uint32_t *__restrict__ F2,
-                                                   int first, const uint32_t *__restrict__ prev,
-                                                   uint32_t *__restrict__ off, uint16_t *__restrict__ cost,
-                                                   int sx, int sy, int strip_rows)
-{
-    const int gx = (sx + kThreads - 1) / kThreads;
-    int bx, y;
-    strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, strip_rows, bx, y);
-    const int x = bx * kThreads + threadIdx.x;
-    if (x >= sx) return;
-    const int idx = y * sx + x;
-    int nx = 0, ny = 0;
-    if (!first) {
-        const uint32_t o0 = prev[idx];
-        const int dx0 = clampi(off_dx(o0), -x, sx - 1 - x), dy0 = clampi(off_dy(o0), -y, sy - 1 - y);
-        const uint32_t o1 = prev[idx + dy0 * sx + dx0];
-        nx = clampi(off_dx(o1), -x, sx - 1 - x);
-        ny = clampi(off_dy(o1), -y, sy - 1 - y);
-    }
-    uint32_t f1[NSP], f2[NSP];
-    load_desc<NSP>(F1, idx, f1);
-    load_desc<NSP>(F2, idx + ny * sx + nx, f2);
-    off[idx] = pack_off(nx, ny);
-    cost[idx] = (uint16_t)hamming<NSP>(f1, f2);
-}
-
-// ---------------------------------------------------------------------------
-// one jump-flooding level (+ optionally the random search of the iteration)
-
 struct RSArgs {
     int enabled;
     uint32_t iteration, pair, key0, key1;
@@ -447,6 +417,44 @@ __device__ __forceinline__ void random_candidate(const uint32_t (&f1)[NSP], cons
         }
     }
 }
+
+
+// RS: the random search of rs.iteration is inserted after the initial
+// candidate.  After a zero initialisation that is the whole first iteration:
+// every propagation candidate of its levels equals the pixel's own (0, 0) and
+// is skipped, so only the random search of its last level changes the field.
+template <int NSP, bool RS>
+__global__ void __launch_bounds__(kThreads) k_init(const uint32_t *__restrict__ F1, const uint32_t *__restrict__ F2,
+                                                   int first, const uint32_t *__restrict__ prev,
+                                                   uint32_t *__restrict__ off, uint16_t *__restrict__ cost,
+                                                   int sx, int sy, int strip_rows, RSArgs rs)
+{
+    const int gx = (sx + kThreads - 1) / kThreads;
+    int bx, y;
+    strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, strip_rows, bx, y);
+    const int x = bx * kThreads + threadIdx.x;
+    if (x >= sx) return;
+    const int idx = y * sx + x;
+    int nx = 0, ny = 0;
+    if (!first) {
+        const uint32_t o0 = prev[idx];
+        const int dx0 = clampi(off_dx(o0), -x, sx - 1 - x), dy0 = clampi(off_dy(o0), -y, sy - 1 - y);
+        const uint32_t o1 = prev[idx + dy0 * sx + dx0];
+        nx = clampi(off_dx(o1), -x, sx - 1 - x);
+        ny = clampi(off_dy(o1), -y, sy - 1 - y);
+    }
+    uint32_t f1[NSP], f2[NSP];
+    load_desc<NSP>(F1, idx, f1);
+    load_desc<NSP>(F2, idx + ny * sx + nx, f2);
+    uint32_t best = pack_off(nx, ny);
+    int bcost = hamming<NSP>(f1, f2);
+    if constexpr (RS) random_candidate<NSP>(f1, F2, idx, x, y, sx, sy, rs, best, bcost);
+    off[idx] = best;
+    cost[idx] = (uint16_t)bcost;
+}
+
+// ---------------------------------------------------------------------------
+// one jump-flooding level (+ optionally the random search of the iteration)
 
 // PX pixels per thread (256 apart, so every access of a wave stays one
 // coalesced row segment); a tuning choice, results do not depend on it.
@@ -1212,15 +1220,6 @@ hipError_t launch_brief(const Geometry &g, const float *L, const uint8_t *g8, ui
     default: return hipErrorInvalidValue;                                                                       \
     }
 
-hipError_t launch_init(const Geometry &g, const uint32_t *F1, const uint32_t *F2, int first, const uint32_t *prev,
-                       uint32_t *off, uint16_t *cost, hipStream_t st)
-{
-    const dim3 grid(blocks_rows(g.sx, g.sy));
-    BM_NSP_DISPATCH(g.NSP, hipLaunchKernelGGL((k_init<NSP>), grid, dim3(kThreads), 0, st, F1, F2, first, prev, off,
-                                              cost, g.sx, g.sy, g.strip_rows));
-    return hipGetLastError();
-}
-
 static RSArgs make_rs(const Geometry &g, int enabled, uint32_t iteration, uint32_t pair)
 {
     RSArgs rs;
@@ -1231,6 +1230,21 @@ static RSArgs make_rs(const Geometry &g, int enabled, uint32_t iteration, uint32
     rs.key1 = (uint32_t)(g.seed >> 32);
     rs.sigma_q16 = g.rs_sigma_q16;
     return rs;
+}
+
+hipError_t launch_init(const Geometry &g, const uint32_t *F1, const uint32_t *F2, int first, const uint32_t *prev,
+                       uint32_t *off, uint16_t *cost, int rs_iteration, uint32_t pair, hipStream_t st)
+{
+    const dim3 grid(blocks_rows(g.sx, g.sy));
+    const RSArgs rs = make_rs(g, rs_iteration >= 0, (uint32_t)(rs_iteration < 0 ? 0 : rs_iteration), pair);
+    if (rs.enabled) {
+        BM_NSP_DISPATCH(g.NSP, hipLaunchKernelGGL((k_init<NSP, true>), grid, dim3(kThreads), 0, st, F1, F2, first,
+                                                  prev, off, cost, g.sx, g.sy, g.strip_rows, rs));
+    } else {
+        BM_NSP_DISPATCH(g.NSP, hipLaunchKernelGGL((k_init<NSP, false>), grid, dim3(kThreads), 0, st, F1, F2, first,
+                                                  prev, off, cost, g.sx, g.sy, g.strip_rows, rs));
+    }
+    return hipGetLastError();
 }
 
 template <int NSP, bool RS, int PX>
