@@ -351,6 +351,46 @@ def test_sequence_pipelined(W, H, up, N):
     assert np.array_equal(res[(False, "pair")][1], res[(True, "pair")][1])
 
 
+def test_sequence_pipelined_interleaved():
+    """Pipelining with the refinement off (serial path), switched off in the
+    middle of a sequence, and a learned-path call right after pipelined frames:
+    each result equals the one of a context that never pipelined."""
+    say("stream pipelined, interleaved calls")
+    W, H, up, N = 128, 96, 2.0, 64
+    frames = [_pair(W, H, seed=60 + k, max_motion=4.0).frame1 for k in range(4)]
+    keep = [bm.DeviceBuffer(array=f) for f in frames]
+    net = bm.Net(weights=np.zeros(bm.NET_PARAMS, np.float32))
+    out = {}
+    for piped in (False, True):
+        for filt in (1, 0):
+            p = bm.derive_params(W, H, up, up, N=N, filter=filt)
+            ctx = bm.Context(p, seed=9)
+            ctx.set_pipelining(piped)
+            bufs = [(ctx.image_buffer(), ctx.image_buffer()) for _ in frames]
+            for k in range(3):
+                ctx.stream_frame(keep[k], *bufs[k])
+            if piped:
+                ctx.set_pipelining(False)  # settles the pending tail
+            ctx.stream_frame(keep[3], *bufs[3])
+            if filt:
+                ctx.set_pipelining(piped)
+                ctx.stream_frame(keep[0], *bufs[0])  # pending tail again
+                Ul, Vl = ctx.image_buffer(), ctx.image_buffer()
+                ctx.flow_pair_learned(net, keep[1], keep[2], Ul, Vl)
+            ctx.sync()
+            # the sequence's first frame has no flow: its buffers are written only
+            # when frame 0 comes round again (filt = 1)
+            got = [(U.download((H, W), np.float32), V.download((H, W), np.float32))
+                   for U, V in (bufs if filt else bufs[1:])]
+            if filt:
+                got.append((Ul.download((H, W), np.float32), Vl.download((H, W), np.float32)))
+            out[(piped, filt)] = got
+            ctx.close()
+    for filt in (1, 0):
+        for a, b in zip(out[(False, filt)], out[(True, filt)]):
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), filt
+
+
 def _field_invariants(off, cost, F1, F2, NS, row0=0, row_step=1):
     """Size-independent checks of a final field on the rows row0::row_step:
     in-image targets, and the stored cost is the Hamming distance between the
