@@ -9,11 +9,16 @@
 //     up-sampled frame never reaches HBM;
 //   * jump flooding is level-synchronous (one launch per level, ping-pong
 //     buffers), with all four candidate gathers issued before any cost is
-//     compared, and the random search fused into the last level;
+//     compared, and the random search fused into the last level (or, after a
+//     zero initialisation, into the initialisation);
+//   * the proposal-seeded initialisation of the learned path gathers its
+//     candidates in groups and carries the random search that follows it;
 //   * the down-sampling chain is one kernel that evaluates the separable
-//     Gaussian only at the nearest-neighbour sample points;
+//     Gaussian only at the nearest-neighbour sample points, from an LDS copy
+//     of each output tile's field window;
 //   * the trilateral filter streams bands of pre-scaled (u, v, I) through LDS
-//     and evaluates one exp2 per tap.
+//     and evaluates one exp2 per tap; a variant accumulates its three weighted
+//     sums with 4x4x1 FP32 MFMA blocks.
 #include <algorithm>
 
 #include "bm_internal.h"
