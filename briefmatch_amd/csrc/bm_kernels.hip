@@ -272,14 +272,6 @@ __device__ __forceinline__ uint32_t bil_lds(const uint8_t *gt, uint32_t xv, uint
     return __umul24(top, 256u - ay) + __umul24(bot, ay);
 }
 
-// Shift one comparison bit into a descriptor word: a > b for non-negative
-// values compared as unsigned integers (FP32 bit patterns of non-negative
-// floats order like the floats), bit = sign of (b - a), funnel-shifted in.
-__device__ __forceinline__ uint32_t push_bit(uint32_t word, uint32_t a, uint32_t b)
-{
-    return __builtin_amdgcn_alignbit(word, b - a, 31);
-}
-
 template <int NSP, int RN, int PY>
 __global__ void __launch_bounds__(kThreads) k_brief_lds(BriefArgs a)
 {
@@ -337,7 +329,9 @@ __global__ void __launch_bounds__(kThreads) k_brief_lds(BriefArgs a)
     // The plan lists each word's tests near-first: record {o1, o2, bit, 0} with
     // o1/o2 the LDS offsets (row * PITCH + col) of a near test, or the packed
     // (ox, oy) int16 pairs of a far one.  Bits are placed explicitly, so the
-    // order of evaluation does not matter.
+    // order of evaluation does not matter.  A near bit is the sign of b - a on
+    // the FP32 bit patterns: samples are non-negative (gray levels below 256,
+    // patterns below 2^31), and such floats order like their bit patterns.
 #pragma unroll
     for (int w = 0; w < NSP; ++w) {
         const uint4 *rec = a.plan + 32 * w;

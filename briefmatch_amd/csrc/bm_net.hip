@@ -306,7 +306,7 @@ __global__ void __launch_bounds__(256) k_loss(const float *__restrict__ pred, co
     red[threadIdx.x] = l;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
     if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(256) k_sum(const float *__restrict__ x, int n,
     red[threadIdx.x] = s;
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1) {
-        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = red[0];
