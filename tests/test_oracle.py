@@ -479,3 +479,22 @@ def test_golden_stage_vectors():
         off, cost, U, V = oracle.flow_pair(p, d["frame1"], d["frame2"], seed=seed, pair=int(d["pair"]))
         assert np.array_equal(off, d["off"]) and np.array_equal(cost, d["cost"])
         assert np.array_equal(U, d["U"]) and np.array_equal(V, d["V"])
+
+
+def test_median_networks(tmp_path):
+    """The kernels' pruned median-selection networks (bm_median_net.h) select
+    the middle element: exhaustive on 0/1 inputs up to 19 taps, random floats
+    with ties up to 47 taps, checked by a host build of the same header."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no host C++ compiler")
+    here = os.path.dirname(os.path.abspath(__file__))
+    csrc = os.path.join(os.path.dirname(here), "briefmatch_amd", "csrc")
+    exe = str(tmp_path / "median_net_check")
+    subprocess.run([gxx, "-O2", "-std=c++17", "-I" + csrc, os.path.join(here, "median_net_check.cpp"), "-o", exe],
+                   check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("mismatches 0") == 24
