@@ -144,7 +144,8 @@ def lib():
         "bm_ctx_frames": (I, [V, V, V]),
         "bm_proposal_features": (I, [V, V, V, V]),
         "bm_coarse_flow": (I, [V, V, V, V]),
-        "bm_proposal_offsets": (I, [V, V, V, V]),
+        "bm_proposal_far": (I, [V, V, V, V]),
+        "bm_proposal_offsets": (I, [V, V, V, V, V]),
         "bm_init_proposal": (I, [V, V, V, I, V, V, V, V]),
         "bm_flow_pair_learned": (I, [V, V, V, V, U32, V, V]),
     }
@@ -464,8 +465,12 @@ class Context:
     def coarse_flow(self, U, V, label):
         self._c(lib().bm_coarse_flow(self.h, _ptr(U), _ptr(V), _ptr(label)))
 
-    def proposal_offsets(self, net_out, feat, coff):
-        self._c(lib().bm_proposal_offsets(self.h, _ptr(net_out), _ptr(feat), _ptr(coff)))
+    def proposal_far(self, g1, g2, far):
+        """Far block match [H / 16][W / 16][2] int32 of two gray frames."""
+        self._c(lib().bm_proposal_far(self.h, _ptr(g1), _ptr(g2), _ptr(far)))
+
+    def proposal_offsets(self, net_out, feat, coff, far=None):
+        self._c(lib().bm_proposal_offsets(self.h, _ptr(net_out), _ptr(feat), _ptr(far), _ptr(coff)))
 
     def init_proposal(self, F1, F2, coff, off, cost, first=True, prev_off=None):
         self._c(lib().bm_init_proposal(self.h, _ptr(F1), _ptr(F2), 1 if first else 0, _ptr(prev_off), _ptr(coff),

@@ -80,6 +80,7 @@ struct bm_ctx {
     float *d_nout = nullptr;               // [ch][cw][16]
     int32_t *d_coff = nullptr;             // [ch][cw][4] Q8
     int32_t *d_bsum = nullptr;             // [2][ch][cw] block sums
+    int32_t *d_far = nullptr;              // [ch / 2][cw / 2][2] far block match (frames of 16x16 and up)
     // second stream (own hardware queue): the proposal network beside the
     // descriptor kernels, and the refinement tail of a pipelined sequence
     hipStream_t side = nullptr;
@@ -1115,18 +1116,20 @@ int proposal_ready(bm_ctx *ctx)
     if (!r) r = dev_alloc(ctx, &ctx->d_nout, cells * bm::net::kOutC);
     if (!r) r = dev_alloc(ctx, &ctx->d_coff, cells * 4);
     if (!r) r = dev_alloc(ctx, &ctx->d_bsum, cells * 2);
+    if (!r && cw >= 2 && ch >= 2) r = dev_alloc(ctx, &ctx->d_far, (size_t)(cw / 2) * (ch / 2) * 2);
     if (!r) r = side_ready(ctx);
     if (r) ctx->d_feat = nullptr;
     return r;
 }
 
-hipError_t launch_offsets(bm_ctx *ctx, const float *nout, const uint16_t *feat, int32_t *coff, hipStream_t st)
+hipError_t launch_offsets(bm_ctx *ctx, const float *nout, const uint16_t *feat, const int32_t *far, int32_t *coff,
+                          hipStream_t st)
 {
     const bm::Geometry &g = ctx->g;
     const int cw = g.W / bm::net::kCell, ch = g.H / bm::net::kCell;
     const float scx = (float)(256 * bm::net::kCell * g.sx) / (float)g.W;
     const float scy = (float)(256 * bm::net::kCell * g.sy) / (float)g.H;
-    return bm::net::launch_proposal_offsets(nout, feat, cw * ch, scx, scy, 256 * g.sx, 256 * g.sy, coff, st);
+    return bm::net::launch_proposal_offsets(nout, feat, far, cw, ch, scx, scy, 256 * g.sx, 256 * g.sy, coff, st);
 }
 
 }  // namespace
@@ -1144,6 +1147,22 @@ int bm_proposal_features(bm_ctx *ctx, const uint8_t *g1, const uint8_t *g2, uint
     return BM_OK;
 }
 
+int bm_proposal_far(bm_ctx *ctx, const uint8_t *g1, const uint8_t *g2, int32_t *far)
+{
+    int r = check(ctx, g1, "g1");
+    if (!r) r = check(ctx, g2, "g2");
+    if (!r) r = check(ctx, far, "far");
+    if (!r && (ctx->g.W < bm::net::kFarCell || ctx->g.H < bm::net::kFarCell))
+        r = fail(ctx, BM_E_PARAM, "the far block match needs frames of at least 16x16 pixels");
+    if (!r) r = proposal_ready(ctx);
+    if (r) return r;
+    const bm::Geometry &g = ctx->g;
+    const int cw = g.W / bm::net::kCell, ch = g.H / bm::net::kCell;
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, bm::net::launch_block_sums(g1, g2, g.W, cw, ch, ctx->d_bsum, ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, bm::net::launch_far_search(ctx->d_bsum, cw, ch, far, ctx->stream));
+    return BM_OK;
+}
+
 int bm_coarse_flow(bm_ctx *ctx, const float *U, const float *V, float *label)
 {
     int r = check(ctx, U, "U");
@@ -1156,7 +1175,7 @@ int bm_coarse_flow(bm_ctx *ctx, const float *U, const float *V, float *label)
     return BM_OK;
 }
 
-int bm_proposal_offsets(bm_ctx *ctx, const float *nout, const uint16_t *feat, int32_t *coff)
+int bm_proposal_offsets(bm_ctx *ctx, const float *nout, const uint16_t *feat, const int32_t *far, int32_t *coff)
 {
     int r = check(ctx, nout, "net_out");
     if (!r) r = check(ctx, feat, "feat");
@@ -1164,7 +1183,9 @@ int bm_proposal_offsets(bm_ctx *ctx, const float *nout, const uint16_t *feat, in
     if (r) return r;
     if (ctx->g.W < bm::net::kCell || ctx->g.H < bm::net::kCell)
         return fail(ctx, BM_E_PARAM, "the learned proposal needs frames of at least 8x8 pixels");
-    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, launch_offsets(ctx, nout, feat, coff, ctx->stream));
+    if (far && (ctx->g.W < bm::net::kFarCell || ctx->g.H < bm::net::kFarCell))
+        return fail(ctx, BM_E_PARAM, "the far block match needs frames of at least 16x16 pixels");
+    BM_LAUNCH(ctx, BM_STAGE_PROPOSAL, launch_offsets(ctx, nout, feat, far, coff, ctx->stream));
     return BM_OK;
 }
 
@@ -1208,11 +1229,14 @@ int bm_flow_pair_learned(bm_ctx *ctx, bm_net *net, const uint8_t *rgb1, const ui
     BM_HIP(ctx, hipStreamWaitEvent(side, ctx->fork, 0));
     BM_LAUNCH_ON(ctx, BM_STAGE_PROPOSAL, side,
                  bm::net::launch_features(ctx->d_g8[0], ctx->d_g8[1], g.W, cw, ch, ctx->d_bsum, ctx->d_feat, side));
+    if (ctx->d_far)
+        BM_LAUNCH_ON(ctx, BM_STAGE_PROPOSAL, side, bm::net::launch_far_search(ctx->d_bsum, cw, ch, ctx->d_far, side));
     // the packed weights are written on the net's stream
     BM_HIP(ctx, hipStreamWaitEvent(side, net->ready, 0));
     BM_LAUNCH_ON(ctx, BM_STAGE_PROPOSAL, side,
                  bm::net::forward(net->pk, ctx->d_feat, 1, ch, cw, ctx->d_nact[0], ctx->d_nact[1], ctx->d_nout, side));
-    BM_LAUNCH_ON(ctx, BM_STAGE_PROPOSAL, side, launch_offsets(ctx, ctx->d_nout, ctx->d_feat, ctx->d_coff, side));
+    BM_LAUNCH_ON(ctx, BM_STAGE_PROPOSAL, side,
+                 launch_offsets(ctx, ctx->d_nout, ctx->d_feat, ctx->d_far, ctx->d_coff, side));
     BM_HIP(ctx, hipEventRecord(ctx->join, side));
     r = describe(ctx, ctx->d_g8[0], ctx->d_F[0]);
     if (!r) r = describe(ctx, ctx->d_g8[1], ctx->d_F[1]);

@@ -26,6 +26,14 @@ constexpr int kSadClamp = 9 * 64 * 64;
 // feature channels of the lowest-cost displacement (dx / 8, dy / 8), added
 // times 8 to output channels 0, 1
 constexpr int kSkipChannel = kCost + 2;
+// far block match: 16x16 blocks (two coarse pixels), displacements -5..5
+// blocks (+-80 input pixels, twice the cost volume's reach); a match at ring
+// max(|dx|, |dy|) >= kFarMin whose SAD is below 3/4 of the best nearer one
+// replaces the network's proposal of the cells under the block
+constexpr int kFarCell = 2 * kCell;
+constexpr int kFarRadius = 5;
+constexpr int kFarN = (2 * kFarRadius + 1) * (2 * kFarRadius + 1);
+constexpr int kFarMin = 3;
 
 struct LayerSpec {
     int cin, cout, dil, relu;
@@ -55,6 +63,10 @@ enum { ACT_NONE = 0, ACT_RELU = 1, ACT_MASK = 2 };
 // S: scratch of 2 * cw * ch int32 (block sums)
 hipError_t launch_features(const uint8_t *g1, const uint8_t *g2, int W, int cw, int ch, int32_t *S, uint16_t *feat,
                            hipStream_t st);
+hipError_t launch_block_sums(const uint8_t *g1, const uint8_t *g2, int W, int cw, int ch, int32_t *S, hipStream_t st);
+// Far block match of the 8x8 block sums S [2][ch][cw] (as launch_features
+// leaves them): far [ch / 2][cw / 2][2] int32, (dx, dy) in 16-pixel blocks.
+hipError_t launch_far_search(const int32_t *S, int cw, int ch, int32_t *far, hipStream_t st);
 // skip (optional, the feature map): output channels 0, 1 += 8 * skip[p][kSkipChannel + c]
 hipError_t launch_conv(int cout, const uint16_t *in, int cin, const uint16_t *w, const float *bias, int npix, int H,
                        int W, int dil, int act, const uint16_t *ref, const uint16_t *skip, uint16_t *out_bf,
@@ -69,9 +81,11 @@ hipError_t launch_adam(float *w, float *m, float *v, const float *g, int n, floa
                        float c1, float c2, hipStream_t st);
 hipError_t launch_pack(const float *w, int cout, int cin, uint16_t *wb, uint16_t *wt, hipStream_t st);
 hipError_t launch_coarse_flow(const float *U, const float *V, int W, int cw, int ch, float *out, hipStream_t st);
-// coff [n][4]: network (u, v) and cost-volume argmin (u, v), Q8 matched pixels
-hipError_t launch_proposal_offsets(const float *net_out, const uint16_t *feat, int n, float scale_x, float scale_y,
-                                   int lim_x, int lim_y, int32_t *coff, hipStream_t st);
+// coff [ch][cw][4]: network (u, v) and cost-volume argmin (u, v), Q8 matched
+// pixels; far (optional, [ch / 2][cw / 2][2]) overrides the network's (u, v)
+// where it reaches ring kFarMin
+hipError_t launch_proposal_offsets(const float *net_out, const uint16_t *feat, const int32_t *far, int cw, int ch,
+                                   float scale_x, float scale_y, int lim_x, int lim_y, int32_t *coff, hipStream_t st);
 // Row indices of a mini-batch, passed by value as a kernel argument.
 constexpr int kMaxBatch = 256;
 struct RowList {

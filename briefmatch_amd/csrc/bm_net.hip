@@ -102,6 +102,64 @@ __global__ void __launch_bounds__(128) k_features(const int32_t *__restrict__ S,
     feat[(size_t)cp * kFeatC + c] = f2bf(v);
 }
 
+// Far block match, one 128-thread work-group per 16x16 block (X, Y) of a
+// cfw x cfh grid (cfw = cw / 2): the 16x16 block sums of the 3x3 window of
+// frame 1 and of the 13x13 window of frame 2 around the block (clamped to
+// the grid) are summed from four 8x8 block sums each into LDS; thread c
+// scores displacement c = (dy + 5) * 11 + dx + 5.  The minimum of the key
+// (SAD * 6 + ring) * 121 + c picks the lowest SAD, then the lowest ring
+// max(|dx|, |dy|), then the lowest c (SAD <= 9 * 65280 keeps the key below
+// 2^29, and key / 726 recovers the SAD).  The best displacement over all rings
+// is reported when its SAD is below 3/4 of the best over rings < kFarMin,
+// else the latter.
+__global__ void __launch_bounds__(128) k_far_search(const int32_t *__restrict__ S, int cw, int ch,
+                                                    int32_t *__restrict__ far)
+{
+    constexpr int R = kFarRadius, P = 2 * (R + 1) + 1;  // frame-2 window side
+    constexpr int KD = (R + 1) * kFarN;                  // key divisor of the SAD
+    __shared__ int s1[9], s2[P * P], red[2][2];
+    const int cfw = cw >> 1, cfh = ch >> 1;
+    const int b = blockIdx.x, X = b % cfw, Y = b / cfw, t = threadIdx.x;
+    const int32_t *S2 = S + cw * ch;
+    for (int k = t; k < 9 + P * P; k += 128) {
+        const bool f1 = k < 9;
+        const int q = f1 ? k : k - 9, side = f1 ? 3 : P, half = f1 ? 1 : R + 1;
+        const int by = min(max(Y + q / side - half, 0), cfh - 1), bx = min(max(X + q % side - half, 0), cfw - 1);
+        const int32_t *s = (f1 ? S : S2) + 2 * by * cw + 2 * bx;
+        const int v = s[0] + s[1] + s[cw] + s[cw + 1];
+        if (f1) s1[q] = v;
+        else s2[q] = v;
+    }
+    __syncthreads();
+    int key = 0x7fffffff, near = 0x7fffffff;
+    if (t < kFarN) {
+        const int dx = t % (2 * R + 1) - R, dy = t / (2 * R + 1) - R, ring = max(abs(dx), abs(dy));
+        int cost = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) cost += abs(s1[3 * j + i] - s2[(j + dy + R) * P + i + dx + R]);
+        key = (cost * (R + 1) + ring) * kFarN + t;
+        if (ring < kFarMin) near = key;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        key = min(key, __shfl_xor(key, o));
+        near = min(near, __shfl_xor(near, o));
+    }
+    if ((t & 63) == 0) {
+        red[t >> 6][0] = key;
+        red[t >> 6][1] = near;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const int ka = min(red[0][0], red[1][0]), kn = min(red[0][1], red[1][1]);
+        const int c = (4 * (ka / KD) < 3 * (kn / KD) ? ka : kn) % kFarN;
+        far[2 * b] = c % (2 * R + 1) - R;
+        far[2 * b + 1] = c / (2 * R + 1) - R;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // conv (forward, and data gradient with flipped/transposed weights)
 //
@@ -378,15 +436,27 @@ __global__ void k_coarse_flow(const float *__restrict__ U, const float *__restri
 // coarse flow (output channels 0, 1) and the lowest-cost displacement of the
 // feature map (channels 123, 124 times 8), each rint(clamp(u * scale)),
 // scale = 256 * 8 * sx / W; NaN maps to -lim.  coff [n][4] = (net u, net v,
-// argmin u, argmin v).
-__global__ void k_proposal_offsets(const float *__restrict__ out, const uint16_t *__restrict__ feat, int n,
-                                   float scale_x, float scale_y, int lim_x, int lim_y, int32_t *__restrict__ coff)
+// argmin u, argmin v).  With a far match, a cell whose 16-pixel block (its
+// coordinates halved, clamped to the far grid) matched at ring >= kFarMin
+// takes (2 dx, 2 dy) coarse pixels in place of the network's flow.
+__global__ void k_proposal_offsets(const float *__restrict__ out, const uint16_t *__restrict__ feat,
+                                   const int32_t *__restrict__ far, int cw, int ch, float scale_x, float scale_y,
+                                   int lim_x, int lim_y, int32_t *__restrict__ coff)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const float src[4] = {out[(size_t)i * kOutC], out[(size_t)i * kOutC + 1],
-                          8.0f * bf2f(feat[(size_t)i * kFeatC + kSkipChannel]),
-                          8.0f * bf2f(feat[(size_t)i * kFeatC + kSkipChannel + 1])};
+    if (i >= cw * ch) return;
+    float src[4] = {out[(size_t)i * kOutC], out[(size_t)i * kOutC + 1],
+                    8.0f * bf2f(feat[(size_t)i * kFeatC + kSkipChannel]),
+                    8.0f * bf2f(feat[(size_t)i * kFeatC + kSkipChannel + 1])};
+    if (far) {
+        const int cfw = cw >> 1, cfh = ch >> 1;
+        const int X = min((i % cw) >> 1, cfw - 1), Y = min((i / cw) >> 1, cfh - 1);
+        const int dx = far[2 * (Y * cfw + X)], dy = far[2 * (Y * cfw + X) + 1];
+        if (max(abs(dx), abs(dy)) >= kFarMin) {
+            src[0] = (float)(2 * dx);
+            src[1] = (float)(2 * dy);
+        }
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const float sc = (k & 1) ? scale_y : scale_x, lim = (float)((k & 1) ? lim_y : lim_x);
@@ -413,6 +483,20 @@ hipError_t launch_features(const uint8_t *g1, const uint8_t *g2, int W, int cw, 
     static_assert(kFeatC == 128 && kCost + 5 <= kFeatC, "one thread per channel, two waves");
     hipLaunchKernelGGL(k_block_sums, dim3((2 * cw * ch + 255) / 256), dim3(256), 0, st, g1, g2, W, cw, ch, S);
     hipLaunchKernelGGL(k_features, dim3((unsigned)cw * ch), dim3(kFeatC), 0, st, S, cw, ch, feat);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_sums(const uint8_t *g1, const uint8_t *g2, int W, int cw, int ch, int32_t *S, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_block_sums, dim3((2 * cw * ch + 255) / 256), dim3(256), 0, st, g1, g2, W, cw, ch, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_far_search(const int32_t *S, int cw, int ch, int32_t *far, hipStream_t st)
+{
+    static_assert(kFarN <= 128, "one thread per displacement");
+    if (cw < 2 || ch < 2) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_far_search, dim3((unsigned)(cw >> 1) * (ch >> 1)), dim3(128), 0, st, S, cw, ch, far);
     return hipGetLastError();
 }
 
@@ -504,11 +588,12 @@ hipError_t launch_coarse_flow(const float *U, const float *V, int W, int cw, int
     return hipGetLastError();
 }
 
-hipError_t launch_proposal_offsets(const float *net_out, const uint16_t *feat, int n, float scale_x, float scale_y,
-                                   int lim_x, int lim_y, int32_t *coff, hipStream_t st)
+hipError_t launch_proposal_offsets(const float *net_out, const uint16_t *feat, const int32_t *far, int cw, int ch,
+                                   float scale_x, float scale_y, int lim_x, int lim_y, int32_t *coff, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_proposal_offsets, dim3((n + 255) / 256), dim3(256), 0, st, net_out, feat, n, scale_x, scale_y,
-                       lim_x, lim_y, coff);
+    if (far && (cw < 2 || ch < 2)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_proposal_offsets, dim3((cw * ch + 255) / 256), dim3(256), 0, st, net_out, feat, far, cw, ch,
+                       scale_x, scale_y, lim_x, lim_y, coff);
     return hipGetLastError();
 }
 

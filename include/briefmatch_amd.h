@@ -300,13 +300,26 @@ int bm_ctx_frames(bm_ctx *ctx, uint8_t **g8_dev, uint32_t **F_dev);
  * means of both frames / 255 - 1/2; 123, 124 the lowest-cost displacement
  * (lowest channel on ties) / 8; 125 its scaled cost; 126, 127 zero. */
 int bm_proposal_features(bm_ctx *ctx, const uint8_t *g1_dev, const uint8_t *g2_dev, uint16_t *feat_dev);
+/* Far block match of a pair from its gray frames, [H / 16][W / 16][2] int32
+ * (frames of at least 16x16): per 16x16 block the displacement (dx, dy) in
+ * -5..5 blocks (+-80 pixels, twice the cost volume's reach) with the lowest
+ * sum of absolute differences of 16x16 block sums over a 3x3 block window
+ * (clamp-to-edge); ties go to the lower ring max(|dx|, |dy|), then to the
+ * lower (dy + 5) * 11 + dx + 5.  A displacement of ring >= 3 is reported
+ * only when its sum is below 3/4 of the best sum of rings 0..2; otherwise the
+ * best of rings 0..2 is. */
+int bm_proposal_far(bm_ctx *ctx, const uint8_t *g1_dev, const uint8_t *g2_dev, int32_t *far_dev);
 /* 8x8 block means of a flow field divided by 8 (training labels), [ch][cw][2]. */
 int bm_coarse_flow(bm_ctx *ctx, const float *U_dev, const float *V_dev, float *label_dev);
 /* Per-cell proposals as matched-pixel offsets in Q8, [ch][cw][4] int32: the
  * network's coarse flow (u, v) and the feature map's lowest-cost displacement
  * (8 x channels 123, 124), each rint(clamp(u * (float)(2048 sx) / (float)W,
- * +-256 sx)) (v likewise with sy, H). */
-int bm_proposal_offsets(bm_ctx *ctx, const float *net_out_dev, const uint16_t *feat_dev, int32_t *coff_dev);
+ * +-256 sx)) (v likewise with sy, H).  far_dev (bm_proposal_far, or NULL):
+ * a cell whose block (cx / 2, cy / 2, clamped to the far grid) reports ring
+ * max(|dx|, |dy|) >= 3 (48 pixels and more, beyond the cost volume) takes
+ * (2 dx, 2 dy) coarse pixels in place of the network's flow. */
+int bm_proposal_offsets(bm_ctx *ctx, const float *net_out_dev, const uint16_t *feat_dev, const int32_t *far_dev,
+                        int32_t *coff_dev);
 /* Proposal-seeded initialisation: the plain bm_init candidate first, then for
  * the network's and then the argmin proposal the bilinear interpolation of
  * the four surrounding cells and the four cells' own offsets; out-of-frame
@@ -316,7 +329,8 @@ int bm_init_proposal(bm_ctx *ctx, const uint32_t *F1_dev, const uint32_t *F2_dev
                      const uint32_t *prev_off_dev, const int32_t *coff_dev, uint32_t *off_dev, uint16_t *cost_dev);
 /* bm_flow_pair with the proposal-seeded initialisation followed by one random
  * search (random-search iteration index `iterations`), then the context's
- * `iterations` propagation iterations. */
+ * `iterations` propagation iterations.  The offsets take the far block match
+ * when the frames are at least 16x16 pixels. */
 int bm_flow_pair_learned(bm_ctx *ctx, bm_net *net, const uint8_t *rgb1_dev, const uint8_t *rgb2_dev, uint32_t pair,
                          float *U_dev, float *V_dev);
 

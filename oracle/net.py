@@ -4,7 +4,9 @@ numpy restatement of the learned-proposal path: the feature map (a block
 matching cost volume at 1/8 resolution), the coarse
 labels, the network (forward and backward, float64, optionally with the BF16
 rounding points of the device kernels), the conversion of its output to
-matched-pixel offsets and the proposal-seeded initialisation.  Tests and the
+matched-pixel offsets (with the far block match that overrides the network
+where it finds motion beyond the cost volume's reach) and the
+proposal-seeded initialisation.  Tests and the
 smoke check use it as the checker; the product never imports it.
 
 The reference has no learned component.  The seeded initialisation extends the
@@ -25,6 +27,9 @@ FEAT_C = 128
 OUT_C = 16
 CELL = 8
 SKIP = COST + 2          # feature channels of the lowest-cost displacement / 8
+FAR_CELL = 16            # block size of the far search
+FAR_RADIUS = 5           # far search displacements -5..5 blocks (+-80 input pixels)
+FAR_MIN = 3              # a far match replaces the network proposal from this ring on
 # (cin, cout, dilation, relu) per layer
 LAYERS = ((128, 64, 1, True), (64, 64, 2, True), (64, 64, 4, True), (64, 64, 1, True), (64, 16, 1, False))
 N_PARAMS = sum(9 * co * ci + co for ci, co, _, _ in LAYERS)
@@ -119,14 +124,70 @@ def coarse_flow(U, V):
 # ---------------------------------------------------------------------------
 # proposal offsets and the seeded initialisation
 
-def proposal_offsets(net_out, feat, W, H, sx, sy):
+def far_search(g1, g2):
+    """Far block match [ch16, cw16, 2] int32 (dx, dy in 16-pixel blocks) of one
+    pair of gray frames, ch16 = H // 16, cw16 = W // 16.
+
+    For dx, dy in -5..5: the sum of absolute differences between the 16x16
+    block sums of frame 1 over the 3x3 block window and those of frame 2
+    displaced by (dx, dy) blocks (coordinates clamped to the grid).  The best
+    displacement minimises the sum, ties going to the lower ring
+    max(|dx|, |dy|) and then to the lower (dy + 5) * 11 + dx + 5 (so that
+    equal costs never favour a far displacement); the block reports it when
+    its sum is below 3/4 of the best sum over rings 0..2, else the best
+    displacement of rings 0..2.  No counterpart in the reference."""
+    H, W = g1.shape
+    cw, ch = W // FAR_CELL, H // FAR_CELL
+    res = np.zeros((ch, cw, 2), np.int32)
+    if cw < 1 or ch < 1:
+        return res
+    S1, S2 = [g[:ch * FAR_CELL, :cw * FAR_CELL].astype(np.int64).reshape(ch, FAR_CELL, cw, FAR_CELL).sum(axis=(1, 3))
+              for g in (g1, g2)]
+    yy, xx = np.mgrid[0:ch, 0:cw]
+    n = 2 * FAR_RADIUS + 1
+    big = np.iinfo(np.int64).max
+    best = [np.full((ch, cw), big), np.full((ch, cw), big)]      # all rings, rings < FAR_MIN
+    cost_of = [np.zeros((ch, cw), np.int64), np.zeros((ch, cw), np.int64)]
+    disp = [np.zeros((ch, cw, 2), np.int32), np.zeros((ch, cw, 2), np.int32)]
+    for c in range(n * n):
+        dx, dy = c % n - FAR_RADIUS, c // n - FAR_RADIUS
+        ring = max(abs(dx), abs(dy))
+        cost = np.zeros((ch, cw), np.int64)
+        for j in (-1, 0, 1):
+            for i in (-1, 0, 1):
+                a = S1[np.clip(yy + j, 0, ch - 1), np.clip(xx + i, 0, cw - 1)]
+                b = S2[np.clip(yy + j + dy, 0, ch - 1), np.clip(xx + i + dx, 0, cw - 1)]
+                cost += np.abs(a - b)
+        key = (cost * (FAR_RADIUS + 1) + ring) * (n * n) + c
+        for k in (0, 1) if ring < FAR_MIN else (0,):
+            take = key < best[k]
+            best[k] = np.where(take, key, best[k])
+            cost_of[k] = np.where(take, cost, cost_of[k])
+            disp[k][take] = (dx, dy)
+    far = 4 * cost_of[0] < 3 * cost_of[1]
+    return np.where(far[..., None], disp[0], disp[1]).astype(np.int32)
+
+
+def proposal_offsets(net_out, feat, W, H, sx, sy, far=None):
     """Per cell [.., 4] int32 Q8 matched-pixel offsets: the network output
     (channels 0, 1, coarse pixels) and the feature map's lowest-cost
     displacement (8 x channels 123, 124), each rint(clamp(u * 2048 sx / W,
-    +-256 sx)) (v with sy, H)."""
+    +-256 sx)) (v with sy, H).  With a far match (far_search), a cell whose
+    16-pixel block (cx // 2, cy // 2, clamped to that grid) matched at ring
+    max(|dx|, |dy|) >= 3 takes (2 dx, 2 dy) coarse pixels in place of the
+    network output."""
     out = np.asarray(net_out, np.float32)
     am = bf16_float(feat)[..., SKIP:SKIP + 2] * np.float32(8)
-    srcs = (out[..., 0], out[..., 1], am[..., 0], am[..., 1])
+    u0, v0 = out[..., 0], out[..., 1]
+    if far is not None and far.size:
+        ch, cw = out.shape[-3:-1]
+        fy = np.minimum(np.arange(ch) // 2, far.shape[0] - 1)
+        fx = np.minimum(np.arange(cw) // 2, far.shape[1] - 1)
+        d = far[fy][:, fx]
+        sel = np.abs(d).max(axis=-1) >= FAR_MIN
+        u0 = np.where(sel, (2 * d[..., 0]).astype(np.float32), u0)
+        v0 = np.where(sel, (2 * d[..., 1]).astype(np.float32), v0)
+    srcs = (u0, v0, am[..., 0], am[..., 1])
     res = []
     for k, u in enumerate(srcs):
         s, d = (sy, H) if k & 1 else (sx, W)

@@ -1,7 +1,7 @@
 """Learned proposal on the GPU, through the C ABI.
 
-Integer parts (feature map, labels, offsets, seeded initialisation, the match
-field of the learned path) agree with the oracle bit for bit.  The network's
+Integer parts (feature map, far block match, labels, offsets, seeded
+initialisation, the match field of the learned path) agree with the oracle bit for bit.  The network's
 forward output and gradients are compared with the oracle's float64 network
 evaluated with the same BF16 rounding points: the device accumulates in FP32
 in another order, so they agree to a relative 2e-2 per layer (gradients) and
@@ -81,8 +81,18 @@ def test_offsets_and_seeded_init_bit_exact(W, H, up, N):
     ctx.proposal_offsets(dout, feat, coff)
     ctx.sync()
     got_coff = coff.download((ch, cw, 4), np.int32)
-    want_coff = onet.proposal_offsets(out, onet.features(gs[0], gs[1]), W, H, ctx.sx, ctx.sy)
+    want_feat = onet.features(gs[0], gs[1])
+    want_coff = onet.proposal_offsets(out, want_feat, W, H, ctx.sx, ctx.sy)
     assert np.array_equal(got_coff, want_coff)
+    # with a far match: random displacements of every ring, so that both sides
+    # of the override are taken
+    far = rng.integers(-5, 6, (H // 16, W // 16, 2)).astype(np.int32)
+    coff_far = bm.DeviceBuffer(ch * cw * 4 * 4)
+    ctx.proposal_offsets(dout, feat, coff_far, far=bm.DeviceBuffer(array=far))
+    ctx.sync()
+    want_far = onet.proposal_offsets(out, want_feat, W, H, ctx.sx, ctx.sy, far=far)
+    assert np.array_equal(coff_far.download((ch, cw, 4), np.int32), want_far)
+    assert not np.array_equal(want_far, want_coff)
     M = ctx.sx * ctx.sy
     dx = rng.integers(-9, 10, (ctx.sy, ctx.sx))
     dy = rng.integers(-9, 10, (ctx.sy, ctx.sx))
@@ -96,6 +106,44 @@ def test_offsets_and_seeded_init_bit_exact(W, H, up, N):
         eo, ec = onet.init_proposal(Fs[0], Fs[1], want_coff, W, H, first=first, prev_off=prev)
         assert np.array_equal(off.download((ctx.sy, ctx.sx), np.uint32), eo)
         assert np.array_equal(cost.download((ctx.sy, ctx.sx), np.uint16), ec)
+
+
+def _gray_pair(rng, H, W, shift):
+    g1 = rng.integers(0, 256, (H, W)).astype(np.uint8)
+    g2 = np.roll(g1, shift, axis=(0, 1))
+    g2[rng.random((H, W)) < 0.05] = 0          # some noise, so that costs are not all zero
+    return g1, g2
+
+
+@pytest.mark.parametrize("W,H,shift", [(320, 240, (0, 64)), (272, 256, (-48, 16)), (97, 43, (3, -17)),
+                                       (1920, 1080, (37, -70)), (16, 16, (0, 0)), (40, 33, (5, 9))])
+def test_far_search_bit_exact(W, H, shift):
+    """bm_proposal_far = the oracle's far block match, bit for bit, on moved
+    random frames (odd grids, the smallest 16x16 frame, a 1080p frame)."""
+    say("far search", W, H, shift)
+    p = bm.derive_params(W, H, 1.5, 1.5, N=32)
+    ctx = bm.Context(p, seed=1)
+    rng = np.random.default_rng(W * H)
+    g1, g2 = _gray_pair(rng, H, W, shift)
+    far = bm.DeviceBuffer(array=np.full((H // 16, W // 16, 2), -99, np.int32))
+    ctx.proposal_far(bm.DeviceBuffer(array=g1), bm.DeviceBuffer(array=g2), far)
+    ctx.sync()
+    want = onet.far_search(g1, g2)
+    assert np.array_equal(far.download((H // 16, W // 16, 2), np.int32), want)
+
+
+def test_far_search_errors():
+    """Frames below 16x16 cannot hold a block: a status, no launch."""
+    p = bm.derive_params(40, 15, 1.5, 1.5, N=32)
+    ctx = bm.Context(p, seed=1)
+    g = bm.DeviceBuffer(array=np.zeros((15, 40), np.uint8))
+    far = bm.DeviceBuffer(64)
+    with pytest.raises(bm.BriefMatchError):
+        ctx.proposal_far(g, g, far)
+    cw, ch = ctx.proposal_dims()
+    feat, nout, coff = bm.DeviceBuffer(ch * cw * 256), bm.DeviceBuffer(ch * cw * 64), bm.DeviceBuffer(ch * cw * 16)
+    with pytest.raises(bm.BriefMatchError):
+        ctx.proposal_offsets(nout, feat, coff, far=far)
 
 
 def _random_weights(seed):
@@ -178,18 +226,24 @@ def test_training_lowers_loss_and_is_reproducible():
     assert np.array_equal(nets[0].weights(), nets[1].weights())
 
 
-def test_learned_path_equals_composed_oracle():
-    """bm_flow_pair_learned = features -> network -> offsets -> seeded init ->
-    random search -> propagation + random search -> refinement, each stage on
-    the oracle except the network (its device output is the oracle's input)."""
-    say("learned path c1")
+@pytest.mark.parametrize("motion", ["c1", "far"])
+def test_learned_path_equals_composed_oracle(motion):
+    """bm_flow_pair_learned = features + far match -> network -> offsets ->
+    seeded init -> random search -> propagation + random search ->
+    refinement, each stage on the oracle except the network (its device output
+    is the oracle's input); "far": regions moving up to 60 pixels, where the
+    far match overrides the network in many cells."""
+    say("learned path", motion)
     W, H, up, N = 320, 240, 1.5, 32
     p = bm.derive_params(W, H, up, up, N=N)
     p.iterations = 1
     ctx = bm.Context(p, seed=11)
     path = learned.DEFAULT_WEIGHTS
     net = bm.Net(path=path) if os.path.exists(path) else bm.Net(weights=_random_weights(1))
-    pr = synth.workload_pair("c1_320x240_u1.5_n32", seed=2)
+    if motion == "c1":
+        pr = synth.workload_pair("c1_320x240_u1.5_n32", seed=2)
+    else:
+        pr = synth.make_pair(W, H, motion="regions", n_regions=6, max_motion=60.0, seed=1)
     r1 = bm.DeviceBuffer(array=pr.frame1)
     r2 = bm.DeviceBuffer(array=pr.frame2)
     U, V = ctx.image_buffer(), ctx.image_buffer()
@@ -204,7 +258,10 @@ def test_learned_path_equals_composed_oracle():
     nout = bm.DeviceBuffer(ch * cw * 64)
     net.forward(feat, 1, ch, cw, nout)
     out = nout.download((ch, cw, 16), np.float32)
-    coff = onet.proposal_offsets(out, onet.features(gs[0], gs[1]), W, H, ctx.sx, ctx.sy)
+    far = onet.far_search(gs[0], gs[1])
+    if motion == "far":
+        assert (np.abs(far).max(axis=-1) >= onet.FAR_MIN).sum() > 50
+    coff = onet.proposal_offsets(out, onet.features(gs[0], gs[1]), W, H, ctx.sx, ctx.sy, far=far)
     oF1, oF2 = Fs[0][..., :ctx.NS].copy(), Fs[1][..., :ctx.NS].copy()
     eo, ec = onet.init_proposal(oF1, oF2, coff, W, H)
     eo, ec = oracle.random_search(oF1, oF2, 1, 6, 11, eo, ec)     # after the seeded init: index = iterations

@@ -150,6 +150,106 @@ def test_proposal_offsets_known_values():
     assert got[0, 1, 2] == 0 and got[0, 1, 3] == 0
 
 
+def _textured(rng, H, W):
+    return rng.integers(0, 256, (H, W)).astype(np.uint8)
+
+
+@pytest.mark.parametrize("sx,sy", [(4, 0), (0, -3), (-5, 2), (3, 3)])
+def test_far_search_translation(sx, sy):
+    """Frame 2 = frame 1 moved by (16 sx, 16 sy) pixels: the interior blocks,
+    whose windows never clamp, find exactly (sx, sy) (a wrong sign or a
+    swapped axis fails here)."""
+    rng = np.random.default_rng(100 + sx + 11 * sy)
+    g1 = _textured(rng, 256, 272)
+    g2 = _textured(rng, 256, 272)
+    dy, dx = 16 * sy, 16 * sx
+    ys, xs = slice(max(dy, 0), 256 + min(dy, 0)), slice(max(dx, 0), 272 + min(dx, 0))
+    g2[ys, xs] = g1[max(-dy, 0):256 - max(dy, 0), max(-dx, 0):272 - max(dx, 0)]
+    got = onet.far_search(g1, g2)
+    assert got.shape == (16, 17, 2) and got.dtype == np.int32
+    inner = got[6:10, 6:11]
+    assert (inner[..., 0] == sx).all() and (inner[..., 1] == sy).all()
+
+
+def test_far_search_ties():
+    """Equal costs resolve to the lowest ring, then the lowest index:
+    flat frames give (0, 0) everywhere (not the first displacement (-5, -5));
+    a texture that repeats every 32 pixels in x, moved by 16 pixels, matches
+    at dx = -5, -3, -1, 1, 3, 5 with equal cost and gives (-1, 0)."""
+    flat = np.full((96, 128), 77, np.uint8)
+    assert (onet.far_search(flat, flat) == 0).all()
+    rng = np.random.default_rng(3)
+    col = rng.integers(0, 256, (224, 32)).astype(np.uint8)
+    g1 = np.tile(col, (1, 7))
+    g2 = np.roll(g1, 16, axis=1)
+    got = onet.far_search(g1, g2)
+    inner = got[6:8, 6:8]
+    assert (inner[..., 0] == -1).all() and (inner[..., 1] == 0).all()
+    # the same texture, not moved: rings 0, 2, 4 tie, ring 0 wins
+    assert (onet.far_search(g1, g1)[6:8, 6:8] == 0).all()
+
+
+def test_far_search_brute_force():
+    """Per-block loops over the displacements, with 16x16 sums taken from the
+    image directly, on a pair with regions moving up to 60 pixels: the far
+    match where it beats 3/4 of the best ring-0..2 sum, else that nearer
+    displacement; both outcomes occur."""
+    import oracle
+    from briefmatch_amd import synth
+    pr = synth.make_pair(160, 128, motion="regions", n_regions=4, max_motion=60.0, seed=4)
+    g1, g2 = oracle.gray(pr.frame1).astype(np.int64), oracle.gray(pr.frame2).astype(np.int64)
+    ch, cw = 128 // 16, 160 // 16
+    bsum = lambda g, by, bx: int(g[16 * by:16 * by + 16, 16 * bx:16 * bx + 16].sum())
+    S1 = [[bsum(g1, y, x) for x in range(cw)] for y in range(ch)]
+    S2 = [[bsum(g2, y, x) for x in range(cw)] for y in range(ch)]
+    cl = lambda v, n: min(max(v, 0), n - 1)
+    got = onet.far_search(g1.astype(np.uint8), g2.astype(np.uint8))
+    fell_back = took_far = 0
+    for y in range(ch):
+        for x in range(cw):
+            scored = []
+            for dy in range(-5, 6):
+                for dx in range(-5, 6):
+                    sad = sum(abs(S1[cl(y + j, ch)][cl(x + i, cw)] - S2[cl(y + j + dy, ch)][cl(x + i + dx, cw)])
+                              for j in (-1, 0, 1) for i in (-1, 0, 1))
+                    scored.append((sad, max(abs(dx), abs(dy)), dy, dx))
+            best = min(scored)
+            near = min(t for t in scored if t[1] <= 2)
+            pick = best if 4 * best[0] < 3 * near[0] else near
+            assert tuple(got[y, x]) == (pick[3], pick[2]), (y, x)
+            fell_back += best[1] >= 3 and pick is near
+            took_far += pick[1] >= 3
+    assert fell_back > 0 and took_far > 0
+
+
+def test_far_search_small_frames():
+    assert onet.far_search(np.zeros((15, 40), np.uint8), np.zeros((15, 40), np.uint8)).shape == (0, 2, 2)
+
+
+def test_proposal_offsets_far_override():
+    """A far match at ring >= 3 replaces the network's flow of the cells under
+    its block (cells 2X, 2X + 1; the odd last column uses the last block) by
+    (2 dx, 2 dy) coarse pixels; rings below 3 leave it."""
+    W, H, sx, sy = 104, 24, 151, 35
+    ch, cw = H // 8, W // 8                              # 3 x 13 cells, 1 x 6 blocks
+    out = np.full((ch, cw, 16), 0.5, np.float32)
+    feat = onet.bf16_bits(np.zeros((ch, cw, 128), np.float32))
+    far = np.array([[[3, -1], [2, 2], [0, 0], [1, -2], [-2, 1], [-4, 0]]], np.int32)
+    got = onet.proposal_offsets(out, feat, W, H, sx, sy, far=far)
+    base = onet.proposal_offsets(out, feat, W, H, sx, sy)
+    for cy in range(ch):
+        for cx in range(cw):
+            X = min(cx // 2, 5)
+            if X in (0, 5):
+                dx, dy = far[0, X]
+                assert got[cy, cx, 0] == round(2 * dx * 2048 * sx / W)
+                assert got[cy, cx, 1] == round(2 * dy * 2048 * sy / H)
+            else:
+                assert (got[cy, cx, :2] == base[cy, cx, :2]).all()
+            assert (got[cy, cx, 2:] == base[cy, cx, 2:]).all()
+    assert base[0, 0, 0] == round(0.5 * 2048 * sx / W)
+
+
 def _init_loops(F1, F2, coff, W, H, first, prev):
     sy, sx = F1.shape[:2]
     ch, cw = coff.shape[:2]
