@@ -60,24 +60,38 @@ def _offset(buf, nbytes):
     return C.c_void_p(buf.ptr.value + int(nbytes))
 
 
-def train_pair(seed, W=640, H=480, with_gt=False):
-    """Training pair `seed`: moving rectangles over an affine background
-    (the family of the 1920x1080 benchmark pair), motion up to 16..40 px, the
-    frames a window of a scene 1 to 3 times their size (so texture and
-    rectangle scales span those of 640x480 and 1920x1080 renderings).
+FAMILIES = ("rects", "mixed")
+
+
+def train_pair(seed, W=640, H=480, with_gt=False, family="rects", regions_share=0.5):
+    """Training pair `seed`.  family "rects": moving rectangles over an affine
+    background (the family of the 1920x1080 benchmark pair), motion up to
+    16..40 px, the frames a window of a scene 1 to 3 times their size (so
+    texture and rectangle scales span those of 640x480 and 1920x1080
+    renderings).  family "mixed": a share `regions_share` of the seeds
+    (drawn per seed) smoothly blended affine regions (the family of the
+    1280x720 benchmark pair) with 3..8 regions and motion up to 20..60 px,
+    the others as "rects".
     Returns (frame1, frame2) or, with_gt, (frame1, frame2, gt flow (H, W, 2))."""
+    if family not in FAMILIES:
+        raise ValueError("family must be one of %s" % (FAMILIES,))
     rng = np.random.default_rng([seed, 7])
-    n_rects = int(rng.integers(3, 11))
-    max_motion = float(rng.uniform(16.0, 40.0))
-    k = float(rng.uniform(1.0, 3.0))
-    p = synth.make_pair(W, H, motion="rects", n_rects=n_rects, max_motion=max_motion, seed=seed,
-                        scene=(int(W * k), int(H * k)))
+    if family == "mixed" and np.random.default_rng([seed, 11]).random() < regions_share:
+        n_regions = int(rng.integers(3, 9))
+        max_motion = float(rng.uniform(20.0, 60.0))
+        p = synth.make_pair(W, H, motion="regions", n_regions=n_regions, max_motion=max_motion, seed=seed)
+    else:
+        n_rects = int(rng.integers(3, 11))
+        max_motion = float(rng.uniform(16.0, 40.0))
+        k = float(rng.uniform(1.0, 3.0))
+        p = synth.make_pair(W, H, motion="rects", n_rects=n_rects, max_motion=max_motion, seed=seed,
+                            scene=(int(W * k), int(H * k)))
     return (p.frame1, p.frame2, p.gt) if with_gt else (p.frame1, p.frame2)
 
 
 def _gen(args):
-    seed, W, H, with_gt = args
-    return train_pair(seed, W, H, with_gt)
+    seed, W, H, with_gt, family, share = args
+    return train_pair(seed, W, H, with_gt, family, share)
 
 
 class Dataset:
@@ -96,7 +110,7 @@ class Dataset:
 
 
 def build_dataset(pairs, seed0=1000, W=640, H=480, up=1.0, N=128, iterations=4, workers=16, labels="plain",
-                  log=print):
+                  family="rects", regions_share=0.5, log=print):
     """Generate `pairs` seeded pairs (seeds seed0..), label them and store
     features and labels on the device.  labels="plain": the plain method's
     flow (`iterations` propagation iterations); labels="gt": the pairs' known
@@ -113,7 +127,7 @@ def build_dataset(pairs, seed0=1000, W=640, H=480, up=1.0, N=128, iterations=4, 
     U, V = ctx.image_buffer(), ctx.image_buffer()
     g8, F = ctx.frames()
     t0 = time.time()
-    jobs = [(seed0 + i, W, H, with_gt) for i in range(pairs)]
+    jobs = [(seed0 + i, W, H, with_gt, family, regions_share) for i in range(pairs)]
     pool = mp.get_context("spawn").Pool(workers) if workers > 1 else None
     it = pool.imap(_gen, jobs, chunksize=4) if pool else map(_gen, jobs)
     try:
@@ -234,6 +248,9 @@ def main(argv=None):
     t.add_argument("--seed", type=int, default=0)
     t.add_argument("--labels", choices=["plain", "gt"], default="plain",
                    help="label flow: the plain method's output or the pairs' known flow")
+    t.add_argument("--family", choices=FAMILIES, default="rects",
+                   help="training pairs: moving rectangles, or rectangles and large-motion regions")
+    t.add_argument("--regions-share", type=float, default=0.5, help="share of region pairs in the mixed family")
     t.add_argument("--out", required=True)
     t.add_argument("--eval", action="store_true", help="evaluate on the 1920x1080 workload afterwards")
     e = sub.add_parser("eval")
@@ -243,7 +260,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     if a.cmd == "train":
-        ds = build_dataset(a.pairs, workers=a.workers, labels=a.labels, log=log)
+        ds = build_dataset(a.pairs, workers=a.workers, labels=a.labels, family=a.family,
+                           regions_share=a.regions_share, log=log)
         net = Net(weights=init_weights(a.seed))
         rows = np.arange(a.pairs - a.val)
         val = np.arange(a.pairs - a.val, a.pairs)

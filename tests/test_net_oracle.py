@@ -321,3 +321,21 @@ def test_oracle_plain_candidate_is_orc_init():
     off, cost = onet.init_proposal(F1, F2, coff, 24, 16)
     o2, c2 = oracle.init(F1, F2, first=True)
     assert np.array_equal(off, o2) and np.array_equal(cost, c2)
+
+
+def test_training_families():
+    """The mixed training family: a drawn share of the seeds are region pairs
+    with motion of 20..60 px, the others the rectangle pairs of the default
+    family; the known flow comes with each pair."""
+    from briefmatch_amd import learned
+    regions = [s for s in range(1000, 1040)
+               if not np.array_equal(learned.train_pair(s, 96, 64, True, "mixed", 0.25)[2],
+                                     learned.train_pair(s, 96, 64, True, "rects")[2])]
+    assert 3 <= len(regions) <= 20
+    f1, f2, gt = learned.train_pair(regions[0], 160, 120, with_gt=True, family="mixed", regions_share=0.25)
+    assert f1.shape == (120, 160, 3) and gt.shape == (120, 160, 2)
+    assert 20.0 - 1e-3 <= np.sqrt((gt.astype(np.float64) ** 2).sum(-1)).max() <= 60.0 + 1e-3
+    assert all(np.array_equal(learned.train_pair(s, 96, 64, True, "mixed", 0.0)[2],
+                              learned.train_pair(s, 96, 64, True, "rects")[2]) for s in range(1000, 1004))
+    with pytest.raises(ValueError):
+        learned.train_pair(1, family="other")
