@@ -32,7 +32,10 @@ from . import Context, DeviceBuffer, Net, derive_params, epe, NET_PARAMS
 from . import synth
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-# trained on 640x480 pairs labelled by the plain method (128-bit descriptors)
+# trained on 16,000 640x480 pairs, half moving rectangles and half large-motion
+# regions, labelled with their known flow:
+#   python -m briefmatch_amd.learned train --pairs 16000 --epochs 30 --lr 5e-4 \
+#       --labels gt --family mixed --regions-share 0.5 --out proposal_640x480.bnet
 DEFAULT_WEIGHTS = os.path.join(_HERE, "weights", "proposal_640x480.bnet")
 
 # (cin, cout) per layer, as in the C ABI's parameter layout

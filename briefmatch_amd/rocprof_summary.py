@@ -31,7 +31,8 @@ STAGE_OF = OrderedDict([("k_gray", "gray"), ("k_upsample", "upsample"), ("k_brie
                         ("k_init_proposal", "proposal"), ("k_init", "init"), ("k_level<", "level"), ("k_random_search", "random_search"),
                         ("k_downsample", "downsample"), ("k_median", "median"), ("k_trilateral", "trilateral"),
                         ("k_copy", "copy"), ("k_features", "proposal"),
-                        ("k_block_sums", "proposal"), ("k_conv", "proposal"), ("k_proposal_offsets", "proposal")])
+                        ("k_block_sums", "proposal"), ("k_far_search", "proposal"), ("k_conv", "proposal"),
+                        ("k_proposal_offsets", "proposal")])
 
 
 def stage_of(short):
