@@ -620,7 +620,7 @@ int bm_ctx_create(const bm_params *params, uint64_t seed, bm_ctx **out)
     g.tri_sigma_s = sS;
     const double log2e = 1.4426950408889634;
     g.tri_s = p.filter ? (int)std::ceil(2.5 * (double)std::sqrt(sS)) : 0;
-    g.tri_a = p.filter ? (float)std::sqrt(log2e / (2.0 * (double)sE)) : 1.0f;
+    g.tri_a2 = p.filter ? (float)(log2e / (2.0 * (double)sE)) : 1.0f;
     g.tri_b = p.filter ? (float)(std::sqrt(log2e / (2.0 * (double)sI)) / 255.0) : 1.0f;
     if (p.filter && (g.tri_s < 1 || g.tri_s > 2000)) {
         delete ctx;

@@ -31,7 +31,7 @@ struct Geometry {
     const int *brief_plan_count;  // per descriptor word: near and far record counts
     uint64_t seed;                // pattern + random-search key
     int64_t rs_sigma_q16;         // random-search sigma in Q16
-    float tri_a, tri_b;           // trilateral range scales (see bm_kernels.hip)
+    float tri_a2, tri_b;          // trilateral range scales: of the squared flow difference, of the gray level
     float tri_sigma_epe, tri_sigma_i, tri_sigma_s;  // squared sigmas
     int tri_s;                    // trilateral half window
     // device tables

@@ -853,10 +853,15 @@ __global__ void __launch_bounds__(kThreads) k_median(const float *__restrict__ i
 // ---------------------------------------------------------------------------
 // Cross-trilateral filter.  Work-group: 64 x 16 output tile, thread (lx, ly)
 // owns the four consecutive rows 4 ly + k.  Source rows stream through LDS in
-// bands of BR rows as float4 (a u, a v, b I, 0), where a^2 = log2(e) / (2
-// sigmaEPE) and b^2 = log2(e) / (2 sigmaI 255^2), so one tap is
-//   t = (du')^2 + (dv')^2 + (dI')^2 + log2(e) i^2 / (2 sigmaS),  w = exp2(-t),
-// the product of the reference's three exponentials.  The j (row) part of the
+// bands of BR rows as float4 (u, v, b I, 0), where b^2 = log2(e) / (2 sigmaI
+// 255^2), and one tap is
+//   t = a2 ((du)^2 + (dv)^2) + (dI')^2 + log2(e) i^2 / (2 sigmaS),  w = exp2(-t),
+// a2 = log2(e) / (2 sigmaEPE): the product of the reference's three
+// exponentials.  The flow differences are taken before any scaling, so their
+// rounding is relative to the difference, not to the flow: scaled values
+// (a u - a um) carry an error of about |a u| 2^-24 into every exponent, which
+// the normalisation cancels except where the weight sum falls below its 1e-5
+// floor, and there reaches 1e-3 px at 36 px of flow.  The j (row) part of the
 // spatial term is applied once per source row.
 
 constexpr int kTriTW = 64, kTriTH = 16, kTriPY = 4;
@@ -866,7 +871,7 @@ __global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict
                                                          const float *__restrict__ Um, const float *__restrict__ Vm,
                                                          const float *__restrict__ I, float *__restrict__ Uo,
                                                          float *__restrict__ Vo, int W, int H, int s, int BR,
-                                                         float a, float bI, const float *__restrict__ spt)
+                                                         float a2, float bI, const float *__restrict__ spt)
 {
     extern __shared__ float4 band[];
     const int pitch = kTriTW + 2 * s;
@@ -884,8 +889,8 @@ __global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict
 #pragma unroll
     for (int k = 0; k < kTriPY; ++k) {
         const size_t o = (size_t)min(y0 + k, H - 1) * W + xc;
-        um[k] = Um[o] * a;
-        vm[k] = Vm[o] * a;
+        um[k] = Um[o];
+        vm[k] = Vm[o];
         g0[k] = I[o] * bI;
         au[k] = av[k] = aw[k] = 0.0f;
     }
@@ -899,7 +904,7 @@ __global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict
             const float *ru = Ui + (size_t)gy * W, *rv = Vi + (size_t)gy * W, *ri = I + (size_t)gy * W;
             for (int c = lx; c < pitch; c += 64) {
                 const int gx = clampi(tx0 - s + c, 0, W - 1);
-                band[r * pitch + c] = make_float4(ru[gx] * a, rv[gx] * a, ri[gx] * bI, 0.0f);
+                band[r * pitch + c] = make_float4(ru[gx], rv[gx], ri[gx] * bI, 0.0f);
             }
         }
         __syncthreads();
@@ -920,8 +925,8 @@ __global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict
                 for (int k = 0; k < kTriPY; ++k) {
                     float du = qa.x - um[k], dv = qa.y - vm[k], dg = qa.z - g0[k];
                     float t = fmaf(dg, dg, sp.x);
-                    t = fmaf(dv, dv, t);
-                    t = fmaf(du, du, t);
+                    float e = fmaf(dv, dv, du * du);
+                    t = fmaf(e, a2, t);
                     float w = __builtin_amdgcn_exp2f(-t);
                     rw[k] += w;
                     ru[k] = fmaf(w, qa.x, ru[k]);
@@ -930,8 +935,8 @@ __global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict
                     dv = qb.y - vm[k];
                     dg = qb.z - g0[k];
                     t = fmaf(dg, dg, sp.y);
-                    t = fmaf(dv, dv, t);
-                    t = fmaf(du, du, t);
+                    e = fmaf(dv, dv, du * du);
+                    t = fmaf(e, a2, t);
                     w = __builtin_amdgcn_exp2f(-t);
                     rw[k] += w;
                     ru[k] = fmaf(w, qb.x, ru[k]);
@@ -950,14 +955,13 @@ __global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict
         }
     }
     if (x < W) {
-        const float ia = 1.0f / a;
 #pragma unroll
         for (int k = 0; k < kTriPY; ++k) {
             const int y = y0 + k;
             if (y < H) {
                 const float den = fmaxf(1e-5f, aw[k]);
-                Uo[(size_t)y * W + x] = (au[k] * ia) / den;
-                Vo[(size_t)y * W + x] = (av[k] * ia) / den;
+                Uo[(size_t)y * W + x] = au[k] / den;
+                Vo[(size_t)y * W + x] = av[k] / den;
             }
         }
     }
@@ -971,13 +975,13 @@ __global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict
 // times the column factor exp2(-spatial_i) (the B operand); D[i][j], held by
 // lane 4b+j in register i, accumulates row i's sum over q of w * B_j.  The row
 // part of the spatial term is the first addend of the exponent (+inf outside
-// the window), so a tap costs 3 sub, 3 FMA and one exp2 per pixel on the VALU;
+// the window), so a tap costs 3 sub, 1 mul, 3 FMA and one exp2 per pixel on the VALU;
 // the per-source B scaling is shared by the R groups of four rows a wave owns.
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <int R, int MASK>
 __device__ __forceinline__ void tri_taps(const float4 *__restrict__ src, const float *__restrict__ srcf,
-                                         const float *__restrict__ es, int n, const float (&um)[R],
+                                         const float *__restrict__ es, int n, float a2, const float (&um)[R],
                                          const float (&vm)[R], const float (&g0)[R], const float (&ck)[R],
                                          v4f (&acc)[R])
 {
@@ -990,8 +994,7 @@ __device__ __forceinline__ void tri_taps(const float4 *__restrict__ src, const f
             if (!((MASK >> g) & 1)) continue;
             const float du = q.x - um[g], dv = q.y - vm[g], dg = q.z - g0[g];
             float t = fmaf(dg, dg, ck[g]);
-            t = fmaf(dv, dv, t);
-            t = fmaf(du, du, t);
+            t = fmaf(fmaf(dv, dv, du * du), a2, t);
             const float w = __builtin_amdgcn_exp2f(-t);
             acc[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(w, bq, acc[g], 0, 0, 0);
         }
@@ -1002,14 +1005,14 @@ __device__ __forceinline__ void tri_taps(const float4 *__restrict__ src, const f
 template <int R, int M>
 __device__ __forceinline__ void tri_taps_mask(int mask, const float4 *__restrict__ src,
                                               const float *__restrict__ srcf, const float *__restrict__ es, int n,
-                                              const float (&um)[R], const float (&vm)[R], const float (&g0)[R],
-                                              const float (&ck)[R], v4f (&acc)[R])
+                                              float a2, const float (&um)[R], const float (&vm)[R],
+                                              const float (&g0)[R], const float (&ck)[R], v4f (&acc)[R])
 {
     if constexpr (M > 0) {
         if (mask == M)
-            tri_taps<R, M>(src, srcf, es, n, um, vm, g0, ck, acc);
+            tri_taps<R, M>(src, srcf, es, n, a2, um, vm, g0, ck, acc);
         else
-            tri_taps_mask<R, M - 1>(mask, src, srcf, es, n, um, vm, g0, ck, acc);
+            tri_taps_mask<R, M - 1>(mask, src, srcf, es, n, a2, um, vm, g0, ck, acc);
     }
 }
 
@@ -1020,7 +1023,7 @@ __global__ void __launch_bounds__(kThreads) k_trilateral_mfma(const float *__res
                                                               const float *__restrict__ Vm,
                                                               const float *__restrict__ I, float *__restrict__ Uo,
                                                               float *__restrict__ Vo, int W, int H, int s, int BR,
-                                                              float a, float bI, const float *__restrict__ spt)
+                                                              float a2, float bI, const float *__restrict__ spt)
 {
     constexpr int TH = 4 * R;
     extern __shared__ float4 band[];
@@ -1042,8 +1045,8 @@ __global__ void __launch_bounds__(kThreads) k_trilateral_mfma(const float *__res
 #pragma unroll
     for (int g = 0; g < R; ++g) {
         const size_t o = (size_t)min(ty0 + 4 * g + k, H - 1) * W + xc;
-        um[g] = Um[o] * a;
-        vm[g] = Vm[o] * a;
+        um[g] = Um[o];
+        vm[g] = Vm[o];
         g0[g] = I[o] * bI;
         acc[g] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     }
@@ -1056,7 +1059,7 @@ __global__ void __launch_bounds__(kThreads) k_trilateral_mfma(const float *__res
             const float *ru = Ui + (size_t)gy * W, *rv = Vi + (size_t)gy * W, *ri = I + (size_t)gy * W;
             for (int c = lane; c < pitch; c += 64) {
                 const int gx = clampi(tx0 - s + c, 0, W - 1);
-                band[r * pitch + c] = make_float4(ru[gx] * a, rv[gx] * a, ri[gx] * bI, 1.0f);
+                band[r * pitch + c] = make_float4(ru[gx], rv[gx], ri[gx] * bI, 1.0f);
             }
         }
         __syncthreads();
@@ -1074,11 +1077,10 @@ __global__ void __launch_bounds__(kThreads) k_trilateral_mfma(const float *__res
             }
             const float4 *src = band + r * pitch + col;
             const float *srcf = reinterpret_cast<const float *>(src) + comp;
-            tri_taps_mask<R, (1 << R) - 1>(mask, src, srcf, es, 2 * s, um, vm, g0, ck, acc);
+            tri_taps_mask<R, (1 << R) - 1>(mask, src, srcf, es, 2 * s, a2, um, vm, g0, ck, acc);
         }
     }
     // lane 4b+j holds, in register i, row i's sum of w * (1, u, v, I)_j * es
-    const float ia = 1.0f / a;
 #pragma unroll
     for (int g = 0; g < R; ++g) {
         float den[4];
@@ -1089,7 +1091,7 @@ __global__ void __launch_bounds__(kThreads) k_trilateral_mfma(const float *__res
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int y = ty0 + 4 * g + i;
-                if (y < H) out[(size_t)y * W + x] = (acc[g][i] * ia) / den[i];
+                if (y < H) out[(size_t)y * W + x] = acc[g][i] / den[i];
             }
         }
     }
@@ -1380,15 +1382,15 @@ hipError_t launch_trilateral(const Geometry &g, const float *Ui, const float *Vi
     switch (g.tri_groups) {
     case 0:
         hipLaunchKernelGGL(k_trilateral, dim3(gxb * ((g.H + kTriTH - 1) / kTriTH)), dim3(kThreads), lds, st, Ui, Vi,
-                           Um, Vm, I, Uo, Vo, g.W, g.H, s, BR, g.tri_a, g.tri_b, g.tri_spatial);
+                           Um, Vm, I, Uo, Vo, g.W, g.H, s, BR, g.tri_a2, g.tri_b, g.tri_spatial);
         break;
     case 2:
         hipLaunchKernelGGL(k_trilateral_mfma<2>, dim3(gxb * ((g.H + 7) / 8)), dim3(kThreads), lds, st, Ui, Vi, Um, Vm,
-                           I, Uo, Vo, g.W, g.H, s, BR, g.tri_a, g.tri_b, g.tri_spatial);
+                           I, Uo, Vo, g.W, g.H, s, BR, g.tri_a2, g.tri_b, g.tri_spatial);
         break;
     case 4:
         hipLaunchKernelGGL(k_trilateral_mfma<4>, dim3(gxb * ((g.H + 15) / 16)), dim3(kThreads), lds, st, Ui, Vi, Um,
-                           Vm, I, Uo, Vo, g.W, g.H, s, BR, g.tri_a, g.tri_b, g.tri_spatial);
+                           Vm, I, Uo, Vo, g.W, g.H, s, BR, g.tri_a2, g.tri_b, g.tri_spatial);
         break;
     default: return hipErrorInvalidValue;
     }

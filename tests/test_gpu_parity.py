@@ -260,6 +260,38 @@ def test_trilateral_kernels(W, H, sS):
         assert np.abs(got[groups][0] - got[0][0]).max() <= FLOW_TOL
 
 
+@pytest.mark.parametrize("W,H,sS", [(160, 120, None), (97, 61, 6.0)])
+def test_trilateral_large_flow(W, H, sS):
+    """Flows of 35-45 px with medians 2 px off: a fifth of the pixels have a
+    weight sum below the 1e-5 floor, where the normalisation no longer cancels
+    an error common to a pixel's weights.  Differences taken after scaling by
+    the range factor (rounding relative to the flow, not the difference) were
+    off by 1.5e-3 px here; every kernel stays within 1e-3 px of the oracle."""
+    say("trilateral large flow", W, H, sS)
+    rng = np.random.default_rng(77)
+    p = bm.derive_params(W, H, 1.0, 1.0, N=32)
+    if sS is not None:
+        p.sS = sS
+    ctx = bm.Context(p, seed=1)
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float32)
+    U = (np.where(xx > W / 3, 38.0, -41.0) + 0.02 * yy + rng.normal(0, 1.5, (H, W))).astype(np.float32)
+    V = (np.where(yy > H / 2, -35.0, 44.0) + rng.normal(0, 1.5, (H, W))).astype(np.float32)
+    Um = (U + rng.normal(0, 2.0, (H, W))).astype(np.float32)
+    Vm = (V + rng.normal(0, 2.0, (H, W))).astype(np.float32)
+    guide = rng.integers(0, 256, (H, W)).astype(np.float32)
+    tU, tV = oracle.trilateral(U, V, Um, Vm, guide, p.sEPE ** 2, p.sI ** 2, p.sS ** 2)
+    ins = [bm.DeviceBuffer(array=a) for a in (U, V, Um, Vm, guide)]
+    for groups in (0, 2, 4):
+        ctx.set_tuning(tri_groups=groups)
+        Uo, Vo = ctx.image_buffer(), ctx.image_buffer()
+        ctx.trilateral(*ins, Uo, Vo)
+        ctx.sync()
+        eu = np.abs(Uo.download((H, W), np.float32) - tU).max()
+        ev = np.abs(Vo.download((H, W), np.float32) - tV).max()
+        print("[tri large]", W, H, sS, groups, eu, ev, flush=True)
+        assert eu <= FLOW_TOL and ev <= FLOW_TOL, (groups, eu, ev)
+
+
 @pytest.mark.parametrize("filt", [1, 0])
 def test_flow_pair_c1(filt):
     """The 320x240 / u1.5 / 32-bit workload, whole path, against the oracle."""
