@@ -288,6 +288,70 @@ def test_trained_proposal_gain():
     assert r["learned_fewer"] < r["plain_fewer"]
 
 
+def _staged_learned_epe(ctx, net, p, pr, with_far):
+    """The learned path through its public stages (features, far match,
+    network, offsets, seeded init, random search, one propagation iteration,
+    refinement), with or without the far match; returns the EPE."""
+    W, H = ctx.W, ctx.H
+    r1, r2 = bm.DeviceBuffer(array=pr.frame1), bm.DeviceBuffer(array=pr.frame2)
+    U, V = ctx.image_buffer(), ctx.image_buffer()
+    ctx.flow_pair(r1, r2, U, V)                    # gray frames and descriptors of the pair
+    g8, F = ctx.frames()
+    cw, ch = ctx.proposal_dims()
+    feat = bm.DeviceBuffer(ch * cw * 256)
+    ctx.proposal_features(g8[0], g8[1], feat)
+    far = None
+    if with_far:
+        far = bm.DeviceBuffer((H // 16) * (W // 16) * 8)
+        ctx.proposal_far(g8[0], g8[1], far)
+    ctx.sync()
+    nout = bm.DeviceBuffer(ch * cw * 64)
+    net.forward(feat, 1, ch, cw, nout)
+    coff = bm.DeviceBuffer(ch * cw * 16)
+    ctx.proposal_offsets(nout, feat, coff, far=far)
+    off, cost = ctx.field_buffers()
+    ctx.init_proposal(F[0], F[1], coff, off, cost, first=True)
+    ctx.random_search(F[0], F[1], off, cost, iteration=p.iterations, pair=0)
+    ctx.propagate(F[0], F[1], off, cost, -1)
+    ctx.random_search(F[0], F[1], off, cost, iteration=0, pair=0)
+    U0, V0 = ctx.image_buffer(), ctx.image_buffer()
+    ctx.downsample(off, U0, V0)
+    g32 = ctx.image_buffer()
+    ctx.gray(r1, None, g32)
+    ctx.refine(U0, V0, g32, U, V)
+    ctx.sync()
+    return bm.epe(U.download((H, W), np.float32), V.download((H, W), np.float32), pr.gt)[0]
+
+
+@pytest.mark.skipif(not os.path.exists(learned.DEFAULT_WEIGHTS), reason="no trained proposal weights")
+def test_far_match_gain():
+    """With one iteration, the far match lowers the learned path's EPE by at
+    least 10 % on 1280x720 pairs moving up to 60 px (beyond the cost volume's
+    40 px; measured 17 % with the committed weights, which were trained on
+    that motion too) and costs at most 3 % on the 1920x1080 pairs; the
+    composed stages with the far match are bm_flow_pair_learned itself."""
+    net = bm.Net(path=learned.DEFAULT_WEIGHTS)
+    for name, seeds in (("c3_1280x720_u4.9_n512", (0, 1)), ("c2_1920x1080_u2.9_n128", (0, 1))):
+        wl = synth.WORKLOADS[name]
+        p = bm.derive_params(wl["W"], wl["H"], wl["up"], wl["up"], N=wl["N"])
+        p.iterations = 1
+        ctx = bm.Context(p, seed=0)
+        e = {True: [], False: []}
+        for seed in seeds:
+            say("far match gain", name, seed)
+            pr = synth.workload_pair(name, seed=seed)
+            for with_far in (True, False):
+                e[with_far].append(_staged_learned_epe(ctx, net, p, pr, with_far))
+            U, V = ctx.flow_numpy_learned(net, pr.frame1, pr.frame2)
+            assert bm.epe(U, V, pr.gt)[0] == e[True][-1]
+        far, nofar = float(np.mean(e[True])), float(np.mean(e[False]))
+        print("[far gain] %s with %.3f without %.3f" % (name, far, nofar), flush=True)
+        if name.startswith("c3"):
+            assert far <= 0.9 * nofar
+        else:
+            assert far <= 1.03 * nofar
+
+
 def test_net_api_errors_and_roundtrip(tmp_path):
     """Invalid calls fail with a status before any launch; weights survive a
     save / load round trip bit for bit."""
