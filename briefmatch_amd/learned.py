@@ -33,9 +33,9 @@ from . import synth
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 # trained on 16,000 640x480 pairs, half moving rectangles and half large-motion
-# regions, labelled with their known flow:
-#   python -m briefmatch_amd.learned train --pairs 16000 --epochs 30 --lr 5e-4 \
-#       --labels gt --family mixed --regions-share 0.5 --out proposal_640x480.bnet
+# regions, labelled by the plain method with 8 propagation iterations:
+#   python -m briefmatch_amd.learned train --pairs 16000 --epochs 30 --lr 5e-4 --labels plain \
+#       --label-iterations 8 --family mixed --regions-share 0.5 --out proposal_640x480.bnet
 DEFAULT_WEIGHTS = os.path.join(_HERE, "weights", "proposal_640x480.bnet")
 
 # (cin, cout) per layer, as in the C ABI's parameter layout
@@ -254,6 +254,8 @@ def main(argv=None):
     t.add_argument("--family", choices=FAMILIES, default="rects",
                    help="training pairs: moving rectangles, or rectangles and large-motion regions")
     t.add_argument("--regions-share", type=float, default=0.5, help="share of region pairs in the mixed family")
+    t.add_argument("--label-iterations", type=int, default=4,
+                   help="propagation iterations of the plain method that labels the pairs (--labels plain)")
     t.add_argument("--out", required=True)
     t.add_argument("--eval", action="store_true", help="evaluate on the 1920x1080 workload afterwards")
     e = sub.add_parser("eval")
@@ -264,7 +266,7 @@ def main(argv=None):
     log = lambda m: print(m, flush=True)
     if a.cmd == "train":
         ds = build_dataset(a.pairs, workers=a.workers, labels=a.labels, family=a.family,
-                           regions_share=a.regions_share, log=log)
+                           regions_share=a.regions_share, iterations=a.label_iterations, log=log)
         net = Net(weights=init_weights(a.seed))
         rows = np.arange(a.pairs - a.val)
         val = np.arange(a.pairs - a.val, a.pairs)

@@ -327,8 +327,8 @@ def _staged_learned_epe(ctx, net, p, pr, with_far):
 def test_far_match_gain():
     """With one iteration, the far match lowers the learned path's EPE by at
     least 10 % on 1280x720 pairs moving up to 60 px (beyond the cost volume's
-    40 px; measured 17 % with the committed weights, which were trained on
-    that motion too) and costs at most 3 % on the 1920x1080 pairs; the
+    40 px; measured 56 % with the committed weights, 17 % with weights trained
+    on that motion's known flow) and costs at most 3 % on the 1920x1080 pairs; the
     composed stages with the far match are bm_flow_pair_learned itself."""
     net = bm.Net(path=learned.DEFAULT_WEIGHTS)
     for name, seeds in (("c3_1280x720_u4.9_n512", (0, 1)), ("c2_1920x1080_u2.9_n128", (0, 1))):
