@@ -226,15 +226,16 @@ def test_training_lowers_loss_and_is_reproducible():
     assert np.array_equal(nets[0].weights(), nets[1].weights())
 
 
-@pytest.mark.parametrize("motion", ["c1", "far"])
-def test_learned_path_equals_composed_oracle(motion):
+@pytest.mark.parametrize("motion,W,H", [("c1", 320, 240), ("far", 320, 240), ("small", 56, 14)])
+def test_learned_path_equals_composed_oracle(motion, W, H):
     """bm_flow_pair_learned = features + far match -> network -> offsets ->
     seeded init -> random search -> propagation + random search ->
     refinement, each stage on the oracle except the network (its device output
     is the oracle's input); "far": regions moving up to 60 pixels, where the
-    far match overrides the network in many cells."""
-    say("learned path", motion)
-    W, H, up, N = 320, 240, 1.5, 32
+    far match overrides the network in many cells; "small": 56 x 14 frames,
+    one coarse row and no 16-pixel block row, so there is no far match."""
+    say("learned path", motion, W, H)
+    up, N = 1.5, 32
     p = bm.derive_params(W, H, up, up, N=N)
     p.iterations = 1
     ctx = bm.Context(p, seed=11)
@@ -242,8 +243,10 @@ def test_learned_path_equals_composed_oracle(motion):
     net = bm.Net(path=path) if os.path.exists(path) else bm.Net(weights=_random_weights(1))
     if motion == "c1":
         pr = synth.workload_pair("c1_320x240_u1.5_n32", seed=2)
-    else:
+    elif motion == "far":
         pr = synth.make_pair(W, H, motion="regions", n_regions=6, max_motion=60.0, seed=1)
+    else:
+        pr = synth.make_pair(W, H, motion="rects", n_rects=1, max_motion=3.0, seed=4)
     r1 = bm.DeviceBuffer(array=pr.frame1)
     r2 = bm.DeviceBuffer(array=pr.frame2)
     U, V = ctx.image_buffer(), ctx.image_buffer()
