@@ -273,40 +273,37 @@ bool plan_brief(bm_ctx *ctx, int rmax, int rn_req)
     return true;
 }
 
-// Test records of the fused BRIEF: per descriptor word, the near tests (all
-// offsets within the near halo; record = LDS offsets of both samples) then the
-// far tests (record = packed int16 offsets), each with its bit position.
+// Test records of the fused BRIEF, 64 per descriptor word: first one record per
+// bit of the word in descending bit order (LDS offsets of both samples for a near
+// test, whose offsets all lie within the near halo; both offsets 0 for a far
+// test, which then shifts in a 0), then the far tests (packed int16 offsets and
+// the bit position).  Counts per word: bits, far tests.
 void build_brief_plan(bm_ctx *ctx)
 {
     const bm::Geometry &g = ctx->g;
     const int NSP = g.NSP, RN = g.brief_near, pitch = 64 + 2 * RN;
-    ctx->plan.assign((size_t)NSP * 32 * 4, 0u);
+    ctx->plan.assign((size_t)NSP * 64 * 4, 0u);
     ctx->plan_count.assign((size_t)NSP * 2, 0);
     for (int w = 0; w < g.NS; ++w) {
-        std::vector<int> near_, far_;
-        for (int b = 0; b < 32 && 32 * w + b < g.N; ++b) {
+        const int nbits = std::min(32, g.N - 32 * w);
+        int r = 0, nfar = 0;
+        for (int b = nbits - 1; b >= 0; --b) {
             const int16_t *o = &ctx->pattern[4 * (32 * w + b)];
             const int m = std::max(std::max(std::abs((int)o[0]), std::abs((int)o[1])),
                                    std::max(std::abs((int)o[2]), std::abs((int)o[3])));
-            (m <= RN ? near_ : far_).push_back(b);
+            uint32_t *rec = &ctx->plan[((size_t)64 * w + r++) * 4];
+            if (m <= RN) {
+                rec[0] = (uint32_t)(o[1] * pitch + o[0]);
+                rec[1] = (uint32_t)(o[3] * pitch + o[2]);
+            } else {
+                uint32_t *fr = &ctx->plan[((size_t)64 * w + 32 + nfar++) * 4];
+                fr[0] = (uint32_t)(uint16_t)o[0] | ((uint32_t)(uint16_t)o[1] << 16);
+                fr[1] = (uint32_t)(uint16_t)o[2] | ((uint32_t)(uint16_t)o[3] << 16);
+                fr[2] = (uint32_t)b;
+            }
         }
-        int r = 0;
-        for (int b : near_) {
-            const int16_t *o = &ctx->pattern[4 * (32 * w + b)];
-            uint32_t *rec = &ctx->plan[((size_t)32 * w + r++) * 4];
-            rec[0] = (uint32_t)(o[1] * pitch + o[0]);
-            rec[1] = (uint32_t)(o[3] * pitch + o[2]);
-            rec[2] = (uint32_t)b;
-        }
-        for (int b : far_) {
-            const int16_t *o = &ctx->pattern[4 * (32 * w + b)];
-            uint32_t *rec = &ctx->plan[((size_t)32 * w + r++) * 4];
-            rec[0] = (uint32_t)(uint16_t)o[0] | ((uint32_t)(uint16_t)o[1] << 16);
-            rec[1] = (uint32_t)(uint16_t)o[2] | ((uint32_t)(uint16_t)o[3] << 16);
-            rec[2] = (uint32_t)b;
-        }
-        ctx->plan_count[2 * w] = (int)near_.size();
-        ctx->plan_count[2 * w + 1] = (int)far_.size();
+        ctx->plan_count[2 * w] = nbits;
+        ctx->plan_count[2 * w + 1] = nfar;
     }
 }
 

@@ -27,8 +27,8 @@ struct Geometry {
     int brief_far;                // largest |pattern offset| (fused BRIEF far halo)
     int brief_near;               // fused BRIEF near halo: 8, 16, 24 or 32; 0 = kernel unavailable
     int brief_gw[2], brief_gh[2]; // largest gray tile of the fused BRIEF, [0]: 16-row tiles, [1]: 32-row tiles
-    const uint32_t *brief_plan;   // fused BRIEF test records (4 words each, 32 per descriptor word)
-    const int *brief_plan_count;  // per descriptor word: near and far record counts
+    const uint32_t *brief_plan;   // fused BRIEF test records (4 words each, 64 per descriptor word)
+    const int *brief_plan_count;  // per descriptor word: bit records, far records
     uint64_t seed;                // pattern + random-search key
     int64_t rs_sigma_q16;         // random-search sigma in Q16
     float tri_a2, tri_b;          // trilateral range scales: of the squared flow difference, of the gray level

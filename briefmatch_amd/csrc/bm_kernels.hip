@@ -255,8 +255,8 @@ __global__ void __launch_bounds__(kThreads) k_brief(const float *__restrict__ L,
 struct BriefArgs {
     const uint8_t *g8;
     const uint32_t *up_x, *up_y;
-    const uint4 *plan;      // 32 records per descriptor word, near tests first
-    const int *plan_count;  // per word: near count, far count
+    const uint4 *plan;      // 64 records per descriptor word: one per bit, then the far tests
+    const int *plan_count;  // per word: bit records, far records
     int N, W, H, sx, sy, Rf, gw, gh;
     uint32_t *F;
 };
@@ -327,18 +327,21 @@ __global__ void __launch_bounds__(kThreads) k_brief_lds(BriefArgs a)
 #pragma unroll
         for (int k = 0; k < PY; ++k) words[w][k] = 0u;
 
-    // The plan lists each word's tests near-first: record {o1, o2, bit, 0} with
-    // o1/o2 the LDS offsets (row * PITCH + col) of a near test, or the packed
-    // (ox, oy) int16 pairs of a far one.  Bits are placed explicitly, so the
-    // order of evaluation does not matter.  A near bit is the sign of b - a on
-    // the FP32 bit patterns: samples are non-negative (gray levels below 256,
-    // patterns below 2^31), and such floats order like their bit patterns.
+    // Per word the plan holds one record per bit in descending bit order, then
+    // the far tests.  A bit record {o1, o2} gives the LDS offsets (row * PITCH +
+    // col) of the two samples; the bit is the sign of b - a on the FP32 bit
+    // patterns (samples are non-negative: gray levels below 256, such floats
+    // order like their bit patterns) and is shifted in at the bottom,
+    // word = word << 1 | sign = alignbit(word, b - a, 31), so after the last
+    // record each bit sits at its position.  A far test's bit record has both
+    // offsets 0 and shifts in a 0; the far records {packed int16 offsets, bit}
+    // then OR their bits into place.
 #pragma unroll
     for (int w = 0; w < NSP; ++w) {
-        const uint4 *rec = a.plan + 32 * w;
-        const int nnear = a.plan_count[2 * w], nfar = a.plan_count[2 * w + 1];
+        const uint4 *rec = a.plan + 64 * w;
+        const int nbits = a.plan_count[2 * w], nfar = a.plan_count[2 * w + 1];
         int t = 0;
-        for (; t + 2 <= nnear; t += 2) {
+        for (; t + 2 <= nbits; t += 2) {
             const uint4 r0 = rec[t], r1 = rec[t + 1];
             const uint32_t *p1 = reinterpret_cast<const uint32_t *>(lbase + (int)r0.x);
             const uint32_t *p2 = reinterpret_cast<const uint32_t *>(lbase + (int)r0.y);
@@ -346,19 +349,21 @@ __global__ void __launch_bounds__(kThreads) k_brief_lds(BriefArgs a)
             const uint32_t *p4 = reinterpret_cast<const uint32_t *>(lbase + (int)r1.y);
 #pragma unroll
             for (int k = 0; k < PY; ++k) {
-                words[w][k] |= ((p2[4 * k * PITCH] - p1[4 * k * PITCH]) >> 31) << r0.z;
-                words[w][k] |= ((p4[4 * k * PITCH] - p3[4 * k * PITCH]) >> 31) << r1.z;
+                const uint32_t d0 = p2[4 * k * PITCH] - p1[4 * k * PITCH];
+                const uint32_t d1 = p4[4 * k * PITCH] - p3[4 * k * PITCH];
+                words[w][k] = __builtin_amdgcn_alignbit(__builtin_amdgcn_alignbit(words[w][k], d0, 31), d1, 31);
             }
         }
-        if (t < nnear) {
+        if (t < nbits) {
             const uint4 r0 = rec[t];
             const uint32_t *p1 = reinterpret_cast<const uint32_t *>(lbase + (int)r0.x);
             const uint32_t *p2 = reinterpret_cast<const uint32_t *>(lbase + (int)r0.y);
 #pragma unroll
-            for (int k = 0; k < PY; ++k) words[w][k] |= ((p2[4 * k * PITCH] - p1[4 * k * PITCH]) >> 31) << r0.z;
+            for (int k = 0; k < PY; ++k)
+                words[w][k] = __builtin_amdgcn_alignbit(words[w][k], p2[4 * k * PITCH] - p1[4 * k * PITCH], 31);
         }
         for (int f = 0; f < nfar; ++f) {
-            const uint4 r0 = rec[nnear + f];
+            const uint4 r0 = rec[32 + f];
             const int ox1 = (int16_t)(r0.x & 0xffffu), oy1 = (int16_t)(r0.x >> 16);
             const int ox2 = (int16_t)(r0.y & 0xffffu), oy2 = (int16_t)(r0.y >> 16);
             const uint32_t xa = xrow[ox1], xb = xrow[ox2];
