@@ -144,6 +144,18 @@ void BriefMatch::match(float *im1, float *im2)
     ++m_pairIndex;
 }
 
+void BriefMatch::matchLearned(bm_net *net, const uint8_t *rgb1, const uint8_t *rgb2)
+{
+    if (!m_setup) throw BriefMatchException("BriefMatch is not initialized before use.");
+    if (!net || !rgb1 || !rgb2) throw BriefMatchException("null argument passed to BriefMatch::matchLearned");
+    check(bm_flow_pair_learned(m_ctx, net, rgb1, rgb2, m_pairIndex, m_Uf, m_Vf));
+    m_outU = m_Uf;
+    m_outV = m_Vf;
+    m_haveF1 = false;  // the descriptors of the stage path no longer describe the previous frame
+    m_firstFrame = true;
+    ++m_pairIndex;
+}
+
 // briefmatch.cpp:296-324
 void BriefMatch::downsample()
 {

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <utility>
 
 #include "../../include/bm_io.h"
 
@@ -32,8 +33,8 @@ bool has_extension(const std::string &f, const char *ext)
 OFPipeline::OFPipeline()
     : m_doWriteMOT(false), m_doWriteFLO(false), m_doWriteLDR(false), m_verbose(false), m_doPrintError(false),
       m_havePrev(false), m_endFrame(0), m_frame(0), m_stepFrame(1), m_startFrame(0), m_sx(0), m_sy(0), m_sx_in(0),
-      m_sy_in(0), m_c(0), m_seed(0), m_upsampler("bilinear"), m_bm(nullptr), m_bmP(nullptr), m_timerCtx(nullptr), dev_rgb(nullptr),
-      dev_g8(nullptr), m_cur(0)
+      m_sy_in(0), m_c(0), m_seed(0), m_upsampler("bilinear"), m_net(nullptr), m_bm(nullptr), m_bmP(nullptr), m_timerCtx(nullptr),
+      dev_rgb(nullptr), dev_g8(nullptr), dev_rgbPrev(nullptr), m_cur(0)
 {
     dev_Lin[0] = dev_Lin[1] = dev_L[0] = dev_L[1] = nullptr;
 }
@@ -42,13 +43,15 @@ OFPipeline::~OFPipeline() { clear(); }
 
 void OFPipeline::clear()
 {
-    void *bufs[] = {dev_rgb, dev_g8, dev_Lin[0], dev_Lin[1], dev_L[0], dev_L[1]};
+    void *bufs[] = {dev_rgb, dev_g8, dev_rgbPrev, dev_Lin[0], dev_Lin[1], dev_L[0], dev_L[1]};
     for (void *b : bufs)
         if (b) bm_free(b);
-    dev_rgb = dev_g8 = nullptr;
+    dev_rgb = dev_g8 = dev_rgbPrev = nullptr;
     dev_Lin[0] = dev_Lin[1] = dev_L[0] = dev_L[1] = nullptr;
     delete m_bm;
     delete m_bmP;
+    if (m_net) bm_net_destroy(m_net);
+    m_net = nullptr;
     m_bm = nullptr;
     m_bmP = nullptr;
     m_timerCtx = nullptr;
@@ -126,6 +129,8 @@ bool OFPipeline::setParams(int argc, char *argv[])
         {"--seed", "-sd", "Seed of the BRIEF pattern and the random search", Option::UINT, nullptr, 0, 0},
         {"--upsampler", "-us", "Up-sampling filter: bilinear (default) or lanczos3 (the reference's)", Option::STR,
          &m_upsampler, 0, 0},
+        {"--proposal", "-pp", "Learned-proposal weights (.bnet): every pair through the proposal-seeded path",
+         Option::STR, &m_proposal, 0, 0},
     };
     auto usage = [&]() {
         std::printf("briefmatch -- computes the optical flow of a sequence of frames on an AMD MI355X\n\n"
@@ -235,6 +240,13 @@ void OFPipeline::setup()
     m_bm->setSeed(m_seed);
     m_bm->setup();
     m_timerCtx = m_bm->context();
+    if (!m_proposal.empty()) {
+        if (bm_net_create(&m_net) != BM_OK) throw BriefMatchException(bm_last_error(nullptr));
+        if (bm_net_load(m_net, m_proposal.c_str()) != BM_OK)
+            throw BriefMatchException(std::string("Cannot load the proposal weights.\n\t") + bm_net_last_error(m_net));
+        if (m_upsampler == "lanczos3" && bm_set_upsampler(m_timerCtx, BM_UPSAMPLE_LANCZOS3) != BM_OK)
+            throw BriefMatchException(bm_last_error(m_timerCtx));
+    }
     const size_t Win = (size_t)m_sx_in * m_sy_in, M = (size_t)m_sx * m_sy;
     void *p = nullptr;
     auto alloc = [&](size_t bytes) {
@@ -242,6 +254,7 @@ void OFPipeline::setup()
         return p;
     };
     dev_rgb = (uint8_t *)alloc(Win * 3);
+    if (m_net) dev_rgbPrev = (uint8_t *)alloc(Win * 3);
     dev_g8 = (uint8_t *)alloc(Win);
     for (int k = 0; k < 2; ++k) {
         dev_Lin[k] = (float *)alloc(Win * 4);
@@ -249,6 +262,20 @@ void OFPipeline::setup()
     }
     m_havePrev = false;
     m_cur = 0;
+}
+
+// Download the last pair's flow and write the requested outputs (of_pipeline.cpp:341-373).
+void OFPipeline::writeOutputs()
+{
+    if (m_doWriteMOT || m_doWriteFLO || m_doWriteLDR || m_doPrintError) m_bm->convert();
+    const std::string out = m_output.empty() ? std::string() : framePath(m_output, m_frame);
+    if (m_doWriteMOT) m_bm->printRaw(out);
+    if (m_doWriteFLO) m_bm->print(out);
+    if (m_doWriteLDR) {
+        m_bm->computeColors();
+        m_bm->printVis(out);
+    }
+    if (m_doPrintError) std::printf("Error (EPE) = %f\n\n", m_bm->error(m_flowGT));
 }
 
 // One frame (of_pipeline.cpp:241-387): read, upload, gray + up-sample, match
@@ -284,9 +311,27 @@ bool OFPipeline::processFrame()
         }
     };
     ok(bm_sync(ctx));  // the upload buffer may still be read by the previous frame
+    if (m_net) std::swap(dev_rgb, dev_rgbPrev);  // the last frame's RGB stays on the device
     ok(bm_memcpy(dev_rgb, rgb.data(), rgb.size(), BM_HOST_TO_DEVICE));
     ok(bm_timer_start(ctx));
     const int cur = m_cur, prev = 1 - m_cur;
+    if (m_net) {
+        // the learned path takes both RGB frames and does gray, up-sampling,
+        // matching and refinement itself
+        if (m_havePrev) {
+            m_bm->matchLearned(m_net, dev_rgbPrev, dev_rgb);
+            float ms = 0;
+            ok(bm_timer_stop(ctx, &ms));
+            if (m_verbose) std::printf("\tdevice time (learned path, whole pair): %.2f ms\n", ms);
+            writeOutputs();
+        } else {
+            float ms = 0;
+            ok(bm_timer_stop(ctx, &ms));
+        }
+        m_havePrev = true;
+        m_frame += m_stepFrame;
+        return true;
+    }
     ok(bm_gray(ctx, dev_rgb, dev_g8, dev_Lin[cur]));
     if (m_upsampler == "lanczos3")
         ok(bm_upsample_lanczos(ctx, dev_g8, dev_L[cur]));
@@ -301,15 +346,7 @@ bool OFPipeline::processFrame()
         float ms = 0;
         ok(bm_timer_stop(ctx, &ms));
         if (m_verbose) std::printf("\tdevice time (gray, up-sample, match, refine): %.2f ms\n", ms);
-        if (m_doWriteMOT || m_doWriteFLO || m_doWriteLDR || m_doPrintError) m_bm->convert();
-        const std::string out = m_output.empty() ? std::string() : framePath(m_output, m_frame);
-        if (m_doWriteMOT) m_bm->printRaw(out);
-        if (m_doWriteFLO) m_bm->print(out);
-        if (m_doWriteLDR) {
-            m_bm->computeColors();
-            m_bm->printVis(out);
-        }
-        if (m_doPrintError) std::printf("Error (EPE) = %f\n\n", m_bm->error(m_flowGT));
+        writeOutputs();
         wrote = true;
     } else {
         // the first frame has no predecessor: describe it on the next call

@@ -74,6 +74,11 @@ public:
 
     void swap();
     void match(float *im1, float *im2);
+    // No counterpart in the reference: the whole pair through the learned
+    // proposal (bm_flow_pair_learned) from two device RGB frames at input
+    // resolution, flow into the object's output buffers (no temporal
+    // initialisation); convert() and the writers then work as after process().
+    void matchLearned(bm_net *net, const uint8_t *rgb1, const uint8_t *rgb2);
     void downsample();
     void process(float *im1 = NULL);
 

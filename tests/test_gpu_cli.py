@@ -76,3 +76,32 @@ def test_cli_lanczos_equals_flow_pair(tmp_path):
     ctx.set_upsampler(bm.UPSAMPLE_LANCZOS3)
     pU, pV = ctx.flow_numpy(f10, f11, pair=0)
     assert np.array_equal(U, pU) and np.array_equal(V, pV)
+
+
+def test_cli_proposal_equals_flow_pair_learned(tmp_path):
+    """--proposal through OFPipeline equals bm_flow_pair_learned on the same
+    frames; in a three-frame run the second pair (stream index 1) reads the
+    previous frame's RGB kept on the device."""
+    from briefmatch_amd import learned
+    if not os.path.exists(learned.DEFAULT_WEIGHTS):
+        pytest.skip("no trained proposal weights")
+    _run(["-i", os.path.join(GOLD, "frame%02d.png"), "-f", "9:11", "-ux", "2.9", "-uy", "2.9", "-fl", "128",
+          "-it", "1", "--proposal", learned.DEFAULT_WEIGHTS, "-o", str(tmp_path / "flow%02d.flo")], cwd=tmp_path)
+    assert not (tmp_path / "flow09.flo").exists()
+    frames = {k: bio.read_image(os.path.join(GOLD, "frame%02d.png" % k)) for k in (9, 10, 11)}
+    p = bm.derive_params(584, 388, 2.9, 2.9, N=128)
+    p.iterations = 1
+    ctx = bm.Context(p, seed=0)
+    net = bm.Net(path=learned.DEFAULT_WEIGHTS)
+    for pair, (a, b) in enumerate(((9, 10), (10, 11))):
+        U, V = bio.read_flo(str(tmp_path / ("flow%02d.flo" % b)))
+        pU, pV = ctx.flow_numpy_learned(net, frames[a], frames[b], pair=pair)
+        assert np.array_equal(U, pU) and np.array_equal(V, pV), (a, b)
+
+
+def test_cli_proposal_missing_weights(tmp_path):
+    r = subprocess.run([bio.CLI_PATH, "-i", os.path.join(GOLD, "frame%02d.png"), "-f", "10:11",
+                        "--proposal", str(tmp_path / "none.bnet"), "-o", str(tmp_path / "f%02d.flo")],
+                       capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode != 0
+    assert "proposal weights" in (r.stdout + r.stderr)
