@@ -493,7 +493,8 @@ void orc_median1d(const float *in, float *out, int W, int H, int dir, int fsize)
     const int ks = fsize / 2;
 #pragma omp parallel for schedule(static)
     for (int y = 0; y < H; ++y) {
-        float v[64];
+        std::vector<float> vbuf((size_t)fsize);
+        float *v = vbuf.data();
         for (int x = 0; x < W; ++x) {
             for (int i = 0; i < fsize; ++i) {
                 int xx = dir ? clampi(x - ks + i, 0, W - 1) : x;

@@ -4,6 +4,7 @@ cases that reduce to library routines, and brute force on tiny inputs.  Runs on
 the CPU only."""
 import json
 import os
+import shutil
 
 import numpy as np
 import pytest
@@ -498,3 +499,20 @@ def test_median_networks(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count("mismatches 0") == 24
+
+
+@pytest.mark.skipif(shutil.which("make") is None or shutil.which("g++") is None, reason="no host toolchain")
+def test_oracle_under_sanitizers():
+    """The oracle under AddressSanitizer and UBSan (oracle/san_main.cpp): the
+    whole path with both up-samplers on odd frame sizes and descriptor lengths
+    2..512, the stages alone (temporal init, levels in both directions, random
+    search), median windows wider than the frame, partial trilateral row
+    ranges and EPE with unknown ground truth; any report fails the run."""
+    import subprocess
+    odir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    b = subprocess.run(["make", "-C", odir, "san"], capture_output=True, text=True, timeout=600)
+    assert b.returncode == 0, b.stdout + b.stderr
+    r = subprocess.run([os.path.join(odir, "oracle_san")], capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=1"))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count(" ok") == 6
