@@ -97,6 +97,10 @@ void read_png(const std::string &path, const std::vector<uint8_t> &d, int &W, in
     if (!(depth == 8 || (depth == 16 && ctype != 3))) throw IOError("unsupported PNG bit depth in '" + path + "'");
     if (W <= 0 || H <= 0 || W > 65535 || H > 65535) throw IOError("bad PNG size in '" + path + "'");
     const size_t bpp = (size_t)ch * depth / 8, row = (size_t)W * bpp;
+    // deflate expands at most 1032:1, so a header that promises more pixel data
+    // than the compressed bytes can hold is corrupt; checked before allocating,
+    // so a damaged header cannot request gigabytes
+    if ((row + 1) * H > (idat.size() + 1) * 1032 + 64) throw IOError("corrupt PNG data in '" + path + "'");
     std::vector<uint8_t> raw((row + 1) * H);
     uLongf out = (uLongf)raw.size();
     if (uncompress(raw.data(), &out, idat.data(), (uLong)idat.size()) != Z_OK || out != raw.size())
@@ -162,6 +166,7 @@ void read_pnm(const std::string &path, const std::vector<uint8_t> &d, int &W, in
         while (pos < d.size() && std::isdigit(d[pos])) {
             v = v * 10 + (d[pos++] - '0');
             any = true;
+            if (v > 100000000L) throw IOError("bad PNM header in '" + path + "'");  // no such image fits a file
         }
         if (!any) throw IOError("bad PNM header in '" + path + "'");
         return v;

@@ -120,3 +120,24 @@ def test_cli_parameter_errors(tmp_path):
         r = _cli(*args)
         assert r.returncode == 1, args
         assert msg in r.stderr, (args, r.stderr)
+
+
+@pytest.mark.skipif(__import__("shutil").which("g++") is None, reason="no host compiler")
+def test_readers_under_sanitizers(tmp_path):
+    """The PNG / PNM / .flo readers under AddressSanitizer and UBSan
+    (tests/io_sanitize.cpp): every truncation and 3,000 seeded mutations of
+    valid files each, and headers that promise gigabytes; every input parses
+    or raises IOError, and no input may make the reader allocate more than
+    2 GiB (ASan's allocation cap turns that into a failure)."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "io_sanitize")
+    b = subprocess.run(["g++", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                        "-std=c++17", "-I" + os.path.join(root, "include"),
+                        os.path.join(root, "tests", "io_sanitize.cpp"),
+                        os.path.join(root, "briefmatch_amd", "csrc", "bm_io.cpp"), "-lz", "-o", exe],
+                       capture_output=True, text=True, timeout=600)
+    assert b.returncode == 0, b.stderr
+    env = dict(os.environ, ASAN_OPTIONS="max_allocation_size_mb=2048:allocator_may_return_null=0")
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=600, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ok" in r.stdout
