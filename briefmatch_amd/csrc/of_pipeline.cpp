@@ -3,6 +3,7 @@
 
 #include <strings.h>
 
+#include <cctype>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -56,6 +57,33 @@ void OFPipeline::clear()
     m_bmP = nullptr;
     m_timerCtx = nullptr;
 }
+
+namespace {
+
+// A frame pattern is a printf format applied to the frame number alone: "%%"
+// escapes and at most one integer conversion (flags, width, precision and an
+// h/l length allowed).  Anything else would make snprintf read arguments that
+// are not there.
+bool valid_frame_pattern(const std::string &s)
+{
+    int conversions = 0;
+    for (size_t i = 0; i < s.size(); ++i) {
+        if (s[i] != '%') continue;
+        if (++i < s.size() && s[i] == '%') continue;
+        while (i < s.size() && std::strchr("-+ 0#", s[i])) ++i;
+        while (i < s.size() && std::isdigit((unsigned char)s[i])) ++i;
+        if (i < s.size() && s[i] == '.') {
+            ++i;
+            while (i < s.size() && std::isdigit((unsigned char)s[i])) ++i;
+        }
+        while (i < s.size() && (s[i] == 'h' || s[i] == 'l')) ++i;
+        if (i >= s.size() || !std::strchr("diuxXo", s[i])) return false;
+        ++conversions;
+    }
+    return conversions <= 1;
+}
+
+}  // namespace
 
 std::string OFPipeline::framePath(const std::string &pattern, unsigned int frame)
 {
@@ -180,6 +208,11 @@ bool OFPipeline::setParams(int argc, char *argv[])
             *(float *)o->var = (float)x;
     }
     if (m_input.empty()) throw BriefMatchException("Required input option '--input' (-i) is missing.");
+    for (const std::string *pat : {&m_input, &m_output})
+        if (!valid_frame_pattern(*pat))
+            throw BriefMatchException("Frame pattern '" + *pat +
+                                      "' may hold one integer conversion for the frame number (e.g. frame%02d.png) "
+                                      "and %% escapes only.");
     if (m_upsampler != "bilinear" && m_upsampler != "lanczos3")
         throw BriefMatchException("Argument '--upsampler' expects bilinear or lanczos3, got '" + m_upsampler + "'");
     m_seed = seed;

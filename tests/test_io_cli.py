@@ -115,6 +115,9 @@ def test_cli_parameter_errors(tmp_path):
         (["-i", frames, "-f", "10:11", "-fl"], "No value provided"),
         (["-i", str(tmp_path / "none%02d.png"), "-f", "1:2"], "Cannot read image file"),
         (["-i", frames, "-f", "10:11", "--upsampler", "cubic"], "expects bilinear or lanczos3"),
+        (["-i", os.path.join(GOLD, "frame%s.png"), "-f", "10:11"], "Frame pattern"),
+        (["-i", os.path.join(GOLD, "f%02d_%d.png"), "-f", "10:11"], "Frame pattern"),
+        (["-i", frames, "-f", "10:11", "-o", "out%n.flo"], "Frame pattern"),
     ]
     for args, msg in cases:
         r = _cli(*args)
