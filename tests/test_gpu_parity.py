@@ -88,6 +88,45 @@ def test_gray_upsample_brief(W, H, up, N):
     assert np.array_equal(ctx.download_desc(Ff), oF)
 
 
+@pytest.mark.parametrize("N", [72, 128])
+def test_brief_pattern_halo_boundaries(N):
+    """User patterns whose tests sit exactly at the near halo, one pixel beyond
+    it, at zero and far out, in words that mix them, words of near tests only
+    and words of far tests only (and a partial last word for N = 72): the
+    fused BRIEF (one bit record per bit in bit order, far bits through their
+    own records) equals the oracle for every near halo the tuner can pick."""
+    say("brief pattern boundaries", N)
+    W, H, up = 96, 64, 2.0
+    pr = _pair(W, H, seed=N)
+    p = bm.derive_params(W, H, up, up, N=N)
+    ctx = bm.Context(p, seed=3)
+    oL = oracle.upsample(oracle.gray(pr.frame1), p.sx, p.sy)
+    rgb = bm.DeviceBuffer(array=pr.frame1)
+    g8 = bm.DeviceBuffer(W * H)
+    ctx.gray(rgb, g8=g8)
+    rng = np.random.default_rng(N)
+    for rn in (8, 16, 24, 32, 40, 48):
+        pat = np.zeros((N, 4), np.int16)
+        for b in range(N):
+            kind = (b // 32) % 3 if b // 32 < 3 else rng.integers(0, 3)
+            near = rng.integers(-rn, rn + 1, 4)
+            near[rng.integers(0, 4)] = rng.choice([-rn, rn, 0])
+            if kind == 1 or (kind == 0 and b % 2 == 0):
+                pat[b] = near
+            else:
+                far = near.copy()
+                far[rng.integers(0, 4)] = rng.choice([-(rn + 1), rn + 1, -60, 60])
+                pat[b] = far
+        ctx.set_tuning(brief_near=rn)
+        ctx.set_pattern(pat.ravel())
+        F = ctx.desc_buffer()
+        ctx.brief_gray(g8, F)
+        ctx.sync()
+        oF = _pad(oracle.brief(oL, pat.ravel(), N), ctx.NSP)
+        assert np.array_equal(ctx.download_desc(F), oF), (rn, ctx.tuning().brief_near)
+        print("[brief halo]", N, rn, ctx.tuning().brief_near, flush=True)
+
+
 @pytest.mark.parametrize("N,double_prop", [(32, True), (128, True), (128, False), (256, True), (512, True)])
 def test_levels_random_search_init(N, double_prop):
     """Level / random-search / init parity from a random (non-trivial) snapshot."""
