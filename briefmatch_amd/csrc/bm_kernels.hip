@@ -495,10 +495,25 @@ __global__ void __launch_bounds__(kThreads) k_level(const uint32_t *__restrict__
         // candidate order of the reference: d = -1 (vertical, horizontal), d = +1 (...)
         const int nidx[4] = {idx[q] - sv, idx[q] - sh, idx[q] + sv, idx[q] + sh};
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < 4; ++c)
             valid[q][c] = live[q] && (c >= 2 || double_prop) && (unsigned)nidx[c] < (unsigned)M;
-            o[q][c] = valid[q][c] ? off_in[nidx[c]] : 0u;
+        // a horizontal neighbour less than a wave away in the same row is another
+        // lane's own entry: taken from that lane (through LDS, idle here) instead
+        // of a load, which relieves the address units that bound this kernel
+        bool lane_h[4] = {false, false, false, false};
+        if (p < 64) {  // uniform
+            const int lane = threadIdx.x & 63;
+#pragma unroll
+            for (int c = 1; c < 4; c += 2) {
+                const int dl = c == 1 ? -sh : sh, src = lane + dl, xs = xq[q] + dl;
+                const uint32_t v = __shfl(own[q], src & 63);
+                lane_h[c] = valid[q][c] && src >= 0 && src < 64 && xs >= 0 && xs < sx;
+                o[q][c] = v;
+            }
         }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (!lane_h[c]) o[q][c] = valid[q][c] ? off_in[nidx[c]] : 0u;
     }
     // the random-search candidate depends only on (pixel, iteration, pair)
     uint32_t rcand[PX];
