@@ -470,6 +470,9 @@ class Context:
         self._c(lib().bm_proposal_far(self.h, _ptr(g1), _ptr(g2), _ptr(far)))
 
     def proposal_offsets(self, net_out, feat, coff, far=None):
+        """Per-cell proposals [ch][cw][4] int32 (network u, v and argmin u, v in Q8
+        matched pixels); with a far match (proposal_far) it replaces the network's
+        flow where it reaches ring 3."""
         self._c(lib().bm_proposal_offsets(self.h, _ptr(net_out), _ptr(feat), _ptr(far), _ptr(coff)))
 
     def init_proposal(self, F1, F2, coff, off, cost, first=True, prev_off=None):
