@@ -69,6 +69,13 @@ struct bm_ctx {
     uint16_t *d_cost[2] = {nullptr, nullptr};
     float *d_U0 = nullptr, *d_V0 = nullptr, *d_tU = nullptr, *d_tV = nullptr, *d_Um = nullptr, *d_Vm = nullptr;
     int field = 0;  // d_off[field] holds the last match field
+    // configuration generation: bumped by every change that alters launch
+    // arguments (pattern and BRIEF plan, tuning, up-sampler), so captured
+    // batch graphs know when to re-capture
+    uint64_t gen = 0;
+    // a batch's origin stream when the context belongs to a batch: its graph
+    // may still read the context's tables when the configuration changes
+    hipStream_t batch_stream = nullptr;
     // up-sampler of the whole-path calls (Lanczos tables and frame allocated on first use)
     int upsampler = BM_UPSAMPLE_BILINEAR;
     int *d_lz_x0 = nullptr, *d_lz_y0 = nullptr;
@@ -322,15 +329,45 @@ int pattern_rmax(const bm_ctx *ctx)
     return rmax;
 }
 
-// rn_req: -1 default near halo, 0 stand-alone kernel, else the requested halo.
+// rn_req: -1 default near halo (the stand-alone kernel when the fused one's
+// tiles do not fit), 0 stand-alone kernel, else the requested halo, which must
+// fit: then nothing changes on failure.  The plan buffers on the device hold
+// the largest plan (NSP * 64 records, NSP * 2 counts) from creation on, so a
+// pattern set later never outgrows them.
 int set_pattern_internal(bm_ctx *ctx, const std::vector<int16_t> &pat, int rn_req = -1)
 {
+    const std::vector<int16_t> old_pat = ctx->pattern;
+    const bm::Geometry old_g = ctx->g;
     ctx->pattern = pat;
     const int rmax = pattern_rmax(ctx);
     ctx->g.brief_radius = std::min(rmax, bm::brief_max_radius());
-    if (!plan_brief(ctx, rmax, rn_req)) return fail(ctx, BM_E_PARAM, "BRIEF near halo does not fit in LDS");
-    build_brief_plan(ctx);
+    if (!plan_brief(ctx, rmax, rn_req)) {
+        if (rn_req > 0) {
+            ctx->pattern = old_pat;
+            ctx->g = old_g;
+            return fail(ctx, BM_E_PARAM, "BRIEF near halo does not fit in LDS");
+        }
+        plan_brief(ctx, rmax, 0);  // stand-alone kernel
+    }
+    if (ctx->g.brief_near > 0) {
+        build_brief_plan(ctx);
+    } else {
+        ctx->plan.assign((size_t)ctx->g.NSP * 64 * 4, 0u);
+        ctx->plan_count.assign((size_t)ctx->g.NSP * 2, 0);
+    }
+    ctx->gen++;
     if (ctx->d_pattern) return upload_pattern(ctx);
+    return BM_OK;
+}
+
+// Waits for every stream that may still read the context's tables: its own
+// two and, for a batch member, the batch's origin stream (a captured batch
+// graph runs there).  Called before a configuration change re-uploads tables.
+int quiesce(bm_ctx *ctx)
+{
+    BM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->side) BM_HIP(ctx, hipStreamSynchronize(ctx->side));
+    if (ctx->batch_stream) BM_HIP(ctx, hipStreamSynchronize(ctx->batch_stream));
     return BM_OK;
 }
 
@@ -667,8 +704,8 @@ int bm_ctx_create(const bm_params *params, uint64_t seed, bm_ctx **out)
 #define BM_ALLOC(ptr, n) \
     if (r == BM_OK) r = dev_alloc(ctx, &(ptr), (n))
     BM_ALLOC(ctx->d_pattern, pat.size());
-    BM_ALLOC(ctx->d_plan, ctx->plan.size());
-    BM_ALLOC(ctx->d_plan_count, ctx->plan_count.size());
+    BM_ALLOC(ctx->d_plan, (size_t)NSP * 64 * 4);
+    BM_ALLOC(ctx->d_plan_count, (size_t)NSP * 2);
     BM_ALLOC(ctx->d_up_x, upx.size());
     BM_ALLOC(ctx->d_up_y, upy.size());
     BM_ALLOC(ctx->d_nn_x, nnx.size());
@@ -768,7 +805,8 @@ int bm_set_pattern(bm_ctx *ctx, const int16_t *pat)
 {
     if (!ctx || !pat) return fail(nullptr, BM_E_PARAM, "null argument");
     std::vector<int16_t> v(pat, pat + 4 * (size_t)ctx->p.N);
-    BM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int q = quiesce(ctx);
+    if (q) return q;
     const int keep = ctx->g.brief_near > 0 ? ctx->g.brief_near : -1;
     int r = set_pattern_internal(ctx, v, keep);
     if (r != BM_OK && keep > 0) r = set_pattern_internal(ctx, v, -1);
@@ -870,11 +908,12 @@ int bm_set_upsampler(bm_ctx *ctx, int mode)
     if (!ctx) return fail(nullptr, BM_E_PARAM, "null context");
     if (mode != BM_UPSAMPLE_BILINEAR && mode != BM_UPSAMPLE_LANCZOS3)
         return fail(ctx, BM_E_PARAM, "unknown up-sampler " + std::to_string(mode));
-    if (mode == BM_UPSAMPLE_LANCZOS3) {
-        int r = lanczos_ready(ctx, true);
-        if (r) return r;
-    }
+    if (mode == ctx->upsampler) return BM_OK;
+    int r = quiesce(ctx);
+    if (!r && mode == BM_UPSAMPLE_LANCZOS3) r = lanczos_ready(ctx, true);
+    if (r) return r;
     ctx->upsampler = mode;
+    ctx->gen++;
     return BM_OK;
 }
 
@@ -1422,7 +1461,7 @@ struct bm_batch {
     std::vector<hipEvent_t> joins;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    std::vector<const void *> key;
+    std::vector<uintptr_t> key;
     std::string err;
 };
 
@@ -1446,7 +1485,9 @@ int apply_tuning(bm_ctx *ctx, const bm_tuning &t)
 {
     std::string why;
     if (!tuning_valid(ctx, t, why)) return fail(ctx, BM_E_PARAM, why);
-    BM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int q = quiesce(ctx);
+    if (q) return q;
+    ctx->gen++;
     if (t.brief_near != ctx->g.brief_near) {
         const std::vector<int16_t> pat = ctx->pattern;
         const int r = set_pattern_internal(ctx, pat, t.brief_near);
@@ -1663,6 +1704,7 @@ int bm_batch_create(int n, const bm_params *params, uint64_t seed, int streams, 
         bm_batch_destroy(b);
         return fail(nullptr, BM_E_LAUNCH, m);
     }
+    for (bm_ctx *c : b->ctx) c->batch_stream = b->origin;
     *out = b;
     return BM_OK;
 }
@@ -1710,12 +1752,16 @@ int bm_batch_flow(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r
         if (hipStreamSynchronize(c->stream) != hipSuccess) return batch_fail(b, BM_E_LAUNCH, "stream sync failed");
     if (mode == BM_BATCH_PER_PAIR) return batch_enqueue(b, r1, r2, U, V, true);
     if (mode != BM_BATCH_GRAPH) return batch_fail(b, BM_E_PARAM, "unknown batch mode");
-    std::vector<const void *> key;
+    // the graph bakes in the buffers and every launch argument of every pair:
+    // a new buffer or a configuration change (tuning, pattern, up-sampler) of
+    // any pair's context means a new capture
+    std::vector<uintptr_t> key;
     for (int i = 0; i < n; ++i) {
-        key.push_back(r1[i]);
-        key.push_back(r2[i]);
-        key.push_back(U[i]);
-        key.push_back(V[i]);
+        key.push_back((uintptr_t)r1[i]);
+        key.push_back((uintptr_t)r2[i]);
+        key.push_back((uintptr_t)U[i]);
+        key.push_back((uintptr_t)V[i]);
+        key.push_back((uintptr_t)b->ctx[i]->gen);
     }
     if (!b->exec || key != b->key) {
         if (b->exec) (void)hipGraphExecDestroy(b->exec);

@@ -62,8 +62,9 @@ namespace {
 
 // A frame pattern is a printf format applied to the frame number alone: "%%"
 // escapes and at most one integer conversion (flags, width, precision and an
-// h/l length allowed).  Anything else would make snprintf read arguments that
-// are not there.
+// h or hh length allowed).  The frame number is passed as an unsigned int, so
+// an l, ll, j, z or t length (or any other conversion) would make snprintf read
+// an argument of a type that was never passed.
 bool valid_frame_pattern(const std::string &s)
 {
     int conversions = 0;
@@ -76,7 +77,7 @@ bool valid_frame_pattern(const std::string &s)
             ++i;
             while (i < s.size() && std::isdigit((unsigned char)s[i])) ++i;
         }
-        while (i < s.size() && (s[i] == 'h' || s[i] == 'l')) ++i;
+        for (int h = 0; h < 2 && i < s.size() && s[i] == 'h'; ++h) ++i;
         if (i >= s.size() || !std::strchr("diuxXo", s[i])) return false;
         ++conversions;
     }

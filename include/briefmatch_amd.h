@@ -118,11 +118,14 @@ int bm_memset(void *ptr_dev, int value, size_t bytes);
 
 /* Interleaved RGB uint8 (W*H*3) -> gray uint8 and/or FP32 (either may be NULL). */
 int bm_gray(bm_ctx *ctx, const uint8_t *rgb_dev, uint8_t *gray8_dev, float *gray32_dev);
-/* Up-sampling sx_in x sy_in gray8 -> sx x sy FP32, bilinear in fixed point
- * (replaces CUDA_resizeLanczos, util_cuda.cu:250-257). */
+/* Up-sampling sx_in x sy_in gray8 -> sx x sy FP32, bilinear in fixed point:
+ * the default up-sampler of the whole-path calls (no counterpart in the
+ * reference, whose CUDA_resizeLanczos, util_cuda.h:124, maps to
+ * bm_upsample_lanczos below). */
 int bm_upsample(bm_ctx *ctx, const uint8_t *gray8_dev, float *L_dev);
-/* Lanczos-3 up-sampling sx_in x sy_in gray8 -> sx x sy FP32, the reference's
- * resizeLanczos (util_cuda.cu:126-158) with its 6 x 6 weights factored into
+/* Lanczos-3 up-sampling sx_in x sy_in gray8 -> sx x sy FP32: CUDA_resizeLanczos
+ * (util_cuda.h:124, util_cuda.cu:250-257), the reference's resizeLanczos
+ * (util_cuda.cu:126-158) with its 6 x 6 weights factored into
  * per-column and per-row tables (normalised, FP32) and fused multiply-adds. */
 int bm_upsample_lanczos(bm_ctx *ctx, const uint8_t *gray8_dev, float *L_dev);
 /* Up-sampler of the whole-path calls (bm_flow_pair, bm_stream_frame,

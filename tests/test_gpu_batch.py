@@ -120,3 +120,42 @@ def test_batch_equals_per_pair_and_oracle():
         eo, ec, oU, oV = oracle.flow_pair(op, pr.frame1, pr.frame2, seed=11, pair=i)
         assert np.array_equal(base[i][0], eo) and np.array_equal(base[i][1], ec), i
         assert np.abs(base[i][2] - oU).max() <= 1e-3 and np.abs(base[i][3] - oV).max() <= 1e-3
+
+
+def test_batch_graph_follows_configuration_changes():
+    """A captured batch graph bakes in launch arguments (near halo, level
+    kernel shape, up-sampler) and the BRIEF plan's layout: changing a member
+    context's tuning, pattern or up-sampler between two graph launches must
+    re-capture, so each graph result equals the per-pair launches under the
+    same configuration."""
+    say("batch graph after configuration changes")
+    wl = _batch_workload(3, 9)
+    batch = bm.Batch([p for p, _ in wl], seed=5, streams=2)
+    r1 = [bm.DeviceBuffer(array=pr.frame1) for _, pr in wl]
+    r2 = [bm.DeviceBuffer(array=pr.frame2) for _, pr in wl]
+    Us = [bm.DeviceBuffer(p.sx_in * p.sy_in * 4) for p, _ in wl]
+    Vs = [bm.DeviceBuffer(p.sx_in * p.sy_in * 4) for p, _ in wl]
+
+    def run(mode):
+        batch.flow(r1, r2, Us, Vs, mode=mode)
+        batch.sync()
+        return [(batch.ctx[i].field(), Us[i].download((p.sy_in, p.sx_in), np.float32))
+                for i, (p, _) in enumerate(wl)]
+
+    def same(a, b):
+        return all(np.array_equal(x[0][0], y[0][0]) and np.array_equal(x[0][1], y[0][1]) and
+                   np.array_equal(x[1], y[1]) for x, y in zip(a, b))
+
+    run(bm.BATCH_GRAPH)                                  # captured with the initial configuration
+    c = batch.ctx[1]
+    t = c.tuning()
+    changes = [lambda: c.set_tuning(brief_near=16 if t.brief_near != 16 else 24, strip_rows=64),
+               lambda: c.set_pattern(np.random.default_rng(3).integers(-12, 13, 4 * c.params.N).astype(np.int16)),
+               lambda: c.set_upsampler(bm.UPSAMPLE_LANCZOS3)]
+    for k, change in enumerate(changes):
+        before = run(bm.BATCH_GRAPH)
+        change()
+        got = run(bm.BATCH_GRAPH)
+        want = run(bm.BATCH_PER_PAIR)
+        assert same(got, want), k
+        assert not same(before[1:2], got[1:2]) or k == 0, k  # pattern / up-sampler change the result

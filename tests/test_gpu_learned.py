@@ -226,17 +226,23 @@ def test_training_lowers_loss_and_is_reproducible():
     assert np.array_equal(nets[0].weights(), nets[1].weights())
 
 
-@pytest.mark.parametrize("motion,W,H", [("c1", 320, 240), ("far", 320, 240), ("small", 56, 14)])
-def test_learned_path_equals_composed_oracle(motion, W, H):
+@pytest.mark.parametrize("motion,W,H,ux,uy", [("c1", 320, 240, 1.5, 1.5), ("far", 320, 240, 1.5, 1.5),
+                                             ("small", 56, 14, 1.5, 1.5), ("c1", 200, 150, 0.0, 0.0)])
+def test_learned_path_equals_composed_oracle(motion, W, H, ux, uy):
     """bm_flow_pair_learned = features + far match -> network -> offsets ->
     seeded init -> random search -> propagation + random search ->
     refinement, each stage on the oracle except the network (its device output
     is the oracle's input); "far": regions moving up to 60 pixels, where the
     far match overrides the network in many cells; "small": 56 x 14 frames,
-    one coarse row and no 16-pixel block row, so there is no far match."""
-    say("learned path", motion, W, H)
-    up, N = 1.5, 32
-    p = bm.derive_params(W, H, up, up, N=N)
+    one coarse row and no 16-pixel block row, so there is no far match; the
+    last case runs the reference's default anisotropic up-sampling (2.740 x
+    3.052, of_pipeline.cpp:178-181), where the coarse-to-matched offset scales
+    differ per axis."""
+    say("learned path", motion, W, H, ux, uy)
+    N = 32
+    p = bm.derive_params(W, H, ux, uy, N=N)
+    if (ux, uy) == (0.0, 0.0):
+        assert p.sx * H != p.sy * W  # different scale factors per axis
     p.iterations = 1
     ctx = bm.Context(p, seed=11)
     path = learned.DEFAULT_WEIGHTS
@@ -271,7 +277,9 @@ def test_learned_path_equals_composed_oracle(motion, W, H):
     eo, ec = oracle.propagate(oF1, oF2, -1, p.jfMax, True, eo, ec)
     eo, ec = oracle.random_search(oF1, oF2, 0, 6, 11, eo, ec)
     assert np.array_equal(off, eo) and np.array_equal(cost, ec)
-    oU, oV = oracle.downsample(eo, W, H, ctx.gauss_kernel())
+    kern = oracle.gauss_kernel(ctx.params.sigma)
+    assert np.array_equal(ctx.gauss_kernel(), kern)
+    oU, oV = oracle.downsample(eo, W, H, kern)
     oUm = oracle.median1d(oracle.median1d(oU, 1, p.fsize), 0, p.fsize)
     oVm = oracle.median1d(oracle.median1d(oV, 1, p.fsize), 0, p.fsize)
     tU, tV = oracle.trilateral(oU, oV, oUm, oVm, gs[0].astype(np.float32), p.sEPE ** 2, p.sI ** 2, p.sS ** 2)

@@ -118,6 +118,11 @@ def test_cli_parameter_errors(tmp_path):
         (["-i", os.path.join(GOLD, "frame%s.png"), "-f", "10:11"], "Frame pattern"),
         (["-i", os.path.join(GOLD, "f%02d_%d.png"), "-f", "10:11"], "Frame pattern"),
         (["-i", frames, "-f", "10:11", "-o", "out%n.flo"], "Frame pattern"),
+        # the frame number is an unsigned int: long and long long conversions
+        # would read an argument that was never passed
+        (["-i", os.path.join(GOLD, "frame%02lu.png"), "-f", "10:11"], "Frame pattern"),
+        (["-i", os.path.join(GOLD, "frame%02lld.png"), "-f", "10:11"], "Frame pattern"),
+        (["-i", os.path.join(GOLD, "frame%02hhhd.png"), "-f", "10:11"], "Frame pattern"),
     ]
     for args, msg in cases:
         r = _cli(*args)

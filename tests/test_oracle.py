@@ -233,6 +233,112 @@ def test_init_cost_is_hamming():
     assert np.array_equal(cost, _popcount(F1 ^ F2))
 
 
+def _ref_accepts(a, b, N, words, div_ulps=0):
+    """The reference's candidate test (featDist + insert, briefmatch_cuda.cu:60-80,
+    131-144) in float32: current distance D = b / N, early exit once a partial
+    Hamming sum exceeds N * D, accept iff a / N < D.  `words` are the per-word
+    popcounts summing to a (the order the partial sums arrive in); div_ulps
+    moves every quotient by that many ulps (fast-math division is within 2 ulps),
+    a < 0 towards zero for the candidate and away for D, > 0 the opposite.
+    Equal costs keep equal quotients: the division is a function of its inputs."""
+    f32 = np.float32
+    D = f32(b) / f32(N)
+    q = f32(a) / f32(N)
+    for _ in range(abs(div_ulps) if a != b else 0):
+        q = np.nextafter(q, f32(np.inf) if div_ulps > 0 else f32(0))
+        D = np.nextafter(D, f32(0) if div_ulps > 0 else f32(np.inf))
+    ddmax = f32(N) * D
+    dist = 0
+    for w in words:
+        dist += int(w)
+        if f32(dist) > ddmax:
+            return False
+    return bool(q < D)
+
+
+def test_float_distance_rule_equals_integer_strict_compare():
+    """The integer rule of this build (accept a candidate of Hamming cost a iff
+    a < b, the current cost) accepts exactly what the reference's float rule
+    accepts, for every descriptor length N in [2, 512] and every pair of costs
+    0 <= a, b <= N, with the partial sums arriving in the worst order for the
+    early exit (all of a in the first word) and spread over words, and with
+    each quotient moved by up to 2 ulps against the comparison (fast-math
+    division)."""
+    f32 = np.float32
+    for N in range(2, 513):
+        a = np.arange(N + 1)
+        b = np.arange(N + 1)
+        A, B = np.meshgrid(a, b, indexing="ij")
+        D = b.astype(f32) / f32(N)
+        Q = a.astype(f32) / f32(N)
+        ddmax = f32(N) * D
+        # early exit never rejects a cheaper candidate: every partial sum of a
+        # cost a < b is at most a, and a <= N * D
+        cheaper = A < B
+        assert (A[cheaper].astype(f32) <= ddmax[B[cheaper]]).all(), N
+        # the quotients order exactly like the integers, even 2 ulps apart
+        lo, hi = Q.copy(), D.copy()
+        for _ in range(2):
+            lo = np.nextafter(lo, f32(np.inf))   # candidate quotient rounded up
+            hi = np.nextafter(hi, f32(0))        # current quotient rounded down
+        assert (lo[:-1] < hi[1:]).all(), N      # a = b - 1 still accepted
+        acc = (Q[A] < D[B]) & (A.astype(f32) <= ddmax[B])
+        assert np.array_equal(acc, cheaper), N  # a == b and a > b rejected
+    # spot checks through the sequential form with spread partial sums
+    rng = np.random.default_rng(6)
+    for N in (2, 3, 31, 32, 33, 100, 128, 255, 512):
+        NS = (N + 31) // 32
+        for _ in range(200):
+            a, b = int(rng.integers(0, N + 1)), int(rng.integers(0, N + 1))
+            cuts = np.sort(rng.integers(0, a + 1, NS - 1)) if NS > 1 else np.array([], int)
+            words = np.diff(np.concatenate([[0], cuts, [a]]))
+            for ulps in (0, -2, 2):
+                assert _ref_accepts(a, b, N, words, ulps) == (a < b), (N, a, b, ulps)
+
+
+def test_level_selection_under_the_float_rule():
+    """One oracle propagation level on random descriptors, replayed pixel by
+    pixel with the reference's float distance and early exit (sequential
+    insert over the four candidates of the level-start snapshot, equal offsets
+    skipped): every pixel ends with the oracle's offset and cost."""
+    sy, sx, N = 20, 24, 72
+    NS = (N + 31) // 32
+    rng = np.random.default_rng(8)
+    F1 = _rand_desc(sy, sx, NS, 20)
+    F2 = _rand_desc(sy, sx, NS, 21)
+    last = (1 << (N - 32 * (NS - 1))) - 1
+    F1[..., -1] &= np.uint32(last)
+    F2[..., -1] &= np.uint32(last)
+    xx, yy = np.meshgrid(np.arange(sx), np.arange(sy))
+    dx = np.clip(rng.integers(-6, 7, (sy, sx)), -xx, sx - 1 - xx).astype(np.int16)
+    dy = np.clip(rng.integers(-6, 7, (sy, sx)), -yy, sy - 1 - yy).astype(np.int16)
+    off0 = _pack(dx, dy)
+    cost0 = _popcount(F1 ^ F2[yy + dy, xx + dx]).astype(np.uint16)
+    f32 = np.float32
+    for p, pd in ((4, -1), (1, 1), (8, 1)):
+        eo, ec = oracle.propagate_level(F1, F2, p, pd, True, off0, cost0)
+        edx, edy = _unpack(eo)
+        for y in range(sy):
+            for x in range(sx):
+                idx = y * sx + x
+                bx, by, D = int(dx[y, x]), int(dy[y, x]), f32(int(cost0[y, x])) / f32(N)
+                best = int(cost0[y, x])
+                for d in (-1, 1):
+                    for xy in (0, 1):
+                        n = idx + pd * p * d * (1 if xy else sx)
+                        if n < 0 or n >= sx * sy:
+                            continue
+                        nx = min(max(int(dx.flat[n]), -x), sx - 1 - x)
+                        ny = min(max(int(dy.flat[n]), -y), sy - 1 - y)
+                        if (nx, ny) == (bx, by):
+                            continue
+                        words = [bin(int(v)).count("1") for v in F1[y, x] ^ F2[y + ny, x + nx]]
+                        a = sum(words)
+                        if _ref_accepts(a, best, N, words) and f32(a) / f32(N) < D:
+                            bx, by, D, best = nx, ny, f32(a) / f32(N), a
+                assert (edx[y, x], edy[y, x], ec[y, x]) == (bx, by, best), (p, pd, x, y)
+
+
 def test_temporal_init_follows_previous_field():
     sy, sx = 6, 8
     F1, F2 = _rand_desc(sy, sx, 1, 2), _rand_desc(sy, sx, 1, 3)
