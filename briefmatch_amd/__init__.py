@@ -108,6 +108,7 @@ def lib():
         "bm_trilateral": (I, [V, V, V, V, V, V, V, V]),
         "bm_refine": (I, [V, V, V, V, V, V]),
         "bm_flow_pair": (I, [V, V, V, U32, V, V]),
+        "bm_flow_pair_static": (I, [C.POINTER(Params), U64, V, V, U32, V, V, V, V, V]),
         "bm_stream_frame": (I, [V, V, V, V, C.POINTER(I)]),
         "bm_stream_reset": (I, [V]),
         "bm_set_pipelining": (I, [V, I]),
@@ -188,6 +189,18 @@ def derive_params(W, H, ux=0.0, uy=0.0, patch_radius=3.0, sigma_ds=0.025, median
     for k, v in overrides.items():
         setattr(p, k, v)
     return p
+
+
+STATIC_SCRATCH_BYTES = 64 * 1024 * 1024  # BM_STATIC_SCRATCH_BYTES
+
+
+def flow_pair_static(params, seed, rgb1, rgb2, U, V, pair=0, off=None, cost=None, stream=None):
+    """bm_flow_pair_static: one isolated pair with no device allocation (its
+    buffers live in the library's static device array); device buffers in and
+    out, queued on `stream` (a hipStream_t as int, None = the null stream)."""
+    _check(lib().bm_flow_pair_static(C.byref(params), seed, _ptr(rgb1), _ptr(rgb2), pair, _ptr(U), _ptr(V),
+                                     _ptr(off) if off is not None else None,
+                                     _ptr(cost) if cost is not None else None, stream))
 
 
 def philox4x32_10(ctr, key):

@@ -1,6 +1,7 @@
 // C-ABI of the BriefMatch flow engine: parameters, context (tables, device
-// buffers, stream), stage entry points, the whole-pair path and sequence mode.
-// Host code only; the kernels are in bm_kernels.hip.
+// buffers, stream), stage entry points, the whole-pair path and sequence mode,
+// and the pair path without device allocation (its one device array is the
+// only device code here; the kernels are in bm_kernels.hip).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -8,9 +9,11 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <new>
 #include <sstream>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "bm_internal.h"
@@ -525,6 +528,165 @@ int run_flow_tail(bm_ctx *ctx, const float *guide, float *U, float *V)
     return run_refine(ctx, ctx->d_U0, ctx->d_V0, guide, U, V);
 }
 
+// Host-side coordinate and filter tables of a configuration (uploaded to the
+// device by the context or the no-allocation entry point).
+struct HostTables {
+    std::vector<uint32_t> upx, upy, nnx, nny;
+    std::vector<float> spt;
+};
+
+// The limits of BriefMatch::setup (briefmatch.cpp:78-96) and of this build's
+// 32-bit indexing and int16 offsets.
+int validate_params(const bm_params &p)
+{
+    char m[512];
+    if (p.N < 2 || p.N > BM_MAX_FEATURE_BITS) {
+        std::snprintf(m, sizeof m, "Feature length exceeds maximum value. Current size is %u and maximum is %d.", p.N,
+                      BM_MAX_FEATURE_BITS);
+        return fail(nullptr, BM_E_PARAM, m);
+    }
+    if (p.fsize > BM_MAX_MEDIAN) {
+        std::snprintf(m, sizeof m, "Median filter size exceeds maximum value. Current size is %u and maximum is %d.",
+                      p.fsize, BM_MAX_MEDIAN);
+        return fail(nullptr, BM_E_PARAM, m);
+    }
+    if (p.filter && (p.fsize % 2) == 0)
+        return fail(nullptr, BM_E_PARAM, "Median filter size must be odd.");
+    if (std::ceil(p.sigma * 3.0f) * 2.0f + 1.0f > BM_MAX_GAUSS_TAPS) {
+        std::snprintf(m, sizeof m,
+                      "Maximum size of Gaussian down-sampling filter exceeded. Currently using sigma = %0.4f and "
+                      "maximum is %0.4f.",
+                      p.sigma, std::floor((BM_MAX_GAUSS_TAPS - 1.0f) / 2.0f) / 3.0f);
+        return fail(nullptr, BM_E_PARAM, m);
+    }
+    if (p.sx_in == 0 || p.sy_in == 0 || p.sx == 0 || p.sy == 0)
+        return fail(nullptr, BM_E_PARAM, "empty resolution");
+    if (p.sx > 32767 || p.sy > 32767)
+        return fail(nullptr, BM_E_PARAM, "matched resolution above 32767 (offsets are int16)");
+    if (p.sigma <= 0.0f) return fail(nullptr, BM_E_PARAM, "down-sampling sigma must be positive");
+    if (p.filter && (p.sEPE <= 0.0f || p.sI <= 0.0f || p.sS <= 0.0f))
+        return fail(nullptr, BM_E_PARAM, "trilateral sigmas must be positive");
+    const size_t M = (size_t)p.sx * p.sy;
+    const unsigned NS = (p.N + 31) / 32, NSP = next_pow2(NS);
+    if ((double)M * NSP >= 4294967296.0 || (double)p.jfMax * p.sx + (double)M >= 2147483647.0)
+        return fail(nullptr, BM_E_PARAM, "matched resolution too large for 32-bit indexing");
+
+    return BM_OK;
+}
+
+// Everything of a context that lives on the host: parameters, geometry,
+// tuning defaults, Gaussian, pattern and fused-BRIEF plan, and the tables in t.
+int prepare_host(bm_ctx *ctx, const bm_params &p, uint64_t seed, HostTables &t)
+{
+    const unsigned NS = (p.N + 31) / 32, NSP = next_pow2(NS);
+    ctx->p = p;
+    bm::Geometry &g = ctx->g;
+    g.W = (int)p.sx_in;
+    g.H = (int)p.sy_in;
+    g.sx = (int)p.sx;
+    g.sy = (int)p.sy;
+    g.N = (int)p.N;
+    g.NS = (int)NS;
+    g.NSP = (int)NSP;
+    g.fsize = (int)p.fsize;
+    g.jfMax = (int)p.jfMax;
+    g.iterations = (int)p.iterations;
+    g.doubleProp = p.doubleProp ? 1 : 0;
+    g.seed = seed;
+    g.level_px = NSP <= 4 ? 2 : 1;
+    g.strip_rows = 256;
+    const float rs_sigma = std::sqrt(std::ceil((float)p.sx * 0.5f / 3.0f));
+    g.rs_sigma_q16 = (int64_t)std::llround((double)rs_sigma * 65536.0);
+
+    make_gauss(p.sigma, ctx->gauss);
+    g.P = (int)ctx->gauss.size();
+    ctx->p.P = (unsigned)g.P;
+    std::vector<int16_t> pat;
+    make_pattern(seed, p.N, p.patchArea, pat);
+    set_pattern_internal(ctx, pat);
+
+    // trilateral constants (sigmas squared as BriefMatch::process passes them)
+    const float sE = p.sEPE * p.sEPE, sI = p.sI * p.sI, sS = p.sS * p.sS;
+    g.tri_sigma_epe = sE;
+    g.tri_sigma_i = sI;
+    g.tri_sigma_s = sS;
+    const double log2e = 1.4426950408889634;
+    g.tri_s = p.filter ? (int)std::ceil(2.5 * (double)std::sqrt(sS)) : 0;
+    g.tri_a2 = p.filter ? (float)(log2e / (2.0 * (double)sE)) : 1.0f;
+    g.tri_b = p.filter ? (float)(std::sqrt(log2e / (2.0 * (double)sI)) / 255.0) : 1.0f;
+    if (p.filter && (g.tri_s < 1 || g.tri_s > 2000))
+        return fail(nullptr, BM_E_PARAM, "trilateral window outside [1, 2000]");
+    g.tri_band = p.filter ? bm::tri_max_band(g.tri_s) : 1;
+    // the MFMA kernel's 64 x 8 tiles fill the device where the VALU kernel's
+    // 64 x 16 tiles cannot (measured: 2x faster at 320x240, 75 tiles; 3 % at
+    // 1280x720, 900 tiles; 2 % slower at 1920x1080, 2040 tiles)
+    g.tri_groups = ((p.sx_in + 63) / 64) * ((p.sy_in + 15) / 16) <= 1024 ? 2 : 0;
+    std::vector<float> &spt = t.spt;
+    spt.assign(std::max(2 * g.tri_s, 1), 0.0f);
+    for (int m_ = 0; m_ < 2 * g.tri_s; ++m_) {
+        const double i = m_ - g.tri_s;
+        spt[m_] = (float)(log2e * i * i / (2.0 * (double)sS));
+    }
+
+    // coordinate tables
+    std::vector<uint32_t> &upx = t.upx, &upy = t.upy, &nnx = t.nnx, &nny = t.nny;
+    upx.assign(p.sx, 0u);
+    upy.assign(p.sy, 0u);
+    nnx.assign(p.sx_in, 0u);
+    nny.assign(p.sy_in, 0u);
+    for (unsigned x = 0; x < p.sx; ++x) upx[x] = (uint32_t)((uint64_t)x * p.sx_in * 256u / p.sx);
+    for (unsigned y = 0; y < p.sy; ++y) upy[y] = (uint32_t)((uint64_t)y * p.sy_in * 256u / p.sy);
+    for (unsigned x = 0; x < p.sx_in; ++x)
+        nnx[x] = (uint32_t)std::min<uint64_t>((2ull * x * p.sx + p.sx_in) / (2ull * p.sx_in), p.sx - 1);
+    for (unsigned y = 0; y < p.sy_in; ++y)
+        nny[y] = (uint32_t)std::min<uint64_t>((2ull * y * p.sy + p.sy_in) / (2ull * p.sy_in), p.sy - 1);
+    // largest field window of a down-sampling output tile (NN samples +- P/2)
+    {
+        unsigned cols = 0, rows = 0;
+        for (unsigned x0 = 0; x0 < p.sx_in; x0 += bm::kDsTileW)
+            cols = std::max(cols, nnx[std::min(x0 + bm::kDsTileW, p.sx_in) - 1] - nnx[x0] + 1);
+        for (unsigned y0 = 0; y0 < p.sy_in; y0 += bm::kDsTileH)
+            rows = std::max(rows, nny[std::min(y0 + bm::kDsTileH, p.sy_in) - 1] - nny[y0] + 1);
+        const size_t bytes = (size_t)(cols + g.P - 1) * (rows + g.P - 1) * sizeof(uint32_t);
+        g.ds_lds = bytes <= (size_t)bm::kDsLdsMax ? (int)bytes : 0;
+    }
+
+    return BM_OK;
+}
+
+// Device table pointers of the geometry, from the context's buffers.
+void bind_tables(bm_ctx *ctx)
+{
+    bm::Geometry &g = ctx->g;
+    g.pattern = ctx->d_pattern;
+    g.brief_plan = ctx->d_plan;
+    g.brief_plan_count = ctx->d_plan_count;
+    g.up_x = ctx->d_up_x;
+    g.up_y = ctx->d_up_y;
+    g.nn_x = ctx->d_nn_x;
+    g.nn_y = ctx->d_nn_y;
+    g.gauss = ctx->d_gauss;
+    g.tri_spatial = ctx->d_spt;
+}
+
+
+// One isolated pair on the context's buffers and stream (bm_flow_pair).
+int run_pair(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, uint32_t pair, float *U, float *V)
+{
+    const bm::Geometry &g = ctx->g;
+    BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb1, ctx->d_g8[0], ctx->d_g32[0], ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb2, ctx->d_g8[1], nullptr, ctx->stream));
+    int r = describe(ctx, ctx->d_g8[0], ctx->d_F[0]);
+    if (!r) r = describe(ctx, ctx->d_g8[1], ctx->d_F[1]);
+    if (r) return r;
+    r = run_match(ctx, ctx->d_F[0], ctx->d_F[1], 1, nullptr, pair);
+    if (r) return r;
+    // an isolated pair breaks any running sequence
+    ctx->have_prev = 0;
+    ctx->field_valid = 0;
+    return run_flow_tail(ctx, ctx->d_g32[0], U, V);
+}
+
 }  // namespace
 
 extern "C" {
@@ -586,109 +748,18 @@ int bm_ctx_create(const bm_params *params, uint64_t seed, bm_ctx **out)
 {
     if (!params || !out) return fail(nullptr, BM_E_PARAM, "null argument");
     *out = nullptr;
-    const bm_params &p = *params;
-    char m[512];
-    if (p.N < 2 || p.N > BM_MAX_FEATURE_BITS) {
-        std::snprintf(m, sizeof m, "Feature length exceeds maximum value. Current size is %u and maximum is %d.", p.N,
-                      BM_MAX_FEATURE_BITS);
-        return fail(nullptr, BM_E_PARAM, m);
-    }
-    if (p.fsize > BM_MAX_MEDIAN) {
-        std::snprintf(m, sizeof m, "Median filter size exceeds maximum value. Current size is %u and maximum is %d.",
-                      p.fsize, BM_MAX_MEDIAN);
-        return fail(nullptr, BM_E_PARAM, m);
-    }
-    if (p.filter && (p.fsize % 2) == 0)
-        return fail(nullptr, BM_E_PARAM, "Median filter size must be odd.");
-    if (std::ceil(p.sigma * 3.0f) * 2.0f + 1.0f > BM_MAX_GAUSS_TAPS) {
-        std::snprintf(m, sizeof m,
-                      "Maximum size of Gaussian down-sampling filter exceeded. Currently using sigma = %0.4f and "
-                      "maximum is %0.4f.",
-                      p.sigma, std::floor((BM_MAX_GAUSS_TAPS - 1.0f) / 2.0f) / 3.0f);
-        return fail(nullptr, BM_E_PARAM, m);
-    }
-    if (p.sx_in == 0 || p.sy_in == 0 || p.sx == 0 || p.sy == 0)
-        return fail(nullptr, BM_E_PARAM, "empty resolution");
-    if (p.sx > 32767 || p.sy > 32767)
-        return fail(nullptr, BM_E_PARAM, "matched resolution above 32767 (offsets are int16)");
-    if (p.sigma <= 0.0f) return fail(nullptr, BM_E_PARAM, "down-sampling sigma must be positive");
-    if (p.filter && (p.sEPE <= 0.0f || p.sI <= 0.0f || p.sS <= 0.0f))
-        return fail(nullptr, BM_E_PARAM, "trilateral sigmas must be positive");
-    const size_t M = (size_t)p.sx * p.sy;
-    const unsigned NS = (p.N + 31) / 32, NSP = next_pow2(NS);
-    if ((double)M * NSP >= 4294967296.0 || (double)p.jfMax * p.sx + (double)M >= 2147483647.0)
-        return fail(nullptr, BM_E_PARAM, "matched resolution too large for 32-bit indexing");
-
+    int r = validate_params(*params);
+    if (r) return r;
     bm_ctx *ctx = new (std::nothrow) bm_ctx;
     if (!ctx) return fail(nullptr, BM_E_ALLOC, "out of host memory");
-    ctx->p = p;
-    bm::Geometry &g = ctx->g;
-    g.W = (int)p.sx_in;
-    g.H = (int)p.sy_in;
-    g.sx = (int)p.sx;
-    g.sy = (int)p.sy;
-    g.N = (int)p.N;
-    g.NS = (int)NS;
-    g.NSP = (int)NSP;
-    g.fsize = (int)p.fsize;
-    g.jfMax = (int)p.jfMax;
-    g.iterations = (int)p.iterations;
-    g.doubleProp = p.doubleProp ? 1 : 0;
-    g.seed = seed;
-    g.level_px = NSP <= 4 ? 2 : 1;
-    g.strip_rows = 256;
-    const float rs_sigma = std::sqrt(std::ceil((float)p.sx * 0.5f / 3.0f));
-    g.rs_sigma_q16 = (int64_t)std::llround((double)rs_sigma * 65536.0);
-
-    make_gauss(p.sigma, ctx->gauss);
-    g.P = (int)ctx->gauss.size();
-    ctx->p.P = (unsigned)g.P;
-    std::vector<int16_t> pat;
-    make_pattern(seed, p.N, p.patchArea, pat);
-    set_pattern_internal(ctx, pat);
-
-    // trilateral constants (sigmas squared as BriefMatch::process passes them)
-    const float sE = p.sEPE * p.sEPE, sI = p.sI * p.sI, sS = p.sS * p.sS;
-    g.tri_sigma_epe = sE;
-    g.tri_sigma_i = sI;
-    g.tri_sigma_s = sS;
-    const double log2e = 1.4426950408889634;
-    g.tri_s = p.filter ? (int)std::ceil(2.5 * (double)std::sqrt(sS)) : 0;
-    g.tri_a2 = p.filter ? (float)(log2e / (2.0 * (double)sE)) : 1.0f;
-    g.tri_b = p.filter ? (float)(std::sqrt(log2e / (2.0 * (double)sI)) / 255.0) : 1.0f;
-    if (p.filter && (g.tri_s < 1 || g.tri_s > 2000)) {
+    HostTables t;
+    r = prepare_host(ctx, *params, seed, t);
+    if (r) {
         delete ctx;
-        return fail(nullptr, BM_E_PARAM, "trilateral window outside [1, 2000]");
+        return r;
     }
-    g.tri_band = p.filter ? bm::tri_max_band(g.tri_s) : 1;
-    // the MFMA kernel's 64 x 8 tiles fill the device where the VALU kernel's
-    // 64 x 16 tiles cannot (measured: 2x faster at 320x240, 75 tiles; 3 % at
-    // 1280x720, 900 tiles; 2 % slower at 1920x1080, 2040 tiles)
-    g.tri_groups = ((p.sx_in + 63) / 64) * ((p.sy_in + 15) / 16) <= 1024 ? 2 : 0;
-    std::vector<float> spt(std::max(2 * g.tri_s, 1));
-    for (int m_ = 0; m_ < 2 * g.tri_s; ++m_) {
-        const double i = m_ - g.tri_s;
-        spt[m_] = (float)(log2e * i * i / (2.0 * (double)sS));
-    }
-
-    // coordinate tables
-    std::vector<uint32_t> upx(p.sx), upy(p.sy), nnx(p.sx_in), nny(p.sy_in);
-    for (unsigned x = 0; x < p.sx; ++x) upx[x] = (uint32_t)((uint64_t)x * p.sx_in * 256u / p.sx);
-    for (unsigned y = 0; y < p.sy; ++y) upy[y] = (uint32_t)((uint64_t)y * p.sy_in * 256u / p.sy);
-    for (unsigned x = 0; x < p.sx_in; ++x)
-        nnx[x] = (uint32_t)std::min<uint64_t>((2ull * x * p.sx + p.sx_in) / (2ull * p.sx_in), p.sx - 1);
-    for (unsigned y = 0; y < p.sy_in; ++y)
-        nny[y] = (uint32_t)std::min<uint64_t>((2ull * y * p.sy + p.sy_in) / (2ull * p.sy_in), p.sy - 1);
-    // largest field window of a down-sampling output tile (NN samples +- P/2)
-    {
-        unsigned cols = 0, rows = 0;
-        for (unsigned x0 = 0; x0 < p.sx_in; x0 += bm::kDsTileW)
-            cols = std::max(cols, nnx[std::min(x0 + bm::kDsTileW, p.sx_in) - 1] - nnx[x0] + 1);
-        for (unsigned y0 = 0; y0 < p.sy_in; y0 += bm::kDsTileH)
-            rows = std::max(rows, nny[std::min(y0 + bm::kDsTileH, p.sy_in) - 1] - nny[y0] + 1);
-        const size_t bytes = (size_t)(cols + g.P - 1) * (rows + g.P - 1) * sizeof(uint32_t);
-        g.ds_lds = bytes <= (size_t)bm::kDsLdsMax ? (int)bytes : 0;
-    }
+    const bm_params &p = ctx->p;
+    const bm::Geometry &g = ctx->g;
 
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
@@ -699,19 +770,18 @@ int bm_ctx_create(const bm_params *params, uint64_t seed, bm_ctx **out)
         return fail(nullptr, BM_E_LAUNCH, msg);
     }
 
-    const size_t Win = (size_t)p.sx_in * p.sy_in;
-    int r = BM_OK;
+    const size_t Win = (size_t)p.sx_in * p.sy_in, M = (size_t)p.sx * p.sy, NSP = (size_t)g.NSP;
 #define BM_ALLOC(ptr, n) \
     if (r == BM_OK) r = dev_alloc(ctx, &(ptr), (n))
-    BM_ALLOC(ctx->d_pattern, pat.size());
-    BM_ALLOC(ctx->d_plan, (size_t)NSP * 64 * 4);
-    BM_ALLOC(ctx->d_plan_count, (size_t)NSP * 2);
-    BM_ALLOC(ctx->d_up_x, upx.size());
-    BM_ALLOC(ctx->d_up_y, upy.size());
-    BM_ALLOC(ctx->d_nn_x, nnx.size());
-    BM_ALLOC(ctx->d_nn_y, nny.size());
+    BM_ALLOC(ctx->d_pattern, ctx->pattern.size());
+    BM_ALLOC(ctx->d_plan, NSP * 64 * 4);
+    BM_ALLOC(ctx->d_plan_count, NSP * 2);
+    BM_ALLOC(ctx->d_up_x, t.upx.size());
+    BM_ALLOC(ctx->d_up_y, t.upy.size());
+    BM_ALLOC(ctx->d_nn_x, t.nnx.size());
+    BM_ALLOC(ctx->d_nn_y, t.nny.size());
     BM_ALLOC(ctx->d_gauss, ctx->gauss.size());
-    BM_ALLOC(ctx->d_spt, spt.size());
+    BM_ALLOC(ctx->d_spt, t.spt.size());
     for (int k = 0; k < 2; ++k) {
         BM_ALLOC(ctx->d_g8[k], Win);
         BM_ALLOC(ctx->d_g32[k], Win);
@@ -727,26 +797,18 @@ int bm_ctx_create(const bm_params *params, uint64_t seed, bm_ctx **out)
     BM_ALLOC(ctx->d_Vm, Win);
 #undef BM_ALLOC
     if (r == BM_OK) r = upload_pattern(ctx);
-    if (r == BM_OK) r = upload(ctx, ctx->d_up_x, upx);
-    if (r == BM_OK) r = upload(ctx, ctx->d_up_y, upy);
-    if (r == BM_OK) r = upload(ctx, ctx->d_nn_x, nnx);
-    if (r == BM_OK) r = upload(ctx, ctx->d_nn_y, nny);
+    if (r == BM_OK) r = upload(ctx, ctx->d_up_x, t.upx);
+    if (r == BM_OK) r = upload(ctx, ctx->d_up_y, t.upy);
+    if (r == BM_OK) r = upload(ctx, ctx->d_nn_x, t.nnx);
+    if (r == BM_OK) r = upload(ctx, ctx->d_nn_y, t.nny);
     if (r == BM_OK) r = upload(ctx, ctx->d_gauss, ctx->gauss);
-    if (r == BM_OK) r = upload(ctx, ctx->d_spt, spt);
+    if (r == BM_OK) r = upload(ctx, ctx->d_spt, t.spt);
     if (r != BM_OK) {
         std::string msg = ctx->err;
         bm_ctx_destroy(ctx);
         return fail(nullptr, r, msg);
     }
-    g.pattern = ctx->d_pattern;
-    g.brief_plan = ctx->d_plan;
-    g.brief_plan_count = ctx->d_plan_count;
-    g.up_x = ctx->d_up_x;
-    g.up_y = ctx->d_up_y;
-    g.nn_x = ctx->d_nn_x;
-    g.nn_y = ctx->d_nn_y;
-    g.gauss = ctx->d_gauss;
-    g.tri_spatial = ctx->d_spt;
+    bind_tables(ctx);
     *out = ctx;
     return BM_OK;
 }
@@ -1082,18 +1144,7 @@ int bm_flow_pair(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, uint32_t
     if (!r) r = check(ctx, U, "U");
     if (!r) r = check(ctx, V, "V");
     if (r) return r;
-    const bm::Geometry &g = ctx->g;
-    BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb1, ctx->d_g8[0], ctx->d_g32[0], ctx->stream));
-    BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb2, ctx->d_g8[1], nullptr, ctx->stream));
-    r = describe(ctx, ctx->d_g8[0], ctx->d_F[0]);
-    if (!r) r = describe(ctx, ctx->d_g8[1], ctx->d_F[1]);
-    if (r) return r;
-    r = run_match(ctx, ctx->d_F[0], ctx->d_F[1], 1, nullptr, pair);
-    if (r) return r;
-    // an isolated pair breaks any running sequence
-    ctx->have_prev = 0;
-    ctx->field_valid = 0;
-    return run_flow_tail(ctx, ctx->d_g32[0], U, V);
+    return run_pair(ctx, rgb1, rgb2, pair, U, V);
 }
 
 int bm_proposal_dims(const bm_ctx *ctx, unsigned *cw, unsigned *ch)
@@ -1824,3 +1875,166 @@ int bm_batch_timer_stop(bm_batch *b, float *ms)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// bm_flow_pair_static: the pair path without device allocation.  Its tables
+// and buffers are carved out of one file-scope device array; each GPU holds
+// its own copy of the array (from the first call on that GPU until the process
+// ends), found with hipGetSymbolAddress once per device.  In host code the
+// array's name is only a stand-in for that address: every use goes through the
+// per-device pointer.
+
+static __device__ __align__(256) unsigned char scratch[BM_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kStaticMaxDevices = 64;
+
+struct StaticDevice {
+    unsigned char *base = nullptr;  // device address of `scratch` on this GPU, null until set up
+    bm_ctx ctx;                     // geometry and buffer pointers carved from base
+    HostTables tabs;                // host tables of the configuration on the device
+    bm_params key_p{};              // that configuration
+    uint64_t key_seed = 0;
+    bool have_key = false;
+};
+
+std::mutex g_static_mu;
+StaticDevice g_static[kStaticMaxDevices];
+// the last configuration whose scratch size was checked
+bm_params g_checked_p{};
+uint64_t g_checked_seed = 0;
+bool g_checked = false;
+
+template <class T>
+void carve_one(T **ptr, size_t count, unsigned char *base, size_t &at)
+{
+    at = (at + 255) & ~(size_t)255;
+    *ptr = base ? reinterpret_cast<T *>(base + at) : nullptr;
+    at += std::max<size_t>(count * sizeof(T), 1);
+}
+
+// Lays the context's tables and buffers out from `base` (each 256-byte
+// aligned); with base == nullptr it only measures.  Returns the bytes used.
+size_t carve_static(bm_ctx *ctx, const HostTables &t, unsigned char *base)
+{
+    size_t at = 0;
+    const bm::Geometry &g = ctx->g;
+    const size_t Win = (size_t)g.W * g.H, M = (size_t)g.sx * g.sy, NSP = (size_t)g.NSP;
+    carve_one(&ctx->d_pattern, ctx->pattern.size(), base, at);
+    carve_one(&ctx->d_plan, NSP * 64 * 4, base, at);
+    carve_one(&ctx->d_plan_count, NSP * 2, base, at);
+    carve_one(&ctx->d_up_x, t.upx.size(), base, at);
+    carve_one(&ctx->d_up_y, t.upy.size(), base, at);
+    carve_one(&ctx->d_nn_x, t.nnx.size(), base, at);
+    carve_one(&ctx->d_nn_y, t.nny.size(), base, at);
+    carve_one(&ctx->d_gauss, ctx->gauss.size(), base, at);
+    carve_one(&ctx->d_spt, t.spt.size(), base, at);
+    for (int k = 0; k < 2; ++k) {
+        carve_one(&ctx->d_g8[k], Win, base, at);
+        carve_one(&ctx->d_F[k], M * NSP, base, at);
+        carve_one(&ctx->d_off[k], M, base, at);
+        carve_one(&ctx->d_cost[k], M, base, at);
+    }
+    carve_one(&ctx->d_g32[0], Win, base, at);  // the guide; frame 2 is never needed in FP32
+    ctx->d_g32[1] = nullptr;
+    for (float **b : {&ctx->d_U0, &ctx->d_V0, &ctx->d_tU, &ctx->d_tV, &ctx->d_Um, &ctx->d_Vm})
+        carve_one(b, Win, base, at);
+    return at;
+}
+
+bool same_config(const bm_params &a, uint64_t sa, const bm_params &b, uint64_t sb)
+{
+    return sa == sb && std::memcmp(&a, &b, sizeof a) == 0;
+}
+
+}  // namespace
+
+extern "C" int bm_flow_pair_static(const bm_params *params, uint64_t seed, const uint8_t *rgb1, const uint8_t *rgb2,
+                                   uint32_t pair, float *U, float *V, uint32_t *off_out, uint16_t *cost_out,
+                                   void *stream)
+{
+    if (!params || !rgb1 || !rgb2 || !U || !V) return fail(nullptr, BM_E_PARAM, "null argument");
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    if (!g_checked || !same_config(g_checked_p, g_checked_seed, *params, seed)) {
+        // limits and scratch size on the host, before any device work
+        g_checked = false;
+        int r = validate_params(*params);
+        if (r) return r;
+        bm_ctx probe;
+        HostTables tabs;
+        r = prepare_host(&probe, *params, seed, tabs);
+        if (r) return r;
+        const size_t need = carve_static(&probe, tabs, nullptr);
+        if (need > (size_t)BM_STATIC_SCRATCH_BYTES) {
+            char m[320];
+            std::snprintf(m, sizeof m,
+                          "bm_flow_pair_static: %ux%u -> %ux%u with %u-bit descriptors needs %zu bytes of scratch, "
+                          "more than the static maximum of %u bytes (BM_STATIC_SCRATCH_BYTES)",
+                          params->sx_in, params->sy_in, params->sx, params->sy, params->N, need,
+                          (unsigned)BM_STATIC_SCRATCH_BYTES);
+            return fail(nullptr, BM_E_PARAM, m);
+        }
+        g_checked_p = *params;
+        g_checked_seed = seed;
+        g_checked = true;
+    }
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(nullptr, BM_E_LAUNCH, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kStaticMaxDevices) return fail(nullptr, BM_E_PARAM, "device index out of range");
+    StaticDevice &sd = g_static[dev];
+    if (!sd.base) {
+        // once per device, with that device current: the address of its copy
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(nullptr, BM_E_LAUNCH, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        sd.base = static_cast<unsigned char *>(ptr);
+    }
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    bm_ctx &ctx = sd.ctx;
+    if (!sd.have_key || !same_config(sd.key_p, sd.key_seed, *params, seed)) {
+        // a new configuration on this GPU: geometry and tables into its array,
+        // copied from host tables that stay alive until the copies are done
+        sd.have_key = false;
+        ctx = bm_ctx();
+        int r = prepare_host(&ctx, *params, seed, sd.tabs);
+        if (r) return r;
+        carve_static(&ctx, sd.tabs, sd.base);
+        bind_tables(&ctx);
+        e = hipSuccess;
+        auto put = [&](void *dst, const void *src, size_t bytes) {
+            if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+        };
+        put(ctx.d_pattern, ctx.pattern.data(), ctx.pattern.size() * sizeof(int16_t));
+        put(ctx.d_plan, ctx.plan.data(), ctx.plan.size() * sizeof(uint32_t));
+        put(ctx.d_plan_count, ctx.plan_count.data(), ctx.plan_count.size() * sizeof(int));
+        put(ctx.d_up_x, sd.tabs.upx.data(), sd.tabs.upx.size() * sizeof(uint32_t));
+        put(ctx.d_up_y, sd.tabs.upy.data(), sd.tabs.upy.size() * sizeof(uint32_t));
+        put(ctx.d_nn_x, sd.tabs.nnx.data(), sd.tabs.nnx.size() * sizeof(uint32_t));
+        put(ctx.d_nn_y, sd.tabs.nny.data(), sd.tabs.nny.size() * sizeof(uint32_t));
+        put(ctx.d_gauss, ctx.gauss.data(), ctx.gauss.size() * sizeof(float));
+        put(ctx.d_spt, sd.tabs.spt.data(), sd.tabs.spt.size() * sizeof(float));
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return fail(nullptr, BM_E_LAUNCH, std::string("table upload: ") + hipGetErrorString(e));
+        sd.key_p = *params;
+        sd.key_seed = seed;
+        sd.have_key = true;
+    }
+    ctx.stream = st;
+    ctx.field = 0;
+    ctx.err.clear();
+    int r = run_pair(&ctx, rgb1, rgb2, pair, U, V);
+    const size_t M = (size_t)ctx.g.sx * ctx.g.sy;
+    if (!r && off_out) {
+        e = hipMemcpyAsync(off_out, ctx.d_off[ctx.field], M * sizeof(uint32_t), hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) r = hip_fail(&ctx, e, "field copy");
+    }
+    if (!r && cost_out) {
+        e = hipMemcpyAsync(cost_out, ctx.d_cost[ctx.field], M * sizeof(uint16_t), hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) r = hip_fail(&ctx, e, "cost copy");
+    }
+    if (r) return fail(nullptr, r, ctx.err);
+    return BM_OK;
+}

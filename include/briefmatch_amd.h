@@ -204,6 +204,30 @@ int bm_stream_reset(bm_ctx *ctx);
  * them on the device. */
 int bm_set_pipelining(bm_ctx *ctx, int on);
 
+/* ---- the pair path without device allocation ----------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  bm_flow_pair for one isolated pair without a context:
+ * every table and intermediate buffer (pattern, BRIEF plan, coordinate tables,
+ * gray frames, descriptors, both match fields, refinement planes) is carved
+ * from one static device array of BM_STATIC_SCRATCH_BYTES that each GPU holds
+ * from the first call on it until the process ends; the call allocates no
+ * device memory.  Configurations whose buffers need more are rejected with
+ * BM_E_PARAM (640x480 at x2 with 128-bit descriptors fits; 1920x1080 at x2.9
+ * does not).  Bilinear up-sampling and default launch shapes; results are
+ * those of bm_flow_pair on a context created with the same parameters and
+ * seed.  Work is queued on `stream` (a hipStream_t, NULL = the null stream);
+ * off_dev / cost_dev (sx*sy entries each, either may be NULL) receive the
+ * match field.  A call with new parameters or seed uploads their tables and
+ * waits for `stream`.  All calls on one GPU share its array, which keeps the
+ * last call's contents (every buffer is written in full before it is read,
+ * so none needs clearing): calls that overlap on one GPU, on different streams
+ * or from several threads, race; consecutive calls on one stream are ordered. */
+#define BM_STATIC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int bm_flow_pair_static(const bm_params *params, uint64_t seed, const uint8_t *rgb1_dev, const uint8_t *rgb2_dev,
+                        uint32_t pair, float *U_dev, float *V_dev, uint32_t *off_dev, uint16_t *cost_dev,
+                        void *stream);
+
 /* ---- tuning ------------------------------------------------------------------ */
 
 /* Launch-shape choices of one context.  Every valid tuning computes identical
