@@ -1,6 +1,8 @@
 """Turns rocprofv3 CSV output into the short text summaries kept under profiles/.
 
     python -m briefmatch_amd.rocprof_summary stats  <kernel_stats.csv> <out.txt> --cmd "<command>"
+    python -m briefmatch_amd.rocprof_summary counters <counter_collection.csv> <kernel_stats.csv> <out.txt> --cmd ...
+    python -m briefmatch_amd.rocprof_summary memory <mem1.csv> <mem2.csv> <kernel_stats.csv> <out.txt> --cmd ...
     python -m briefmatch_amd.rocprof_summary traffic <fetch_counter_collection.csv> <write_counter_collection.csv>
            <copy_fetch.csv> <copy_write.csv> <copy_bytes> <out.txt> <traffic.json> --cmd "<commands>"
 
@@ -11,8 +13,11 @@ x 2.4 GHz issue peak of one MI355X).
 
 `traffic` scales each kernel's mean FETCH_SIZE and WRITE_SIZE (KiB per dispatch)
 by what the same counter reads for the plain copy kernel (k_copy) of known
-size, and writes the HBM bytes per launch per kernel to traffic.json, which
-bench.py reports as roofline.traffic.
+size, and merges the HBM bytes per launch of each stage of that workload into
+traffic.json under "workloads" -> workload -> stage (other workloads kept),
+which bench.py reports as roofline.traffic.  Profile a run whose launch shapes
+come from an autotuner cache (bench.py --tune-cache), so each stage runs one
+kernel variant.
 """
 import argparse
 import csv
@@ -86,6 +91,38 @@ def counters(path, stats_path, out, cmd):
         f.write("\n".join(lines) + "\n")
 
 
+def memory(mem1, mem2, stats_path, out, cmd):
+    """Cache and address-unit figures of each kernel from two counter passes:
+    l2_hit = TCC_HIT / (TCC_HIT + TCC_MISS); ta_busy = TA_TA_BUSY_sum / (256
+    TAs x GRBM_GUI_ACTIVE / 8 XCDs), the fraction of cycles the CUs'
+    texture-address units are busy; ta_cyc/vmem = TA busy cycles per vector
+    memory read instruction; wait = SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES;
+    vmem_rd/wave = SQ_INSTS_VMEM_RD / SQ_WAVES."""
+    acc = defaultdict(lambda: defaultdict(list))
+    for path in (mem1, mem2):
+        for r in csv.DictReader(open(path)):
+            acc[short_name(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    dur = {short_name(r["Name"]): float(r["AverageNs"]) / 1e3 for r in csv.DictReader(open(stats_path))}
+    lines = ["# rocprofv3 cache and address-unit counters, mean per dispatch", "# commands: " + cmd,
+             "# " + memory.__doc__.split("\n", 1)[1].replace("\n    ", "\n# ").strip(), "",
+             "%-28s %9s %8s %12s %8s %10s %6s %12s" % ("kernel", "dur_us", "l2_hit", "l2_requests", "ta_busy",
+                                                    "ta_cyc/vmem", "wait", "vmem_rd/wave")]
+    for k in sorted(acc, key=lambda k: -dur.get(k, 0.0)):
+        v = {n: sum(x) / len(x) for n, x in acc[k].items()}
+        if k.startswith("__amd") or "TCC_HIT_sum" not in v or "TA_TA_BUSY_sum" not in v:
+            continue
+        req = v["TCC_HIT_sum"] + v["TCC_MISS_sum"]
+        cyc = v["GRBM_GUI_ACTIVE"] / 8.0
+        lines.append("%-28s %9.1f %8.3f %12.4g %8.2f %10.1f %6.2f %12.1f" % (
+            k, dur.get(k, float("nan")), v["TCC_HIT_sum"] / req if req else 0.0, req,
+            v["TA_TA_BUSY_sum"] / (256 * cyc) if cyc else 0.0,
+            v["TA_TA_BUSY_sum"] / v["SQ_INSTS_VMEM_RD"] if v.get("SQ_INSTS_VMEM_RD") else 0.0,
+            v["SQ_WAIT_INST_ANY"] / v["SQ_WAVE_CYCLES"] if v.get("SQ_WAVE_CYCLES") else 0.0,
+            v["SQ_INSTS_VMEM_RD"] / v["SQ_WAVES"] if v.get("SQ_WAVES") else 0.0))
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def _mean_by_kernel(path):
     acc = defaultdict(list)
     for r in csv.DictReader(open(path)):
@@ -93,7 +130,7 @@ def _mean_by_kernel(path):
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
-def traffic(fetch, write, cfetch, cwrite, copy_bytes, out, out_json, cmd):
+def traffic(fetch, write, cfetch, cwrite, copy_bytes, out, out_json, cmd, workload):
     f = _mean_by_kernel(fetch)
     w = _mean_by_kernel(write)
     cf = _mean_by_kernel(cfetch)
@@ -107,7 +144,13 @@ def traffic(fetch, write, cfetch, cwrite, copy_bytes, out, out_json, cmd):
                  copy_bytes, cf["k_copy"], cw["k_copy"]),
              "#   scale factors fetch x%.4f, write x%.4f" % (sf, sw), "",
              "%-34s %14s %14s %14s" % ("kernel", "fetch_MB", "write_MB", "total_MB")]
-    tj = {"_calibration": {"copy_bytes": copy_bytes, "fetch_scale": sf, "write_scale": sw}}
+    try:
+        with open(out_json) as fo:
+            tj = json.load(fo)
+    except (OSError, ValueError):
+        tj = {}
+    tj = {"_calibration": tj.get("_calibration", {}), "workloads": tj.get("workloads", {})}
+    tj["_calibration"][workload] = {"copy_bytes": copy_bytes, "fetch_scale": sf, "write_scale": sw}
     per_stage = defaultdict(list)
     for k in f:
         if k == "k_copy" or k.startswith("__amd"):
@@ -115,8 +158,7 @@ def traffic(fetch, write, cfetch, cwrite, copy_bytes, out, out_json, cmd):
         fb, wb = f[k] * sf * 1024, w.get(k, 0.0) * sw * 1024
         lines.append("%-34s %14.1f %14.1f %14.1f" % (k, fb / 1e6, wb / 1e6, (fb + wb) / 1e6))
         per_stage[stage_of(k)].append(fb + wb)
-    for st, v in per_stage.items():
-        tj[st] = sum(v) / len(v)
+    tj["workloads"][workload] = {st: sum(v) / len(v) for st, v in per_stage.items()}
     with open(out, "w") as fo:
         fo.write("\n".join(lines) + "\n")
     with open(out_json, "w") as fo:
@@ -137,19 +179,28 @@ def main():
     b.add_argument("out")
     b.add_argument("out_json")
     b.add_argument("--cmd", required=True)
+    b.add_argument("--workload", required=True)
     c = sub.add_parser("counters")
     c.add_argument("csv")
     c.add_argument("stats_csv")
     c.add_argument("out")
     c.add_argument("--cmd", required=True)
+    d = sub.add_parser("memory")
+    d.add_argument("mem1_csv")
+    d.add_argument("mem2_csv")
+    d.add_argument("stats_csv")
+    d.add_argument("out")
+    d.add_argument("--cmd", required=True)
     args = ap.parse_args()
-    if args.mode == "stats":
+    if args.mode == "memory":
+        memory(args.mem1_csv, args.mem2_csv, args.stats_csv, args.out, args.cmd)
+    elif args.mode == "stats":
         stats(args.csv, args.out, args.cmd)
     elif args.mode == "counters":
         counters(args.csv, args.stats_csv, args.out, args.cmd)
     else:
         traffic(args.fetch, args.write, args.copy_fetch, args.copy_write, args.copy_bytes, args.out, args.out_json,
-                args.cmd)
+                args.cmd, args.workload)
 
 
 if __name__ == "__main__":

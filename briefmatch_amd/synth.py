@@ -210,3 +210,44 @@ WORKLOADS = {
 def workload_pair(name, seed=0):
     wl = WORKLOADS[name]
     return make_pair(wl["W"], wl["H"], seed=seed, **wl["gen"])
+
+
+def batch_specs(n, seed=0):
+    """The uneven-batch workload: n pairs with sizes uniform in [320x240,
+    1920x1080] (width and height drawn independently, so the aspect varies),
+    feature length 32 or 128 bits, equal up-sampling factors uniform in
+    [1.5, 2.9], rectangles moving up to 8 px at 320 wide and 40 px at 1920.
+    Returns one dict per pair: W, H, N, up, and the generator arguments."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        W, H = int(rng.integers(320, 1921)), int(rng.integers(240, 1081))
+        N = int(rng.choice([32, 128]))
+        u = float(rng.uniform(1.5, 2.9))
+        mm = 8.0 + 32.0 * (max(W, H) - 320) / 1600.0
+        out.append(dict(W=W, H=H, N=N, up=u,
+                        gen=dict(motion="rects", n_rects=3 + (i % 6), max_motion=mm, seed=1000 + i)))
+    return out
+
+
+def _batch_pair(spec):
+    return make_pair(spec["W"], spec["H"], **spec["gen"])
+
+
+def batch_pairs(specs, workers=None):
+    """The pairs of batch_specs, generated in a pool of processes (the pairs
+    are independent and each is fixed by its own seed).  The workers are
+    spawned, not forked, so a caller that already uses the GPU can call this."""
+    import concurrent.futures
+    import multiprocessing
+    import os
+    if workers is None:
+        workers = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 4)
+    if workers <= 1 or len(specs) <= 1:
+        return [_batch_pair(s) for s in specs]
+    # largest first so the pool's tail is short; results in spec order
+    order = sorted(range(len(specs)), key=lambda i: -specs[i]["W"] * specs[i]["H"])
+    ctx = multiprocessing.get_context("spawn")
+    with concurrent.futures.ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as pool:
+        done = dict(zip(order, pool.map(_batch_pair, [specs[i] for i in order])))
+    return [done[i] for i in range(len(specs))]

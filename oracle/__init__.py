@@ -62,12 +62,19 @@ def lib():
         _lib.orc_trilateral.argtypes = [V, V, V, V, V, C.c_int, C.c_int, C.c_float, C.c_float,
                                         C.c_float, C.c_int, C.c_int, V, V]
         _lib.orc_flow_pair.argtypes = [C.POINTER(Params), C.c_uint64, C.c_uint32, V, V, V, V, V, V]
+        _lib.orc_set_num_threads.argtypes = [C.c_int]
+        _lib.orc_set_num_threads.restype = None
         _lib.orc_epe.argtypes = [V, V, V, V, V, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     return _lib
 
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def set_num_threads(n):
+    """OpenMP threads of the oracle's parallel regions from now on."""
+    lib().orc_set_num_threads(int(n))
 
 
 def philox(ctr, key):

@@ -79,6 +79,16 @@ int orc_num_threads(void)
 #endif
 }
 
+/* Threads of the parallel regions that follow (timing at a chosen core count). */
+void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 /* Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw, SC'11):
  * ten rounds of two 32x32->64 multiplies with the published multipliers and
  * Weyl key increments. */

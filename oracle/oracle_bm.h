@@ -76,6 +76,7 @@ int  orc_flow_pair_up(const orc_params *p, uint64_t seed, uint32_t pair,
 void orc_epe(const float *U, const float *V, const float *gtU, const float *gtV,
              const uint8_t *occluded, int64_t n, double *epe_all, double *epe_noc);
 int  orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }
