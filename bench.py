@@ -345,9 +345,17 @@ def bench_batch(a, bm, n_pairs, specs=None, frames=None):
     Us = [bm.DeviceBuffer(p.sx_in * p.sy_in * 4) for p in params]
     Vs = [bm.DeviceBuffer(p.sx_in * p.sy_in * 4) for p in params]
     pix = sum(p.sx_in * p.sy_in for p in params)
-    modes = [("per_pair", bm.BATCH_PER_PAIR), ("graph", bm.BATCH_GRAPH)]
+    modes = [("per_pair", bm.BATCH_PER_PAIR), ("graph", bm.BATCH_GRAPH), ("multi", bm.BATCH_MULTI)]
     out, fields = {}, {}
+    t0 = time.perf_counter()
+    sched = batch.autotune(r1, r2, Us, Vs)
+    tune_s = time.perf_counter() - t0
+    modes.append(("autotuned", bm.BATCH_AUTO))
     for name, mode in modes:
+        if name in ("graph", "multi"):              # these with the default schedule
+            batch.set_schedule(mode=mode, branches=8, order=0, persist=0)
+        elif name == "autotuned":
+            batch.set_schedule(**sched.as_dict())
         for _ in range(max(1, a.warmup)):
             batch.flow(r1, r2, Us, Vs, mode=mode)
         batch.sync()
@@ -362,10 +370,11 @@ def bench_batch(a, bm, n_pairs, specs=None, frames=None):
     exact = {name: all(np.array_equal(x[0][0], y[0][0]) and np.array_equal(x[0][1], y[0][1]) and
                        np.array_equal(x[1], y[1]) for x, y in zip(ref, fields[name]))
              for name in fields if name != "per_pair"}
-    best = max(out, key=lambda k: out[k]["value"])
     batch.close()
+    best = "autotuned"
     return {"value": out[best]["value"], "unit": "Mpix/s", "ms_per_step": out[best]["ms_per_batch"],
-            "mode": best, "modes": out, "bit_exact_vs_per_pair": exact,
+            "mode": best, "schedule": sched.as_dict(), "autotune_seconds": round(tune_s, 2), "modes": out,
+            "bit_exact_vs_per_pair": exact,
             "config": {"workload": "batch%d" % n_pairs, "pairs": n_pairs, "input_mpix": round(pix / 1e6, 3),
                        "matched_mpix": round(sum(p.sx * p.sy for p in params) / 1e6, 3),
                        "feature_bits": sorted(set(int(p.N) for p in params)), "streams": 8,

@@ -43,6 +43,14 @@ class Tuning(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class BatchSchedule(C.Structure):
+    """bm_batch_schedule: the result-invariant schedule of a batch."""
+    _fields_ = [(n, C.c_int) for n in ("mode", "branches", "order", "persist")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class Params(C.Structure):
     """bm_params: the fields of BMParams (briefmatch.h:65-89)."""
     _fields_ = [(n, C.c_uint) for n in ("sx", "sy", "sx_in", "sy_in", "N", "iterations", "jfMax", "P",
@@ -129,6 +137,9 @@ def lib():
         "bm_batch_timer_start": (I, [V]),
         "bm_batch_timer_stop": (I, [V, C.POINTER(F)]),
         "bm_batch_last_error": (C.c_char_p, [V]),
+        "bm_batch_get_schedule": (I, [V, C.POINTER(BatchSchedule)]),
+        "bm_batch_set_schedule": (I, [V, C.POINTER(BatchSchedule)]),
+        "bm_batch_autotune": (I, [V, V, V, V, V, C.POINTER(BatchSchedule)]),
         "bm_net_create": (I, [C.POINTER(V)]),
         "bm_net_destroy": (None, [V]),
         "bm_net_last_error": (C.c_char_p, [V]),
@@ -533,13 +544,14 @@ class _BorrowedContext(Context):
         self.h = None
 
 
-BATCH_PER_PAIR, BATCH_GRAPH = 0, 1
+BATCH_PER_PAIR, BATCH_GRAPH, BATCH_MULTI, BATCH_AUTO = 0, 1, 2, 3
 UPSAMPLE_BILINEAR, UPSAMPLE_LANCZOS3 = 0, 1
 
 
 class Batch:
     """Pairs of uneven size / up-sampling / feature length run together
-    (bm_batch_*): per-pair launches or one captured hipGraph."""
+    (bm_batch_*): per-pair launches, one captured hipGraph of them, or one
+    launch per stage for all pairs (captured as a hipGraph)."""
 
     def __init__(self, params_list, seed=0, streams=8):
         arr = (Params * len(params_list))(*[p.copy() for p in params_list])
@@ -557,10 +569,28 @@ class Batch:
         if status != BM_OK:
             raise BriefMatchError(status, (lib().bm_batch_last_error(self.h) or b"").decode(errors="replace"))
 
-    def flow(self, rgb1, rgb2, U, V, mode=BATCH_GRAPH):
+    def _arrays(self, rgb1, rgb2, U, V):
         n = self.n
-        arrs = [(C.c_void_p * n)(*[_ptr(b) for b in lst]) for lst in (rgb1, rgb2, U, V)]
-        self._c(lib().bm_batch_flow(self.h, *arrs, mode))
+        return [(C.c_void_p * n)(*[_ptr(b) for b in lst]) for lst in (rgb1, rgb2, U, V)]
+
+    def flow(self, rgb1, rgb2, U, V, mode=BATCH_GRAPH):
+        self._c(lib().bm_batch_flow(self.h, *self._arrays(rgb1, rgb2, U, V), mode))
+
+    def schedule(self):
+        s = BatchSchedule()
+        self._c(lib().bm_batch_get_schedule(self.h, C.byref(s)))
+        return s
+
+    def set_schedule(self, **kw):
+        s = self.schedule()
+        for k, v in kw.items():
+            setattr(s, k, v)
+        self._c(lib().bm_batch_set_schedule(self.h, C.byref(s)))
+
+    def autotune(self, rgb1, rgb2, U, V):
+        s = BatchSchedule()
+        self._c(lib().bm_batch_autotune(self.h, *self._arrays(rgb1, rgb2, U, V), C.byref(s)))
+        return s
 
     def sync(self):
         self._c(lib().bm_batch_sync(self.h))

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1504,15 +1505,40 @@ int bm_epe(const float *U, const float *V, const float *gu, const float *gv, con
 // ---------------------------------------------------------------------------
 // tuning, autotuning, batches
 
+// One multi-pair launch of a batch's plan.
+struct MultiOp {
+    enum Kind { GRAY, BRIEF, INIT, LEVEL, DOWNSAMPLE, MEDIAN, TRILATERAL } kind;
+    int a = 0, b = 0;                     // template keys: (NSP, RN), (NSP, PX), FS, trilateral groups, tiled
+    int p = 0, pd = 0, src = 0, rs = -1;  // level / init arguments
+    int dir = 0;                          // median direction
+    size_t jobs = 0, start = 0;           // byte offsets of the job and tile-start arrays in the plan buffer
+    int njobs = 0, tiles = 0;
+    size_t lds = 0;
+};
+
 struct bm_batch {
     std::vector<bm_ctx *> ctx;
     std::vector<hipStream_t> streams;
     hipStream_t origin = nullptr;
     hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> joins;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    std::vector<uintptr_t> key;
+    bm_batch_schedule sched{BM_BATCH_GRAPH, 0, 0, 0};
+    int cus = 256;
+    // captured graph of the graph mode / of the multi-pair mode, with the key
+    // (buffers, configuration generations, schedule) each was captured for
+    hipGraph_t graph = nullptr, mgraph = nullptr;
+    hipGraphExec_t exec = nullptr, mexec = nullptr;
+    std::vector<uintptr_t> key, mkey;
+    // multi-pair plan: launches, their job arrays on the device, the field
+    // buffer each pair ends in, and the pairs that take the per-pair path
+    std::vector<MultiOp> ops;
+    void *plan_dev = nullptr;
+    std::vector<int> final_field, fallback;
+    std::vector<uintptr_t> pkey;
+    // the field buffer each pair's result lands in when the captured graph of
+    // the graph mode / the multi-pair mode replays (host state the replay
+    // itself does not update)
+    std::vector<int> gfield, mfield;
     std::string err;
 };
 
@@ -1712,6 +1738,9 @@ void bm_batch_destroy(bm_batch *b)
     if (b->origin) (void)hipStreamSynchronize(b->origin);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->mexec) (void)hipGraphExecDestroy(b->mexec);
+    if (b->mgraph) (void)hipGraphDestroy(b->mgraph);
+    if (b->plan_dev) (void)hipFree(b->plan_dev);
     for (bm_ctx *c : b->ctx) bm_ctx_destroy(c);
     for (hipEvent_t e : b->joins) (void)hipEventDestroy(e);
     for (hipEvent_t e : {b->fork, b->ev0, b->ev1})
@@ -1750,12 +1779,16 @@ int bm_batch_create(int n, const bm_params *params, uint64_t seed, int streams, 
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    int dev = 0;
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) {
         std::string m = std::string("HIP stream/event creation failed: ") + hipGetErrorString(e);
         bm_batch_destroy(b);
         return fail(nullptr, BM_E_LAUNCH, m);
     }
     for (bm_ctx *c : b->ctx) c->batch_stream = b->origin;
+    b->sched.branches = (int)b->streams.size();
     *out = b;
     return BM_OK;
 }
@@ -1767,22 +1800,58 @@ int bm_batch_ctx(bm_batch *b, int i, bm_ctx **ctx)
     return BM_OK;
 }
 
-static int batch_fail(bm_batch *b, int code, const std::string &m)
+}  // extern "C"
+
+namespace {
+
+int batch_fail(bm_batch *b, int code, const std::string &m)
 {
     b->err = m;
     return code;
 }
 
-// Enqueue every pair of the batch; pair i runs on streams[i % S] (or all on one stream).
-static int batch_enqueue(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r2, float *const *U,
-                         float *const *V, bool one_stream)
+// Work estimate of a pair for ordering: matched pixels (matching stages) plus
+// trilateral taps per input pixel over 64 (refinement).
+double pair_work(const bm_ctx *c)
+{
+    const double M = (double)c->g.sx * c->g.sy, Win = (double)c->g.W * c->g.H;
+    const double taps = c->p.filter ? 4.0 * c->g.tri_s * c->g.tri_s : 0.0;
+    return M * (1.0 + 0.25 * c->g.NSP) + Win * taps / 64.0;
+}
+
+// Pair indices in the schedule's order: index order, or largest first.
+std::vector<int> pair_order(const bm_batch *b)
+{
+    std::vector<int> idx(b->ctx.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+    if (b->sched.order == 1)
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](int x, int y) { return pair_work(b->ctx[x]) > pair_work(b->ctx[y]); });
+    return idx;
+}
+
+// Per-pair launches: pair i on branch branch[i] (all on the origin stream when one_stream).
+int batch_enqueue(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r2, float *const *U, float *const *V,
+                  bool one_stream)
 {
     const int n = (int)b->ctx.size();
+    const int S = std::max(1, std::min(b->sched.branches, (int)b->streams.size()));
+    std::vector<int> branch(n, 0);
+    if (b->sched.order == 1) {  // longest first onto the least loaded branch
+        std::vector<double> load(S, 0.0);
+        for (int i : pair_order(b)) {
+            const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            branch[i] = k;
+            load[k] += pair_work(b->ctx[i]);
+        }
+    } else {
+        for (int i = 0; i < n; ++i) branch[i] = i % S;
+    }
     for (int i = 0; i < n; ++i) {
         bm_ctx *c = b->ctx[i];
         hipStream_t own = c->stream;
         const bool prof = c->prof;
-        c->stream = one_stream ? b->origin : b->streams[i % b->streams.size()];
+        c->stream = one_stream ? b->origin : b->streams[branch[i]];
         c->prof = false;
         const int r = bm_flow_pair(c, r1[i], r2[i], (uint32_t)i, U[i], V[i]);
         c->stream = own;
@@ -1792,6 +1861,379 @@ static int batch_enqueue(bm_batch *b, const uint8_t *const *r1, const uint8_t *c
     return BM_OK;
 }
 
+// What a captured graph depends on: every pair's buffers and configuration
+// generation, and the schedule.
+std::vector<uintptr_t> batch_key(const bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r2,
+                                 float *const *U, float *const *V)
+{
+    std::vector<uintptr_t> key;
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        key.push_back((uintptr_t)r1[i]);
+        key.push_back((uintptr_t)r2[i]);
+        key.push_back((uintptr_t)U[i]);
+        key.push_back((uintptr_t)V[i]);
+        key.push_back((uintptr_t)b->ctx[i]->gen);
+    }
+    key.push_back((uintptr_t)b->sched.branches);
+    key.push_back((uintptr_t)b->sched.order);
+    key.push_back((uintptr_t)b->sched.persist);
+    return key;
+}
+
+// Host image of the plan buffer: job arrays and tile-start arrays, 16-byte aligned.
+struct PlanBytes {
+    std::vector<unsigned char> buf;
+    template <class T>
+    size_t put(const std::vector<T> &v)
+    {
+        const size_t at = (buf.size() + 15) & ~(size_t)15;
+        buf.resize(at + std::max<size_t>(v.size() * sizeof(T), 1));
+        if (!v.empty()) std::memcpy(buf.data() + at, v.data(), v.size() * sizeof(T));
+        return at;
+    }
+};
+
+// One launch over a group's jobs (job j has tiles_per_job[j] tiles); the job
+// array is stored at `jobs_at`, or appended when jobs_at is npos.
+template <class Job>
+MultiOp group_op(PlanBytes &pb, MultiOp::Kind kind, int a, int bk, const std::vector<Job> &jobs,
+                 const std::vector<int> &tiles_per_job, size_t lds, size_t jobs_at = std::string::npos)
+{
+    MultiOp op;
+    op.kind = kind;
+    op.a = a;
+    op.b = bk;
+    std::vector<int> start(tiles_per_job.size() + 1, 0);
+    for (size_t j = 0; j < tiles_per_job.size(); ++j) start[j + 1] = start[j] + tiles_per_job[j];
+    op.jobs = jobs_at == std::string::npos ? pb.put(jobs) : jobs_at;
+    op.start = pb.put(start);
+    op.njobs = (int)tiles_per_job.size();
+    op.tiles = start.back();
+    op.lds = lds;
+    return op;
+}
+
+// A pair runs in the multi-pair launches when its context uses the fused
+// bilinear BRIEF and the jump-flooding levels; otherwise it takes the
+// per-pair path (bm_flow_pair) inside the same sequence.
+bool multi_eligible(const bm_ctx *c)
+{
+    return c->upsampler == BM_UPSAMPLE_BILINEAR && c->g.brief_near > 0 && c->g.jfMax >= 1;
+}
+
+// The plan of the multi-pair mode for these buffers: every stage of every
+// eligible pair as one launch per kernel variant, the job arrays in one
+// device buffer (re-made only when a buffer, a configuration or the schedule changes).
+int build_plan(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r2, float *const *U, float *const *V)
+{
+    const int n = (int)b->ctx.size();
+    const std::vector<int> order = pair_order(b);
+    PlanBytes pb;
+    std::vector<MultiOp> ops;
+    b->fallback.clear();
+    b->final_field.assign(n, 0);
+    std::vector<int> pairs;
+    for (int i : order) {
+        if (multi_eligible(b->ctx[i]))
+            pairs.push_back(i);
+        else
+            b->fallback.push_back(i);
+    }
+    // gray: both frames of every pair
+    {
+        std::vector<bm::GrayJob> jobs;
+        std::vector<int> tiles;
+        for (int i : pairs) {
+            bm_ctx *c = b->ctx[i];
+            const int nW = c->g.W * c->g.H;
+            jobs.push_back({r1[i], c->d_g8[0], c->d_g32[0], nW});
+            jobs.push_back({r2[i], c->d_g8[1], nullptr, nW});
+            tiles.push_back(bm::gray_tiles(nW));
+            tiles.push_back(bm::gray_tiles(nW));
+        }
+        if (!jobs.empty()) ops.push_back(group_op(pb, MultiOp::GRAY, 0, 0, jobs, tiles, 0));
+    }
+    // BRIEF: per (descriptor stride, near halo), both frames
+    std::vector<std::pair<int, int>> bkeys;
+    for (int i : pairs) {
+        const std::pair<int, int> k(b->ctx[i]->g.NSP, b->ctx[i]->g.brief_near);
+        if (std::find(bkeys.begin(), bkeys.end(), k) == bkeys.end()) bkeys.push_back(k);
+    }
+    for (const auto &k : bkeys) {
+        std::vector<bm::BriefJob> jobs;
+        std::vector<int> tiles;
+        size_t lds = 0;
+        for (int i : pairs) {
+            bm_ctx *c = b->ctx[i];
+            const bm::Geometry &g = c->g;
+            if (g.NSP != k.first || g.brief_near != k.second) continue;
+            const int w = bm::brief_tile_rows(g.NSP) == 32 ? 1 : 0;
+            for (int f = 0; f < 2; ++f) {
+                jobs.push_back({c->d_g8[f], g.up_x, g.up_y, g.brief_plan, g.brief_plan_count, g.N, g.W, g.H, g.sx,
+                                g.sy, g.brief_far, g.brief_gw[w], g.brief_gh[w], c->d_F[f]});
+                tiles.push_back(bm::brief_tiles(g));
+            }
+            lds = std::max(lds, bm::brief_lds(g));
+        }
+        ops.push_back(group_op(pb, MultiOp::BRIEF, k.first, k.second, jobs, tiles, lds));
+    }
+    // matching: per (stride, pixels per thread, jump range, iterations)
+    std::vector<std::array<int, 4>> mkeys;
+    for (int i : pairs) {
+        const bm::Geometry &g = b->ctx[i]->g;
+        const std::array<int, 4> k{g.NSP, g.level_px, g.jfMax, g.iterations};
+        if (std::find(mkeys.begin(), mkeys.end(), k) == mkeys.end()) mkeys.push_back(k);
+    }
+    for (const auto &k : mkeys) {
+        std::vector<bm::MatchJob> jobs;
+        std::vector<int> tinit, tlev, members;
+        for (int i : pairs) {
+            bm_ctx *c = b->ctx[i];
+            const bm::Geometry &g = c->g;
+            if (g.NSP != k[0] || g.level_px != k[1] || g.jfMax != k[2] || g.iterations != k[3]) continue;
+            bm::MatchJob J{};
+            J.F1 = c->d_F[0];
+            J.F2 = c->d_F[1];
+            J.off[0] = c->d_off[0];
+            J.off[1] = c->d_off[1];
+            J.cost[0] = c->d_cost[0];
+            J.cost[1] = c->d_cost[1];
+            J.sx = g.sx;
+            J.sy = g.sy;
+            J.double_prop = g.doubleProp;
+            J.strip_rows = g.strip_rows;
+            J.pair = (uint32_t)i;
+            J.key0 = (uint32_t)g.seed;
+            J.key1 = (uint32_t)(g.seed >> 32);
+            J.sigma_q16 = g.rs_sigma_q16;
+            jobs.push_back(J);
+            tinit.push_back(bm::match_tiles(g, 0));
+            tlev.push_back(bm::match_tiles(g, g.level_px));
+            members.push_back(i);
+        }
+        // the initialisation carries the first iteration's random search (the
+        // levels of that iteration are the identity after a zero field)
+        MultiOp init = group_op(pb, MultiOp::INIT, k[0], 0, jobs, tinit, 0);
+        init.rs = k[3] > 0 ? 0 : -1;
+        ops.push_back(init);
+        // the same job array, with the level kernels' tiles
+        const MultiOp lev = group_op(pb, MultiOp::LEVEL, k[0], k[1], jobs, tlev, 0, init.jobs);
+        int src = 0, pd = -1;
+        for (int it = 0; it < k[3]; ++it) {
+            if (it > 0) {
+                for (int p = k[2]; p > 0; p /= 2) {
+                    MultiOp o = lev;
+                    o.p = p;
+                    o.pd = pd;
+                    o.src = src;
+                    o.rs = p == 1 ? it : -1;
+                    ops.push_back(o);
+                    src = 1 - src;
+                }
+            }
+            pd = -pd;
+        }
+        for (int i : members) b->final_field[i] = src;
+    }
+    // down-sampling: tiled (LDS window) or direct
+    for (int tiled = 1; tiled >= 0; --tiled) {
+        std::vector<bm::DsJob> jobs;
+        std::vector<int> tiles;
+        size_t lds = 0;
+        for (int i : pairs) {
+            bm_ctx *c = b->ctx[i];
+            const bm::Geometry &g = c->g;
+            const int identity = (g.sx == g.W && g.sy == g.H) ? 1 : 0;
+            const int t = (!identity && g.ds_lds > 0) ? 1 : 0;
+            if (t != tiled) continue;
+            float *u = c->p.filter ? c->d_U0 : U[i], *v = c->p.filter ? c->d_V0 : V[i];
+            jobs.push_back({c->d_off[b->final_field[i]], g.nn_x, g.nn_y, g.gauss, g.P, g.sx, g.sy, g.W, g.H,
+                            identity, (float)g.W / (float)g.sx, (float)g.H / (float)g.sy, u, v});
+            tiles.push_back(bm::downsample_tiles(g, t != 0));
+            lds = std::max(lds, (size_t)g.ds_lds);
+        }
+        if (!jobs.empty()) ops.push_back(group_op(pb, MultiOp::DOWNSAMPLE, tiled, 0, jobs, tiles, tiled ? lds : 0));
+    }
+    // medians (x then y) per window size, then the trilateral filter per kernel
+    std::vector<int> fss, tgs;
+    for (int i : pairs) {
+        const bm_ctx *c = b->ctx[i];
+        if (!c->p.filter) continue;
+        if (std::find(fss.begin(), fss.end(), c->g.fsize) == fss.end()) fss.push_back(c->g.fsize);
+        if (std::find(tgs.begin(), tgs.end(), c->g.tri_groups) == tgs.end()) tgs.push_back(c->g.tri_groups);
+    }
+    for (int dir = 1; dir >= 0; --dir) {
+        for (int fs : fss) {
+            std::vector<bm::MedJob> jobs;
+            std::vector<int> tiles;
+            for (int i : pairs) {
+                bm_ctx *c = b->ctx[i];
+                if (!c->p.filter || c->g.fsize != fs) continue;
+                if (dir)
+                    jobs.push_back({c->d_U0, c->d_V0, c->d_tU, c->d_tV, c->g.W, c->g.H});
+                else
+                    jobs.push_back({c->d_tU, c->d_tV, c->d_Um, c->d_Vm, c->g.W, c->g.H});
+                tiles.push_back(2 * bm::median_tiles(c->g));
+            }
+            MultiOp o = group_op(pb, MultiOp::MEDIAN, fs, 0, jobs, tiles, 0);
+            o.dir = dir;
+            ops.push_back(o);
+        }
+    }
+    for (int tg : tgs) {
+        std::vector<bm::TriJob> jobs;
+        std::vector<int> tiles;
+        size_t lds = 0;
+        for (int i : pairs) {
+            bm_ctx *c = b->ctx[i];
+            const bm::Geometry &g = c->g;
+            if (!c->p.filter || g.tri_groups != tg) continue;
+            int BR = 0;
+            const size_t l = bm::trilateral_lds(g, &BR);
+            if (BR < 1) return batch_fail(b, BM_E_PARAM, "pair " + std::to_string(i) + ": trilateral band too small");
+            jobs.push_back({c->d_U0, c->d_V0, c->d_Um, c->d_Vm, c->d_g32[0], U[i], V[i], g.tri_spatial, g.W, g.H,
+                            g.tri_s, BR, g.tri_a2, g.tri_b});
+            tiles.push_back(bm::trilateral_tiles(g, tg));
+            lds = std::max(lds, l);
+        }
+        ops.push_back(group_op(pb, MultiOp::TRILATERAL, tg, 0, jobs, tiles, lds));
+    }
+    // the job arrays on the device (the previous plan's buffer may still be read)
+    hipError_t e = hipStreamSynchronize(b->origin);
+    if (e == hipSuccess && b->plan_dev) {
+        e = hipFree(b->plan_dev);
+        b->plan_dev = nullptr;
+    }
+    if (e == hipSuccess) e = hipMalloc(&b->plan_dev, pb.buf.size());
+    if (e == hipSuccess) e = hipMemcpy(b->plan_dev, pb.buf.data(), pb.buf.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        b->plan_dev = nullptr;
+        (void)hipGetLastError();
+        return batch_fail(b, BM_E_ALLOC, std::string("batch plan: ") + hipGetErrorString(e));
+    }
+    b->ops.swap(ops);
+    return BM_OK;
+}
+
+int multi_grid(const bm_batch *b, int tiles)
+{
+    if (b->sched.persist <= 0) return tiles;
+    const int g = b->sched.persist * b->cus;
+    return g >= tiles ? tiles : std::max(8, g / 8 * 8);
+}
+
+// The plan's launches on the origin stream, and the per-pair path of the
+// pairs the plan does not cover.
+int run_plan(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r2, float *const *U, float *const *V)
+{
+    const hipStream_t st = b->origin;
+    const unsigned char *base = static_cast<const unsigned char *>(b->plan_dev);
+    for (const MultiOp &o : b->ops) {
+        const int *start = reinterpret_cast<const int *>(base + o.start);
+        const void *jobs = base + o.jobs;
+        const int grid = multi_grid(b, o.tiles);
+        hipError_t e = hipSuccess;
+        switch (o.kind) {
+        case MultiOp::GRAY:
+            e = bm::launch_multi_gray(static_cast<const bm::GrayJob *>(jobs), start, o.njobs, o.tiles, grid, st);
+            break;
+        case MultiOp::BRIEF:
+            e = bm::launch_multi_brief(o.a, o.b, static_cast<const bm::BriefJob *>(jobs), start, o.njobs, o.tiles,
+                                       grid, o.lds, st);
+            break;
+        case MultiOp::INIT:
+            e = bm::launch_multi_init(o.a, static_cast<const bm::MatchJob *>(jobs), start, o.njobs, o.tiles, grid,
+                                      o.rs, st);
+            break;
+        case MultiOp::LEVEL:
+            e = bm::launch_multi_level(o.a, o.b, static_cast<const bm::MatchJob *>(jobs), start, o.njobs, o.tiles,
+                                       grid, o.p, o.pd, o.src, o.rs, st);
+            break;
+        case MultiOp::DOWNSAMPLE:
+            e = bm::launch_multi_downsample(o.a != 0, static_cast<const bm::DsJob *>(jobs), start, o.njobs, o.tiles,
+                                            grid, o.lds, st);
+            break;
+        case MultiOp::MEDIAN:
+            e = bm::launch_multi_median(o.a, o.dir, static_cast<const bm::MedJob *>(jobs), start, o.njobs, o.tiles,
+                                        grid, st);
+            break;
+        case MultiOp::TRILATERAL:
+            e = bm::launch_multi_trilateral(o.a, static_cast<const bm::TriJob *>(jobs), start, o.njobs, o.tiles,
+                                            grid, o.lds, st);
+            break;
+        }
+        if (e != hipSuccess) return batch_fail(b, BM_E_LAUNCH, std::string("multi-pair launch: ") + hipGetErrorString(e));
+    }
+    for (int i : b->fallback) {
+        bm_ctx *c = b->ctx[i];
+        hipStream_t own = c->stream;
+        const bool prof = c->prof;
+        c->stream = st;
+        c->prof = false;
+        const int r = bm_flow_pair(c, r1[i], r2[i], (uint32_t)i, U[i], V[i]);
+        c->stream = own;
+        c->prof = prof;
+        if (r != BM_OK) return batch_fail(b, r, "pair " + std::to_string(i) + ": " + c->err);
+    }
+    return BM_OK;
+}
+
+// Captures `enqueue` on the origin stream (forked to the branch streams when
+// `branches`) into a graph, instantiated into *exec.
+template <class F>
+int capture(bm_batch *b, bool branches, hipGraph_t *graph, hipGraphExec_t *exec, F enqueue)
+{
+    if (*exec) (void)hipGraphExecDestroy(*exec);
+    if (*graph) (void)hipGraphDestroy(*graph);
+    *exec = nullptr;
+    *graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal);
+    if (branches) {
+        if (e == hipSuccess) e = hipEventRecord(b->fork, b->origin);
+        for (hipStream_t s : b->streams)
+            if (e == hipSuccess) e = hipStreamWaitEvent(s, b->fork, 0);
+    }
+    int r = BM_OK;
+    if (e == hipSuccess) r = enqueue();
+    if (branches) {
+        for (size_t k = 0; k < b->streams.size() && e == hipSuccess; ++k) {
+            e = hipEventRecord(b->joins[k], b->streams[k]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(b->origin, b->joins[k], 0);
+        }
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(b->origin, &g);
+    if (e == hipSuccess) e = e2;
+    if (r != BM_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return r != BM_OK ? r : batch_fail(b, BM_E_LAUNCH, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        *exec = nullptr;
+        return batch_fail(b, BM_E_LAUNCH, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    *graph = g;
+    return BM_OK;
+}
+
+int schedule_valid(const bm_batch *b, const bm_batch_schedule &s, std::string &why)
+{
+    if (s.mode != BM_BATCH_PER_PAIR && s.mode != BM_BATCH_GRAPH && s.mode != BM_BATCH_MULTI)
+        return why = "schedule mode must be BM_BATCH_PER_PAIR, BM_BATCH_GRAPH or BM_BATCH_MULTI", 0;
+    if (s.branches < 1 || s.branches > (int)b->streams.size())
+        return why = "branches outside [1, streams of the batch]", 0;
+    if (s.order != 0 && s.order != 1) return why = "order must be 0 or 1", 0;
+    if (s.persist < 0 || s.persist > 64) return why = "persist outside [0, 64]", 0;
+    return 1;
+}
+
+}  // namespace
+
+extern "C" {
+
 int bm_batch_flow(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r2, float *const *U, float *const *V,
                   int mode)
 {
@@ -1799,54 +2241,115 @@ int bm_batch_flow(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r
     const int n = (int)b->ctx.size();
     for (int i = 0; i < n; ++i)
         if (!r1[i] || !r2[i] || !U[i] || !V[i]) return batch_fail(b, BM_E_PARAM, "null pointer in batch");
+    if (mode == BM_BATCH_AUTO) mode = b->sched.mode;
+    if (mode != BM_BATCH_PER_PAIR && mode != BM_BATCH_GRAPH && mode != BM_BATCH_MULTI)
+        return batch_fail(b, BM_E_PARAM, "unknown batch mode");
     for (bm_ctx *c : b->ctx)  // work already queued on a pair's own stream comes first
         if (hipStreamSynchronize(c->stream) != hipSuccess) return batch_fail(b, BM_E_LAUNCH, "stream sync failed");
     if (mode == BM_BATCH_PER_PAIR) return batch_enqueue(b, r1, r2, U, V, true);
-    if (mode != BM_BATCH_GRAPH) return batch_fail(b, BM_E_PARAM, "unknown batch mode");
-    // the graph bakes in the buffers and every launch argument of every pair:
-    // a new buffer or a configuration change (tuning, pattern, up-sampler) of
-    // any pair's context means a new capture
-    std::vector<uintptr_t> key;
-    for (int i = 0; i < n; ++i) {
-        key.push_back((uintptr_t)r1[i]);
-        key.push_back((uintptr_t)r2[i]);
-        key.push_back((uintptr_t)U[i]);
-        key.push_back((uintptr_t)V[i]);
-        key.push_back((uintptr_t)b->ctx[i]->gen);
+    const std::vector<uintptr_t> key = batch_key(b, r1, r2, U, V);
+    // after a replay every context's field index is the one its captured
+    // launches end in, and no sequence continues from it
+    auto settle = [&](const std::vector<int> &field) {
+        for (int i = 0; i < n; ++i) {
+            b->ctx[i]->field = field[i];
+            b->ctx[i]->have_prev = 0;
+            b->ctx[i]->field_valid = 0;
+        }
+    };
+    if (mode == BM_BATCH_GRAPH) {
+        if (!b->exec || key != b->key) {
+            b->key.clear();
+            const int r = capture(b, true, &b->graph, &b->exec, [&] { return batch_enqueue(b, r1, r2, U, V, false); });
+            if (r != BM_OK) return r;
+            b->gfield.resize(n);
+            for (int i = 0; i < n; ++i) b->gfield[i] = b->ctx[i]->field;
+            b->key = key;
+        }
+        const hipError_t e = hipGraphLaunch(b->exec, b->origin);
+        if (e != hipSuccess) return batch_fail(b, BM_E_LAUNCH, std::string("graph launch: ") + hipGetErrorString(e));
+        settle(b->gfield);
+        return BM_OK;
     }
-    if (!b->exec || key != b->key) {
-        if (b->exec) (void)hipGraphExecDestroy(b->exec);
-        if (b->graph) (void)hipGraphDestroy(b->graph);
-        b->exec = nullptr;
-        b->graph = nullptr;
-        hipError_t e = hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess) e = hipEventRecord(b->fork, b->origin);
-        for (hipStream_t s : b->streams)
-            if (e == hipSuccess) e = hipStreamWaitEvent(s, b->fork, 0);
+    // multi-pair launches, captured once per key and replayed
+    if (!b->mexec || key != b->mkey) {
+        b->mkey.clear();
         int r = BM_OK;
-        if (e == hipSuccess) r = batch_enqueue(b, r1, r2, U, V, false);
-        for (size_t k = 0; k < b->streams.size() && e == hipSuccess; ++k) {
-            e = hipEventRecord(b->joins[k], b->streams[k]);
-            if (e == hipSuccess) e = hipStreamWaitEvent(b->origin, b->joins[k], 0);
+        if (b->pkey != key) {
+            b->pkey.clear();
+            r = build_plan(b, r1, r2, U, V);
+            if (r != BM_OK) return r;
+            b->pkey = key;
         }
-        hipGraph_t g = nullptr;
-        const hipError_t e2 = hipStreamEndCapture(b->origin, &g);
-        if (e == hipSuccess) e = e2;
-        if (r != BM_OK || e != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            (void)hipGetLastError();
-            return r != BM_OK ? r : batch_fail(b, BM_E_LAUNCH, std::string("graph capture: ") + hipGetErrorString(e));
-        }
-        e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            (void)hipGraphDestroy(g);
-            return batch_fail(b, BM_E_LAUNCH, std::string("graph instantiate: ") + hipGetErrorString(e));
-        }
-        b->graph = g;
-        b->key = key;
+        r = capture(b, false, &b->mgraph, &b->mexec, [&] { return run_plan(b, r1, r2, U, V); });
+        if (r != BM_OK) return r;
+        // the plan's pairs end in final_field; the per-pair path of the
+        // others set theirs while it was captured
+        b->mfield.resize(n);
+        for (int i = 0; i < n; ++i)
+            b->mfield[i] = std::find(b->fallback.begin(), b->fallback.end(), i) != b->fallback.end()
+                               ? b->ctx[i]->field
+                               : b->final_field[i];
+        b->mkey = key;
     }
-    const hipError_t e = hipGraphLaunch(b->exec, b->origin);
+    const hipError_t e = hipGraphLaunch(b->mexec, b->origin);
     if (e != hipSuccess) return batch_fail(b, BM_E_LAUNCH, std::string("graph launch: ") + hipGetErrorString(e));
+    settle(b->mfield);
+    return BM_OK;
+}
+
+int bm_batch_get_schedule(const bm_batch *b, bm_batch_schedule *out)
+{
+    if (!b || !out) return fail(nullptr, BM_E_PARAM, "null argument");
+    *out = b->sched;
+    return BM_OK;
+}
+
+int bm_batch_set_schedule(bm_batch *b, const bm_batch_schedule *s)
+{
+    if (!b || !s) return fail(nullptr, BM_E_PARAM, "null argument");
+    std::string why;
+    if (!schedule_valid(b, *s, why)) return batch_fail(b, BM_E_PARAM, why);
+    b->sched = *s;
+    return BM_OK;
+}
+
+int bm_batch_autotune(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r2, float *const *U,
+                      float *const *V, bm_batch_schedule *out)
+{
+    if (!b || !r1 || !r2 || !U || !V) return fail(nullptr, BM_E_PARAM, "null argument");
+    const bm_batch_schedule keep = b->sched;
+    std::vector<bm_batch_schedule> cands;
+    const int S = (int)b->streams.size();
+    std::vector<int> branch_counts{S};
+    if (S / 2 >= 1 && S / 2 != S) branch_counts.push_back(S / 2);
+    for (int order = 0; order <= 1; ++order) {
+        for (int br : branch_counts) cands.push_back({BM_BATCH_GRAPH, br, order, 0});
+        for (int persist : {0, 2, 4, 8}) cands.push_back({BM_BATCH_MULTI, S, order, persist});
+    }
+    float best_ms = -1.0f;
+    bm_batch_schedule best = keep;
+    for (const bm_batch_schedule &c : cands) {
+        b->sched = c;
+        float ms = 0.0f;
+        int r = BM_OK;
+        for (int rep = 0; rep < 3 && r == BM_OK; ++rep) {  // capture, then two timed replays
+            if (rep == 2 && (r = bm_batch_timer_start(b)) != BM_OK) break;
+            r = bm_batch_flow(b, r1, r2, U, V, c.mode);
+            if (r == BM_OK && rep < 2) r = bm_batch_sync(b);
+        }
+        if (r == BM_OK) r = bm_batch_timer_stop(b, &ms);
+        if (r != BM_OK) {
+            b->sched = keep;
+            return r;
+        }
+        if (best_ms < 0.0f || ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    b->sched = best;
+    if (out) *out = best;
     return BM_OK;
 }
 

@@ -85,6 +85,93 @@ hipError_t launch_trilateral(const Geometry &g, const float *Ui, const float *Vi
 
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 
+// ---------------------------------------------------------------------------
+// Multi-pair launches (batches of uneven pairs).  A job is one pair's (or one
+// frame's) arguments of a stage, kept in device memory; one launch covers the
+// tiles of all its jobs, job j owning tiles [start[j], start[j + 1]) of the
+// launch, with the same tile shapes and in-pair tile order as the single-pair
+// kernels, so results are identical.  grid == tiles gives one work-group per
+// tile (XCD k takes the k-th contiguous slice of the tiles); a smaller grid (a
+// multiple of 8) keeps its work-groups resident and each walks its XCD's slice.
+
+struct GrayJob {
+    const uint8_t *rgb;
+    uint8_t *g8;
+    float *g32;  // may be null
+    int n;       // pixels
+};
+
+// Fused up-sampling + BRIEF of one frame.
+struct BriefJob {
+    const uint8_t *g8;
+    const uint32_t *up_x, *up_y;
+    const uint32_t *plan;   // 4 words per record, 64 records per descriptor word
+    const int *plan_count;  // per word: bit records, far records
+    int N, W, H, sx, sy, Rf, gw, gh;
+    uint32_t *F;
+};
+
+// Initialisation and propagation levels of one pair: the field ping-pongs
+// between off[0] / cost[0] and off[1] / cost[1].
+struct MatchJob {
+    const uint32_t *F1, *F2;
+    uint32_t *off[2];
+    uint16_t *cost[2];
+    int sx, sy, double_prop, strip_rows;
+    uint32_t pair, key0, key1;
+    int64_t sigma_q16;
+};
+
+struct DsJob {
+    const uint32_t *off, *nn_x, *nn_y;
+    const float *gauss;
+    int P, sx, sy, W, H;
+    int identity;  // matched = input resolution: the field itself (direct kernel only)
+    float scx, scy;
+    float *U, *V;
+};
+
+struct MedJob {
+    const float *in0, *in1;
+    float *out0, *out1;
+    int W, H;
+};
+
+struct TriJob {
+    const float *Ui, *Vi, *Um, *Vm, *I;
+    float *Uo, *Vo;
+    const float *spt;
+    int W, H, s, BR;
+    float a2, bI;
+};
+
+// Tiles of one job in each stage (the single-pair launch's grid).
+int gray_tiles(int n);
+int brief_tiles(const Geometry &g);
+int match_tiles(const Geometry &g, int px);  // px = 0: init (one pixel per thread)
+int downsample_tiles(const Geometry &g, bool tiled);
+int median_tiles(const Geometry &g);         // per flow component
+int trilateral_tiles(const Geometry &g, int groups);
+// LDS bytes of the single-pair launch of a stage for this geometry.
+size_t brief_lds(const Geometry &g);
+size_t trilateral_lds(const Geometry &g, int *band);
+
+hipError_t launch_multi_gray(const GrayJob *jobs, const int *start, int njobs, int tiles, int grid, hipStream_t st);
+hipError_t launch_multi_brief(int NSP, int RN, const BriefJob *jobs, const int *start, int njobs, int tiles, int grid,
+                              size_t lds, hipStream_t st);
+// rs_iteration >= 0 inserts that random search after the zero / level candidate.
+hipError_t launch_multi_init(int NSP, const MatchJob *jobs, const int *start, int njobs, int tiles, int grid,
+                             int rs_iteration, hipStream_t st);
+hipError_t launch_multi_level(int NSP, int PX, const MatchJob *jobs, const int *start, int njobs, int tiles, int grid,
+                              int p, int pd, int src, int rs_iteration, hipStream_t st);
+hipError_t launch_multi_downsample(bool tiled, const DsJob *jobs, const int *start, int njobs, int tiles, int grid,
+                                   size_t lds, hipStream_t st);
+// both flow components: tiles = 2 x the per-component tiles of each job
+hipError_t launch_multi_median(int FS, int dir, const MedJob *jobs, const int *start, int njobs, int tiles, int grid,
+                               hipStream_t st);
+hipError_t launch_multi_trilateral(int groups, const TriJob *jobs, const int *start, int njobs, int tiles, int grid,
+                                   size_t lds, hipStream_t st);
+
 // LDS budget of the BRIEF tile: the largest halo radius that fits.
 int brief_max_radius();
 // Fused BRIEF: LDS bytes of one work-group for near halo RN and tile height TH,

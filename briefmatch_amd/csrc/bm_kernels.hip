@@ -77,15 +77,21 @@ __device__ __forceinline__ uint32_t bilinear_q16(const uint8_t *__restrict__ g8,
 // ---------------------------------------------------------------------------
 // gray: integer BT.601 of interleaved RGB
 
-__global__ void __launch_bounds__(kThreads) k_gray(const uint8_t *__restrict__ rgb, uint8_t *__restrict__ g8,
-                                                   float *__restrict__ g32, int n)
+__device__ __forceinline__ void gray_tile(const uint8_t *__restrict__ rgb, uint8_t *__restrict__ g8,
+                                          float *__restrict__ g32, int n, int t)
 {
-    const int i = blockIdx.x * kThreads + threadIdx.x;
+    const int i = t * kThreads + threadIdx.x;
     if (i >= n) return;
     const uint32_t r = rgb[3 * (size_t)i], g = rgb[3 * (size_t)i + 1], b = rgb[3 * (size_t)i + 2];
     const uint32_t y = (77u * r + 150u * g + 29u * b + 128u) >> 8;
     if (g8) g8[i] = (uint8_t)y;
     if (g32) g32[i] = (float)y;
+}
+
+__global__ void __launch_bounds__(kThreads) k_gray(const uint8_t *__restrict__ rgb, uint8_t *__restrict__ g8,
+                                                   float *__restrict__ g32, int n)
+{
+    gray_tile(rgb, g8, g32, n, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -252,15 +258,6 @@ __global__ void __launch_bounds__(kThreads) k_brief(const float *__restrict__ L,
 // per-pixel offsets are immediates because the pitch is a template constant);
 // the rest evaluate the bilinear samples from gt.  No test reads global memory.
 
-struct BriefArgs {
-    const uint8_t *g8;
-    const uint32_t *up_x, *up_y;
-    const uint4 *plan;      // 64 records per descriptor word: one per bit, then the far tests
-    const int *plan_count;  // per word: bit records, far records
-    int N, W, H, sx, sy, Rf, gw, gh;
-    uint32_t *F;
-};
-
 __device__ __forceinline__ uint32_t bil_lds(const uint8_t *gt, uint32_t xv, uint32_t yv, int gw)
 {
     const int x0 = (int)(xv & 0xfffu), dx1 = (int)((xv >> 12) & 1u);
@@ -273,11 +270,12 @@ __device__ __forceinline__ uint32_t bil_lds(const uint8_t *gt, uint32_t xv, uint
     return __umul24(top, 256u - ay) + __umul24(bot, ay);
 }
 
+// One 64 x TH tile b (row-major tile index) of the fused BRIEF; smem is the
+// work-group's dynamic LDS.
 template <int NSP, int RN, int PY>
-__global__ void __launch_bounds__(kThreads) k_brief_lds(BriefArgs a)
+__device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned char *smem)
 {
     constexpr int TH = 4 * PY, PITCH = kBriefTW + 2 * RN, ROWS = TH + 2 * RN;
-    extern __shared__ __align__(16) unsigned char smem[];
     float *lt = reinterpret_cast<float *>(smem);
     const int Rf = a.Rf, XT = kBriefTW + 2 * Rf, YT = TH + 2 * Rf;
     uint32_t *xt = reinterpret_cast<uint32_t *>(lt + PITCH * ROWS);
@@ -285,8 +283,7 @@ __global__ void __launch_bounds__(kThreads) k_brief_lds(BriefArgs a)
     uint8_t *gt = reinterpret_cast<uint8_t *>(yt + YT);
     const int sx = a.sx, sy = a.sy, W = a.W, H = a.H;
 
-    const int gxb = (sx + kBriefTW - 1) / kBriefTW, gyb = (sy + TH - 1) / TH;
-    const int b = xcd_remap(blockIdx.x, gxb * gyb);
+    const int gxb = (sx + kBriefTW - 1) / kBriefTW;
     const int tx0 = (b % gxb) * kBriefTW, ty0 = (b / gxb) * TH;
     const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
 
@@ -338,7 +335,7 @@ __global__ void __launch_bounds__(kThreads) k_brief_lds(BriefArgs a)
     // then OR their bits into place.
 #pragma unroll
     for (int w = 0; w < NSP; ++w) {
-        const uint4 *rec = a.plan + 64 * w;
+        const uint4 *rec = reinterpret_cast<const uint4 *>(a.plan) + 64 * w;
         const int nbits = a.plan_count[2 * w], nfar = a.plan_count[2 * w + 1];
         int t = 0;
         for (; t + 2 <= nbits; t += 2) {
@@ -390,6 +387,15 @@ __global__ void __launch_bounds__(kThreads) k_brief_lds(BriefArgs a)
     }
 }
 
+template <int NSP, int RN, int PY>
+__global__ void __launch_bounds__(kThreads) k_brief_lds(BriefJob a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int TH = 4 * PY;
+    const int gxb = (a.sx + kBriefTW - 1) / kBriefTW, gyb = (a.sy + TH - 1) / TH;
+    brief_tile<NSP, RN, PY>(a, xcd_remap(blockIdx.x, gxb * gyb), smem);
+}
+
 // ---------------------------------------------------------------------------
 // field initialisation
 
@@ -428,15 +434,16 @@ __device__ __forceinline__ void random_candidate(const uint32_t (&f1)[NSP], cons
 // candidate.  After a zero initialisation that is the whole first iteration:
 // every propagation candidate of its levels equals the pixel's own (0, 0) and
 // is skipped, so only the random search of its last level changes the field.
+// lb: the row-segment block in the strip-major order (before strip_block).
 template <int NSP, bool RS>
-__global__ void __launch_bounds__(kThreads) k_init(const uint32_t *__restrict__ F1, const uint32_t *__restrict__ F2,
-                                                   int first, const uint32_t *__restrict__ prev,
-                                                   uint32_t *__restrict__ off, uint16_t *__restrict__ cost,
-                                                   int sx, int sy, int strip_rows, RSArgs rs)
+__device__ __forceinline__ void init_tile(const uint32_t *__restrict__ F1, const uint32_t *__restrict__ F2, int first,
+                                          const uint32_t *__restrict__ prev, uint32_t *__restrict__ off,
+                                          uint16_t *__restrict__ cost, int sx, int sy, int strip_rows,
+                                          const RSArgs &rs, int lb)
 {
     const int gx = (sx + kThreads - 1) / kThreads;
     int bx, y;
-    strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, strip_rows, bx, y);
+    strip_block(lb, gx, sy, strip_rows, bx, y);
     const int x = bx * kThreads + threadIdx.x;
     if (x >= sx) return;
     const int idx = y * sx + x;
@@ -458,23 +465,32 @@ __global__ void __launch_bounds__(kThreads) k_init(const uint32_t *__restrict__ 
     cost[idx] = (uint16_t)bcost;
 }
 
+template <int NSP, bool RS>
+__global__ void __launch_bounds__(kThreads) k_init(const uint32_t *__restrict__ F1, const uint32_t *__restrict__ F2,
+                                                   int first, const uint32_t *__restrict__ prev,
+                                                   uint32_t *__restrict__ off, uint16_t *__restrict__ cost,
+                                                   int sx, int sy, int strip_rows, RSArgs rs)
+{
+    const int gx = (sx + kThreads - 1) / kThreads;
+    init_tile<NSP, RS>(F1, F2, first, prev, off, cost, sx, sy, strip_rows, rs, xcd_remap(blockIdx.x, gx * sy));
+}
+
 // ---------------------------------------------------------------------------
 // one jump-flooding level (+ optionally the random search of the iteration)
 
 // PX pixels per thread (256 apart, so every access of a wave stays one
 // coalesced row segment); a tuning choice, results do not depend on it.
 template <int NSP, bool RS, int PX>
-__global__ void __launch_bounds__(kThreads) k_level(const uint32_t *__restrict__ F1, const uint32_t *__restrict__ F2,
-                                                    const uint32_t *__restrict__ off_in,
-                                                    const uint16_t *__restrict__ cost_in,
-                                                    uint32_t *__restrict__ off_out,
-                                                    uint16_t *__restrict__ cost_out, int sx, int sy, int p,
-                                                    int pd, int double_prop, int strip_rows, RSArgs rs)
+__device__ __forceinline__ void level_tile(const uint32_t *__restrict__ F1, const uint32_t *__restrict__ F2,
+                                           const uint32_t *__restrict__ off_in, const uint16_t *__restrict__ cost_in,
+                                           uint32_t *__restrict__ off_out, uint16_t *__restrict__ cost_out, int sx,
+                                           int sy, int p, int pd, int double_prop, int strip_rows, const RSArgs &rs,
+                                           int lb)
 {
     constexpr int span = kThreads * PX;
     const int gx = (sx + span - 1) / span;
     int bx, y;
-    strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, strip_rows, bx, y);
+    strip_block(lb, gx, sy, strip_rows, bx, y);
     const int M = sx * sy;
     const int sv = pd * p * sx, sh = pd * p;
 
@@ -601,6 +617,19 @@ __global__ void __launch_bounds__(kThreads) k_level(const uint32_t *__restrict__
             cost_out[idx[q]] = (uint16_t)bcost;
         }
     }
+}
+
+template <int NSP, bool RS, int PX>
+__global__ void __launch_bounds__(kThreads) k_level(const uint32_t *__restrict__ F1, const uint32_t *__restrict__ F2,
+                                                    const uint32_t *__restrict__ off_in,
+                                                    const uint16_t *__restrict__ cost_in,
+                                                    uint32_t *__restrict__ off_out,
+                                                    uint16_t *__restrict__ cost_out, int sx, int sy, int p,
+                                                    int pd, int double_prop, int strip_rows, RSArgs rs)
+{
+    const int gx = (sx + kThreads * PX - 1) / (kThreads * PX);
+    level_tile<NSP, RS, PX>(F1, F2, off_in, cost_in, off_out, cost_out, sx, sy, p, pd, double_prop, strip_rows, rs,
+                            xcd_remap(blockIdx.x, gx * sy));
 }
 
 template <int NSP>
@@ -743,15 +772,12 @@ __global__ void __launch_bounds__(kThreads) k_init_proposal(const uint32_t *__re
 // down-sampling: Gaussian (horizontal then vertical, as filterConst twice)
 // evaluated only at the nearest-neighbour sample point, then scaled.
 
-__global__ void __launch_bounds__(kThreads) k_downsample(const uint32_t *__restrict__ off,
-                                                         const uint32_t *__restrict__ nn_x,
-                                                         const uint32_t *__restrict__ nn_y,
-                                                         const float *__restrict__ gauss, int P, int sx, int sy,
-                                                         int W, int H, float scx, float scy, int identity,
-                                                         float *__restrict__ U, float *__restrict__ V)
+__device__ __forceinline__ void ds_direct_tile(const uint32_t *__restrict__ off, const uint32_t *__restrict__ nn_x,
+                                               const uint32_t *__restrict__ nn_y, const float *__restrict__ gauss,
+                                               int P, int sx, int sy, int W, int H, float scx, float scy,
+                                               int identity, float *__restrict__ U, float *__restrict__ V, int b)
 {
     const int gx = (W + kThreads - 1) / kThreads;
-    const int b = xcd_remap(blockIdx.x, gx * H);
     const int x = (b % gx) * kThreads + threadIdx.x, y = b / gx;
     if (x >= W) return;
     const size_t o = (size_t)y * W + x;
@@ -778,21 +804,28 @@ __global__ void __launch_bounds__(kThreads) k_downsample(const uint32_t *__restr
     V[o] = gv * scy;
 }
 
+__global__ void __launch_bounds__(kThreads) k_downsample(const uint32_t *__restrict__ off,
+                                                         const uint32_t *__restrict__ nn_x,
+                                                         const uint32_t *__restrict__ nn_y,
+                                                         const float *__restrict__ gauss, int P, int sx, int sy,
+                                                         int W, int H, float scx, float scy, int identity,
+                                                         float *__restrict__ U, float *__restrict__ V)
+{
+    const int gx = (W + kThreads - 1) / kThreads;
+    ds_direct_tile(off, nn_x, nn_y, gauss, P, sx, sy, W, H, scx, scy, identity, U, V, xcd_remap(blockIdx.x, gx * H));
+}
+
 // The same down-sampling per output tile of kDsTileW x kDsTileH input pixels:
 // the field window the tile's NN samples need (clamped to the frame, as the
 // taps are) is staged in LDS once, so the P x P taps of neighbouring outputs,
 // whose windows overlap, read LDS instead of memory.  Same products in the
 // same order as k_downsample.
-__global__ void __launch_bounds__(kThreads) k_downsample_tile(const uint32_t *__restrict__ off,
-                                                              const uint32_t *__restrict__ nn_x,
-                                                              const uint32_t *__restrict__ nn_y,
-                                                              const float *__restrict__ gauss, int P, int sx,
-                                                              int sy, int W, int H, float scx, float scy,
-                                                              float *__restrict__ U, float *__restrict__ V)
+__device__ __forceinline__ void ds_tile(const uint32_t *__restrict__ off, const uint32_t *__restrict__ nn_x,
+                                        const uint32_t *__restrict__ nn_y, const float *__restrict__ gauss, int P,
+                                        int sx, int sy, int W, int H, float scx, float scy, float *__restrict__ U,
+                                        float *__restrict__ V, int blk, uint32_t *win)
 {
-    extern __shared__ uint32_t win[];
-    const int gxb = (W + kDsTileW - 1) / kDsTileW, gyb = (H + kDsTileH - 1) / kDsTileH;
-    const int blk = xcd_remap(blockIdx.x, gxb * gyb);
+    const int gxb = (W + kDsTileW - 1) / kDsTileW;
     const int X0 = (blk % gxb) * kDsTileW, Y0 = (blk / gxb) * kDsTileH;
     const int P2 = P / 2;
     const int cx0 = (int)nn_x[X0] - P2, ry0 = (int)nn_y[Y0] - P2;
@@ -824,6 +857,18 @@ __global__ void __launch_bounds__(kThreads) k_downsample_tile(const uint32_t *__
     V[o] = gv * scy;
 }
 
+__global__ void __launch_bounds__(kThreads) k_downsample_tile(const uint32_t *__restrict__ off,
+                                                              const uint32_t *__restrict__ nn_x,
+                                                              const uint32_t *__restrict__ nn_y,
+                                                              const float *__restrict__ gauss, int P, int sx,
+                                                              int sy, int W, int H, float scx, float scy,
+                                                              float *__restrict__ U, float *__restrict__ V)
+{
+    extern __shared__ uint32_t win[];
+    const int gxb = (W + kDsTileW - 1) / kDsTileW, gyb = (H + kDsTileH - 1) / kDsTileH;
+    ds_tile(off, nn_x, nn_y, gauss, P, sx, sy, W, H, scx, scy, U, V, xcd_remap(blockIdx.x, gxb * gyb), win);
+}
+
 // ---------------------------------------------------------------------------
 // 1-D median: the fsize window (clamped) in registers, reduced to its middle
 // element by a pruned selection network (bm_median_net.h): 186 min/max
@@ -846,17 +891,13 @@ __device__ __forceinline__ void median_ops(float *v)
 }
 
 template <int FS>
-__global__ void __launch_bounds__(kThreads) k_median(const float *__restrict__ in0, const float *__restrict__ in1,
-                                                     float *__restrict__ out0, float *__restrict__ out1, int W,
-                                                     int H, int dir)
+__device__ __forceinline__ void median_tile(const float *__restrict__ in, float *__restrict__ out, int W, int H,
+                                            int dir, int b)
 {
     constexpr int KS = FS / 2;
     const int gx = (W + kThreads - 1) / kThreads;
-    const int b = xcd_remap(blockIdx.x, gx * H);
     const int x = (b % gx) * kThreads + threadIdx.x, y = b / gx;
     if (x >= W) return;
-    const float *in = blockIdx.y ? in1 : in0;
-    float *out = blockIdx.y ? out1 : out0;
     float v[MedianNet<FS>::N];
     if (dir) {
         const float *row = in + (size_t)y * W;
@@ -868,6 +909,15 @@ __global__ void __launch_bounds__(kThreads) k_median(const float *__restrict__ i
     }
     median_ops<FS, 0>(v);
     out[(size_t)y * W + x] = v[KS];
+}
+
+template <int FS>
+__global__ void __launch_bounds__(kThreads) k_median(const float *__restrict__ in0, const float *__restrict__ in1,
+                                                     float *__restrict__ out0, float *__restrict__ out1, int W,
+                                                     int H, int dir)
+{
+    const int gx = (W + kThreads - 1) / kThreads;
+    median_tile<FS>(blockIdx.y ? in1 : in0, blockIdx.y ? out1 : out0, W, H, dir, xcd_remap(blockIdx.x, gx * H));
 }
 
 // ---------------------------------------------------------------------------
@@ -887,19 +937,17 @@ __global__ void __launch_bounds__(kThreads) k_median(const float *__restrict__ i
 constexpr int kTriTW = 64, kTriTH = 16, kTriPY = 4;
 constexpr int kTriLdsBytes = 64 * 1024;
 
-__global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict__ Ui, const float *__restrict__ Vi,
-                                                         const float *__restrict__ Um, const float *__restrict__ Vm,
-                                                         const float *__restrict__ I, float *__restrict__ Uo,
-                                                         float *__restrict__ Vo, int W, int H, int s, int BR,
-                                                         float a2, float bI, const float *__restrict__ spt)
+__device__ __forceinline__ void tri_tile(const float *__restrict__ Ui, const float *__restrict__ Vi,
+                                         const float *__restrict__ Um, const float *__restrict__ Vm,
+                                         const float *__restrict__ I, float *__restrict__ Uo, float *__restrict__ Vo,
+                                         int W, int H, int s, int BR, float a2, float bI,
+                                         const float *__restrict__ spt, int blk, float4 *band)
 {
-    extern __shared__ float4 band[];
     const int pitch = kTriTW + 2 * s;
     // the column part of the spatial weight (2s entries) after the band
     float *spl = reinterpret_cast<float *>(band + (size_t)BR * pitch);
     for (int i = threadIdx.x; i < 2 * s; i += kThreads) spl[i] = spt[i];
-    const int gxb = (W + kTriTW - 1) / kTriTW, gyb = (H + kTriTH - 1) / kTriTH;
-    const int blk = xcd_remap(blockIdx.x, gxb * gyb);
+    const int gxb = (W + kTriTW - 1) / kTriTW;
     const int tx0 = (blk % gxb) * kTriTW, ty0 = (blk / gxb) * kTriTH;
     const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
     const int x = tx0 + lx, y0 = ty0 + 4 * ly;
@@ -987,6 +1035,17 @@ __global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict
     }
 }
 
+__global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict__ Ui, const float *__restrict__ Vi,
+                                                         const float *__restrict__ Um, const float *__restrict__ Vm,
+                                                         const float *__restrict__ I, float *__restrict__ Uo,
+                                                         float *__restrict__ Vo, int W, int H, int s, int BR,
+                                                         float a2, float bI, const float *__restrict__ spt)
+{
+    extern __shared__ float4 band[];
+    const int gxb = (W + kTriTW - 1) / kTriTW, gyb = (H + kTriTH - 1) / kTriTH;
+    tri_tile(Ui, Vi, Um, Vm, I, Uo, Vo, W, H, s, BR, a2, bI, spt, xcd_remap(blockIdx.x, gxb * gyb), band);
+}
+
 // Cross-trilateral filter with the three weighted sums on the matrix cores.
 // v_mfma_f32_4x4x1f32 runs 16 independent 4x4 outer products per wave; block b
 // is lanes 4b..4b+3 and is one output column.  Lane 4b+k computes the weight w
@@ -1037,22 +1096,18 @@ __device__ __forceinline__ void tri_taps_mask(int mask, const float4 *__restrict
 }
 
 template <int R>
-__global__ void __launch_bounds__(kThreads) k_trilateral_mfma(const float *__restrict__ Ui,
-                                                              const float *__restrict__ Vi,
-                                                              const float *__restrict__ Um,
-                                                              const float *__restrict__ Vm,
-                                                              const float *__restrict__ I, float *__restrict__ Uo,
-                                                              float *__restrict__ Vo, int W, int H, int s, int BR,
-                                                              float a2, float bI, const float *__restrict__ spt)
+__device__ __forceinline__ void tri_mfma_tile(const float *__restrict__ Ui, const float *__restrict__ Vi,
+                                              const float *__restrict__ Um, const float *__restrict__ Vm,
+                                              const float *__restrict__ I, float *__restrict__ Uo,
+                                              float *__restrict__ Vo, int W, int H, int s, int BR, float a2, float bI,
+                                              const float *__restrict__ spt, int blk, float4 *band)
 {
     constexpr int TH = 4 * R;
-    extern __shared__ float4 band[];
     const int pitch = kTriTW + 2 * s;
     // exp2 of minus the column part of the spatial term (2s entries) after the band
     float *es = reinterpret_cast<float *>(band + (size_t)BR * pitch);
     for (int i = threadIdx.x; i < 2 * s; i += kThreads) es[i] = __builtin_amdgcn_exp2f(-spt[i]);
-    const int gxb = (W + kTriTW - 1) / kTriTW, gyb = (H + TH - 1) / TH;
-    const int blk = xcd_remap(blockIdx.x, gxb * gyb);
+    const int gxb = (W + kTriTW - 1) / kTriTW;
     const int tx0 = (blk % gxb) * kTriTW, ty0 = (blk / gxb) * TH;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = wave * 16 + (lane >> 2);  // output column within the tile
@@ -1117,12 +1172,149 @@ __global__ void __launch_bounds__(kThreads) k_trilateral_mfma(const float *__res
     }
 }
 
+template <int R>
+__global__ void __launch_bounds__(kThreads) k_trilateral_mfma(const float *__restrict__ Ui,
+                                                              const float *__restrict__ Vi,
+                                                              const float *__restrict__ Um,
+                                                              const float *__restrict__ Vm,
+                                                              const float *__restrict__ I, float *__restrict__ Uo,
+                                                              float *__restrict__ Vo, int W, int H, int s, int BR,
+                                                              float a2, float bI, const float *__restrict__ spt)
+{
+    extern __shared__ float4 band[];
+    const int gxb = (W + kTriTW - 1) / kTriTW, gyb = (H + 4 * R - 1) / (4 * R);
+    tri_mfma_tile<R>(Ui, Vi, Um, Vm, I, Uo, Vo, W, H, s, BR, a2, bI, spt, xcd_remap(blockIdx.x, gxb * gyb), band);
+}
+
 // plain 16-byte-per-lane copy, grid-stride
 __global__ void __launch_bounds__(kThreads) k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
         dst[i] = src[i];
 }
+
+// ---------------------------------------------------------------------------
+// Multi-pair kernels: each runs the single-pair tile body for the tiles of all
+// its jobs (see tile_of / job_of); a work-group may take several tiles, and
+// kernels with LDS tiles wait for the previous tile's readers before reusing it.
+
+#define BM_MULTI_TILES(...)                                                   \
+    for (int it_ = 0;; ++it_) {                                               \
+        const int t_ = tile_of((int)blockIdx.x, (int)gridDim.x, tiles, it_);  \
+        if (t_ < 0) break;                                                    \
+        const int j_ = job_of(start, njobs, t_);                              \
+        const int lt_ = t_ - start[j_];                                       \
+        __VA_ARGS__                                                           \
+    }
+
+__global__ void __launch_bounds__(kThreads) k_gray_multi(const GrayJob *__restrict__ jobs,
+                                                         const int *__restrict__ start, int njobs, int tiles)
+{
+    BM_MULTI_TILES({
+        const GrayJob &J = jobs[j_];
+        gray_tile(J.rgb, J.g8, J.g32, J.n, lt_);
+    })
+}
+
+template <int NSP, int RN, int PY>
+__global__ void __launch_bounds__(kThreads) k_brief_multi(const BriefJob *__restrict__ jobs,
+                                                          const int *__restrict__ start, int njobs, int tiles)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    BM_MULTI_TILES({
+        if (it_) __syncthreads();
+        brief_tile<NSP, RN, PY>(jobs[j_], lt_, smem);
+    })
+}
+
+__device__ __forceinline__ RSArgs job_rs(const MatchJob &J, int rs_iteration)
+{
+    RSArgs rs;
+    rs.enabled = rs_iteration >= 0;
+    rs.iteration = (uint32_t)(rs_iteration < 0 ? 0 : rs_iteration);
+    rs.pair = J.pair;
+    rs.key0 = J.key0;
+    rs.key1 = J.key1;
+    rs.sigma_q16 = J.sigma_q16;
+    return rs;
+}
+
+// zero initialisation into off[0] / cost[0]
+template <int NSP, bool RS>
+__global__ void __launch_bounds__(kThreads) k_init_multi(const MatchJob *__restrict__ jobs,
+                                                         const int *__restrict__ start, int njobs, int tiles,
+                                                         int rs_iteration)
+{
+    BM_MULTI_TILES({
+        const MatchJob &J = jobs[j_];
+        const RSArgs rs = job_rs(J, rs_iteration);
+        init_tile<NSP, RS>(J.F1, J.F2, 1, nullptr, J.off[0], J.cost[0], J.sx, J.sy, J.strip_rows, rs, lt_);
+    })
+}
+
+// one level from off[src] / cost[src] into the other buffer
+template <int NSP, bool RS, int PX>
+__global__ void __launch_bounds__(kThreads) k_level_multi(const MatchJob *__restrict__ jobs,
+                                                          const int *__restrict__ start, int njobs, int tiles, int p,
+                                                          int pd, int src, int rs_iteration)
+{
+    BM_MULTI_TILES({
+        const MatchJob &J = jobs[j_];
+        const RSArgs rs = job_rs(J, rs_iteration);
+        level_tile<NSP, RS, PX>(J.F1, J.F2, J.off[src], J.cost[src], J.off[1 - src], J.cost[1 - src], J.sx, J.sy,
+                                p, pd, J.double_prop, J.strip_rows, rs, lt_);
+    })
+}
+
+__global__ void __launch_bounds__(kThreads) k_downsample_multi(const DsJob *__restrict__ jobs,
+                                                               const int *__restrict__ start, int njobs, int tiles,
+                                                               int tiled)
+{
+    extern __shared__ uint32_t win[];
+    BM_MULTI_TILES({
+        const DsJob &J = jobs[j_];
+        if (tiled) {
+            if (it_) __syncthreads();
+            ds_tile(J.off, J.nn_x, J.nn_y, J.gauss, J.P, J.sx, J.sy, J.W, J.H, J.scx, J.scy, J.U, J.V, lt_, win);
+        } else {
+            ds_direct_tile(J.off, J.nn_x, J.nn_y, J.gauss, J.P, J.sx, J.sy, J.W, J.H, J.scx, J.scy, J.identity, J.U,
+                           J.V, lt_);
+        }
+    })
+}
+
+// a job's tiles are its first component's row blocks, then its second's
+template <int FS>
+__global__ void __launch_bounds__(kThreads) k_median_multi(const MedJob *__restrict__ jobs,
+                                                           const int *__restrict__ start, int njobs, int tiles,
+                                                           int dir)
+{
+    BM_MULTI_TILES({
+        const MedJob &J = jobs[j_];
+        const int per = ((J.W + kThreads - 1) / kThreads) * J.H;
+        const int c = lt_ >= per;
+        median_tile<FS>(c ? J.in1 : J.in0, c ? J.out1 : J.out0, J.W, J.H, dir, lt_ - c * per);
+    })
+}
+
+// groups = 0: VALU sums; 2, 4: MFMA sums with that many 4-row groups
+template <int GROUPS>
+__global__ void __launch_bounds__(kThreads) k_trilateral_multi(const TriJob *__restrict__ jobs,
+                                                               const int *__restrict__ start, int njobs, int tiles)
+{
+    extern __shared__ float4 band[];
+    BM_MULTI_TILES({
+        const TriJob &J = jobs[j_];
+        if (it_) __syncthreads();
+        if constexpr (GROUPS == 0)
+            tri_tile(J.Ui, J.Vi, J.Um, J.Vm, J.I, J.Uo, J.Vo, J.W, J.H, J.s, J.BR, J.a2, J.bI, J.spt, lt_, band);
+        else
+            tri_mfma_tile<GROUPS>(J.Ui, J.Vi, J.Um, J.Vm, J.I, J.Uo, J.Vo, J.W, J.H, J.s, J.BR, J.a2, J.bI, J.spt,
+                                  lt_, band);
+    })
+}
+
+#undef BM_MULTI_TILES
 
 inline unsigned blocks_rows(int width, int rows) { return (unsigned)(((width + kThreads - 1) / kThreads) * rows); }
 
@@ -1198,8 +1390,8 @@ static hipError_t launch_brief_lds_t(const Geometry &g, const uint8_t *g8, uint3
     constexpr int PY = NSP <= 4 ? 8 : 4;
     constexpr int TH = 4 * PY;
     const size_t lds = brief_lds_bytes(g, RN, TH);
-    BriefArgs a{g8, g.up_x, g.up_y, reinterpret_cast<const uint4 *>(g.brief_plan), g.brief_plan_count, g.N, g.W, g.H, g.sx, g.sy, g.brief_far, g.brief_gw[PY == 8 ? 1 : 0],
-                g.brief_gh[PY == 8 ? 1 : 0], F};
+    const BriefJob a{g8,  g.up_x, g.up_y, g.brief_plan, g.brief_plan_count, g.N, g.W, g.H, g.sx, g.sy, g.brief_far,
+                     g.brief_gw[PY == 8 ? 1 : 0], g.brief_gh[PY == 8 ? 1 : 0], F};
     const int gxb = (g.sx + kBriefTW - 1) / kBriefTW, gyb = (g.sy + TH - 1) / TH;
     hipLaunchKernelGGL((k_brief_lds<NSP, RN, PY>), dim3(gxb * gyb), dim3(kThreads), lds, st, a);
     return hipGetLastError();
@@ -1412,6 +1604,191 @@ hipError_t launch_trilateral(const Geometry &g, const float *Ui, const float *Vi
         hipLaunchKernelGGL(k_trilateral_mfma<4>, dim3(gxb * ((g.H + 15) / 16)), dim3(kThreads), lds, st, Ui, Vi, Um,
                            Vm, I, Uo, Vo, g.W, g.H, s, BR, g.tri_a2, g.tri_b, g.tri_spatial);
         break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------------------------
+// multi-pair launches: tile counts per job (the single-pair grids) and launchers
+
+int gray_tiles(int n) { return (n + kThreads - 1) / kThreads; }
+
+int brief_tiles(const Geometry &g)
+{
+    const int TH = brief_tile_rows(g.NSP);
+    return ((g.sx + kBriefTW - 1) / kBriefTW) * ((g.sy + TH - 1) / TH);
+}
+
+int match_tiles(const Geometry &g, int px)
+{
+    const int span = kThreads * (px > 0 ? px : 1);
+    return ((g.sx + span - 1) / span) * g.sy;
+}
+
+int downsample_tiles(const Geometry &g, bool tiled)
+{
+    if (tiled) return ((g.W + kDsTileW - 1) / kDsTileW) * ((g.H + kDsTileH - 1) / kDsTileH);
+    return (int)blocks_rows(g.W, g.H);
+}
+
+int median_tiles(const Geometry &g) { return (int)blocks_rows(g.W, g.H); }
+
+int trilateral_tiles(const Geometry &g, int groups)
+{
+    const int TH = groups == 0 ? kTriTH : 4 * groups;
+    return ((g.W + kTriTW - 1) / kTriTW) * ((g.H + TH - 1) / TH);
+}
+
+size_t brief_lds(const Geometry &g)
+{
+    const int TH = brief_tile_rows(g.NSP);
+    return brief_lds_bytes(g, g.brief_near, TH);
+}
+
+size_t trilateral_lds(const Geometry &g, int *band)
+{
+    const int s = g.tri_s, pitch = kTriTW + 2 * s;
+    int BR = (kTriLdsBytes - 2 * s * (int)sizeof(float)) / (pitch * (int)sizeof(float4));
+    if (BR > g.tri_band) BR = g.tri_band;
+    if (band) *band = BR;
+    if (BR < 1) return 0;
+    return (size_t)BR * pitch * sizeof(float4) + (size_t)2 * s * sizeof(float);
+}
+
+static dim3 multi_grid(int tiles, int grid) { return dim3((unsigned)std::max(1, std::min(grid, tiles))); }
+
+hipError_t launch_multi_gray(const GrayJob *jobs, const int *start, int njobs, int tiles, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_gray_multi, multi_grid(tiles, grid), dim3(kThreads), 0, st, jobs, start, njobs, tiles);
+    return hipGetLastError();
+}
+
+template <int NSP, int RN>
+static void brief_multi_t(const BriefJob *jobs, const int *start, int njobs, int tiles, int grid, size_t lds,
+                          hipStream_t st)
+{
+    constexpr int PY = NSP <= 4 ? 8 : 4;
+    hipLaunchKernelGGL((k_brief_multi<NSP, RN, PY>), multi_grid(tiles, grid), dim3(kThreads), lds, st, jobs, start,
+                       njobs, tiles);
+}
+
+template <int NSP>
+static hipError_t brief_multi_n(int RN, const BriefJob *jobs, const int *start, int njobs, int tiles, int grid,
+                                size_t lds, hipStream_t st)
+{
+    switch (RN) {
+    case 8: brief_multi_t<NSP, 8>(jobs, start, njobs, tiles, grid, lds, st); break;
+    case 16: brief_multi_t<NSP, 16>(jobs, start, njobs, tiles, grid, lds, st); break;
+    case 24: brief_multi_t<NSP, 24>(jobs, start, njobs, tiles, grid, lds, st); break;
+    case 32: brief_multi_t<NSP, 32>(jobs, start, njobs, tiles, grid, lds, st); break;
+    case 40: brief_multi_t<NSP, 40>(jobs, start, njobs, tiles, grid, lds, st); break;
+    case 48: brief_multi_t<NSP, 48>(jobs, start, njobs, tiles, grid, lds, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_brief(int NSP, int RN, const BriefJob *jobs, const int *start, int njobs, int tiles, int grid,
+                              size_t lds, hipStream_t st)
+{
+    switch (NSP) {
+    case 1: return brief_multi_n<1>(RN, jobs, start, njobs, tiles, grid, lds, st);
+    case 2: return brief_multi_n<2>(RN, jobs, start, njobs, tiles, grid, lds, st);
+    case 4: return brief_multi_n<4>(RN, jobs, start, njobs, tiles, grid, lds, st);
+    case 8: return brief_multi_n<8>(RN, jobs, start, njobs, tiles, grid, lds, st);
+    case 16: return brief_multi_n<16>(RN, jobs, start, njobs, tiles, grid, lds, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_multi_init(int NSP, const MatchJob *jobs, const int *start, int njobs, int tiles, int grid,
+                             int rs_iteration, hipStream_t st)
+{
+    const dim3 gd = multi_grid(tiles, grid);
+    if (rs_iteration >= 0) {
+        BM_NSP_DISPATCH(NSP, hipLaunchKernelGGL((k_init_multi<NSP, true>), gd, dim3(kThreads), 0, st, jobs, start,
+                                                njobs, tiles, rs_iteration));
+    } else {
+        BM_NSP_DISPATCH(NSP, hipLaunchKernelGGL((k_init_multi<NSP, false>), gd, dim3(kThreads), 0, st, jobs, start,
+                                                njobs, tiles, rs_iteration));
+    }
+    return hipGetLastError();
+}
+
+template <int NSP, bool RS>
+static hipError_t level_multi_px(int PX, const MatchJob *jobs, const int *start, int njobs, int tiles, int grid,
+                                 int p, int pd, int src, int rs_iteration, hipStream_t st)
+{
+    const dim3 gd = multi_grid(tiles, grid);
+    if constexpr (NSP <= 4) {
+        if (PX == 4) {
+            hipLaunchKernelGGL((k_level_multi<NSP, RS, 4>), gd, dim3(kThreads), 0, st, jobs, start, njobs, tiles, p,
+                               pd, src, rs_iteration);
+            return hipGetLastError();
+        }
+    }
+    if (PX == 2 && NSP <= 8) {
+        hipLaunchKernelGGL((k_level_multi<NSP, RS, (NSP <= 8 ? 2 : 1)>), gd, dim3(kThreads), 0, st, jobs, start,
+                           njobs, tiles, p, pd, src, rs_iteration);
+    } else if (PX == 1) {
+        hipLaunchKernelGGL((k_level_multi<NSP, RS, 1>), gd, dim3(kThreads), 0, st, jobs, start, njobs, tiles, p, pd,
+                           src, rs_iteration);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_level(int NSP, int PX, const MatchJob *jobs, const int *start, int njobs, int tiles, int grid,
+                              int p, int pd, int src, int rs_iteration, hipStream_t st)
+{
+    if (rs_iteration >= 0) {
+        BM_NSP_DISPATCH(NSP, return (level_multi_px<NSP, true>(PX, jobs, start, njobs, tiles, grid, p, pd, src,
+                                                               rs_iteration, st)));
+    } else {
+        BM_NSP_DISPATCH(NSP, return (level_multi_px<NSP, false>(PX, jobs, start, njobs, tiles, grid, p, pd, src,
+                                                                rs_iteration, st)));
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_multi_downsample(bool tiled, const DsJob *jobs, const int *start, int njobs, int tiles, int grid,
+                                   size_t lds, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_downsample_multi, multi_grid(tiles, grid), dim3(kThreads), tiled ? lds : 0, st, jobs, start,
+                       njobs, tiles, tiled ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_median(int FS, int dir, const MedJob *jobs, const int *start, int njobs, int tiles, int grid,
+                               hipStream_t st)
+{
+    const dim3 gd = multi_grid(tiles, grid);
+    switch (FS) {
+#define BM_MED_MULTI(FSV)                                                                                      \
+    case FSV:                                                                                                  \
+        hipLaunchKernelGGL(k_median_multi<FSV>, gd, dim3(kThreads), 0, st, jobs, start, njobs, tiles, dir); \
+        break;
+    BM_MED_MULTI(1) BM_MED_MULTI(3) BM_MED_MULTI(5) BM_MED_MULTI(7) BM_MED_MULTI(9) BM_MED_MULTI(11)
+    BM_MED_MULTI(13) BM_MED_MULTI(15) BM_MED_MULTI(17) BM_MED_MULTI(19) BM_MED_MULTI(21) BM_MED_MULTI(23)
+    BM_MED_MULTI(25) BM_MED_MULTI(27) BM_MED_MULTI(29) BM_MED_MULTI(31) BM_MED_MULTI(33) BM_MED_MULTI(35)
+    BM_MED_MULTI(37) BM_MED_MULTI(39) BM_MED_MULTI(41) BM_MED_MULTI(43) BM_MED_MULTI(45) BM_MED_MULTI(47)
+#undef BM_MED_MULTI
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_trilateral(int groups, const TriJob *jobs, const int *start, int njobs, int tiles, int grid,
+                                   size_t lds, hipStream_t st)
+{
+    const dim3 gd = multi_grid(tiles, grid);
+    switch (groups) {
+    case 0: hipLaunchKernelGGL(k_trilateral_multi<0>, gd, dim3(kThreads), lds, st, jobs, start, njobs, tiles); break;
+    case 2: hipLaunchKernelGGL(k_trilateral_multi<2>, gd, dim3(kThreads), lds, st, jobs, start, njobs, tiles); break;
+    case 4: hipLaunchKernelGGL(k_trilateral_multi<4>, gd, dim3(kThreads), lds, st, jobs, start, njobs, tiles); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -262,9 +262,34 @@ int bm_batch_create(int n, const bm_params *params, uint64_t seed, int streams, 
 void bm_batch_destroy(bm_batch *b);
 int bm_batch_ctx(bm_batch *b, int i, bm_ctx **ctx);
 #define BM_BATCH_PER_PAIR 0  /* the pairs' launches one pair after the other on one stream */
-#define BM_BATCH_GRAPH 1     /* all pairs as one hipGraph launch (captured on first use) */
+#define BM_BATCH_GRAPH 1     /* every pair's own launches on parallel branches, one hipGraph launch */
+#define BM_BATCH_MULTI 2     /* one launch per stage and kernel variant for all pairs together (per-pair
+                                parameter blocks and a tile index across pairs), captured as one hipGraph */
+#define BM_BATCH_AUTO 3      /* the mode of the batch's schedule (bm_batch_autotune) */
+/* The buffers (and every pair's configuration) of a graph are captured on first
+ * use and replayed while they stay the same.  In the multi mode, pairs whose
+ * context cannot use it (Lanczos up-sampler, a BRIEF pattern too wide for the
+ * fused kernel, jfMax 0) run their own launches inside the same graph. */
 int bm_batch_flow(bm_batch *b, const uint8_t *const *rgb1_dev, const uint8_t *const *rgb2_dev, float *const *U_dev,
                   float *const *V_dev, int mode);
+/* Result-invariant schedule of a batch: which mode BM_BATCH_AUTO runs, the
+ * graph mode's branch count and pair-to-branch assignment, the multi mode's job
+ * order and resident work-groups per CU. */
+typedef struct bm_batch_schedule {
+    int mode;      /* BM_BATCH_PER_PAIR, BM_BATCH_GRAPH or BM_BATCH_MULTI */
+    int branches;  /* graph mode: parallel branches, 1 .. the streams of bm_batch_create */
+    int order;     /* 0: pairs in index order (graph: round-robin over branches);
+                      1: largest first (graph: each onto the least-loaded branch) */
+    int persist;   /* multi mode: resident work-groups per CU walking the tiles; 0 = one per tile */
+} bm_batch_schedule;
+int bm_batch_get_schedule(const bm_batch *b, bm_batch_schedule *out);
+int bm_batch_set_schedule(bm_batch *b, const bm_batch_schedule *s);
+/* Times the batch on these buffers under candidate schedules (both orders;
+ * the graph mode with half and all branches; the multi mode one work-group per
+ * tile and 2, 4, 8 resident per CU), keeps the fastest and returns it in *out
+ * (may be NULL).  Results do not depend on the schedule. */
+int bm_batch_autotune(bm_batch *b, const uint8_t *const *rgb1_dev, const uint8_t *const *rgb2_dev,
+                      float *const *U_dev, float *const *V_dev, bm_batch_schedule *out);
 int bm_batch_sync(bm_batch *b);
 int bm_batch_timer_start(bm_batch *b);
 int bm_batch_timer_stop(bm_batch *b, float *ms);

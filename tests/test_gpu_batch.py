@@ -89,13 +89,21 @@ def _batch_workload(n, seed):
 def test_batch_equals_per_pair_and_oracle():
     say("batch")
     wl = _batch_workload(6, 7)
+    # one pair with a different jump range and iteration count (its own level
+    # group in the multi-pair launches), one without refinement
+    wl[2][0].jfMax, wl[2][0].iterations = 64, 3
+    wl[4][0].filter = 0
     batch = bm.Batch([p for p, _ in wl], seed=11, streams=3)
     r1 = [bm.DeviceBuffer(array=pr.frame1) for _, pr in wl]
     r2 = [bm.DeviceBuffer(array=pr.frame2) for _, pr in wl]
     Us = [bm.DeviceBuffer(p.sx_in * p.sy_in * 4) for p, _ in wl]
     Vs = [bm.DeviceBuffer(p.sx_in * p.sy_in * 4) for p, _ in wl]
     results = {}
-    for mode in (bm.BATCH_PER_PAIR, bm.BATCH_GRAPH, bm.BATCH_GRAPH):  # graph captured, then replayed
+    # graphs captured, then replayed; before each run the swapped pairs go
+    # through the per-pair path, so no mode can pass on a buffer a previous
+    # mode left behind
+    for mode in (bm.BATCH_PER_PAIR, bm.BATCH_GRAPH, bm.BATCH_GRAPH, bm.BATCH_MULTI, bm.BATCH_MULTI):
+        batch.flow(r2, r1, Us, Vs, mode=bm.BATCH_PER_PAIR)
         batch.flow(r1, r2, Us, Vs, mode=mode)
         batch.sync()
         res = []
@@ -105,10 +113,12 @@ def test_batch_equals_per_pair_and_oracle():
                         Vs[i].download((p.sy_in, p.sx_in), np.float32)))
         results.setdefault(mode, []).append(res)
     base = results[bm.BATCH_PER_PAIR][0]
-    for res in results[bm.BATCH_GRAPH]:
-        for a, b in zip(base, res):
-            for x, y in zip(a, b):
-                assert np.array_equal(x, y)
+    for mode in (bm.BATCH_GRAPH, bm.BATCH_MULTI):
+        for rep, res in enumerate(results[mode]):
+            for i, (a, b) in enumerate(zip(base, res)):
+                for k, (x, y) in enumerate(zip(a, b)):
+                    assert np.array_equal(x, y), ("mode", mode, "run", rep, "pair", i, "array", k,
+                                                  "differing", int((x != y).sum()), "of", x.size)
     for i, (p, pr) in enumerate(wl):
         ctx = bm.Context(p, seed=11)
         single = _flow(ctx, pr, pair=i)
@@ -137,6 +147,7 @@ def test_batch_graph_follows_configuration_changes():
     Vs = [bm.DeviceBuffer(p.sx_in * p.sy_in * 4) for p, _ in wl]
 
     def run(mode):
+        batch.flow(r2, r1, Us, Vs, mode=bm.BATCH_PER_PAIR)   # other contents in every buffer first
         batch.flow(r1, r2, Us, Vs, mode=mode)
         batch.sync()
         return [(batch.ctx[i].field(), Us[i].download((p.sy_in, p.sx_in), np.float32))
@@ -146,16 +157,60 @@ def test_batch_graph_follows_configuration_changes():
         return all(np.array_equal(x[0][0], y[0][0]) and np.array_equal(x[0][1], y[0][1]) and
                    np.array_equal(x[1], y[1]) for x, y in zip(a, b))
 
-    run(bm.BATCH_GRAPH)                                  # captured with the initial configuration
-    c = batch.ctx[1]
-    t = c.tuning()
-    changes = [lambda: c.set_tuning(brief_near=16 if t.brief_near != 16 else 24, strip_rows=64),
-               lambda: c.set_pattern(np.random.default_rng(3).integers(-12, 13, 4 * c.params.N).astype(np.int16)),
-               lambda: c.set_upsampler(bm.UPSAMPLE_LANCZOS3)]
-    for k, change in enumerate(changes):
-        before = run(bm.BATCH_GRAPH)
-        change()
-        got = run(bm.BATCH_GRAPH)
-        want = run(bm.BATCH_PER_PAIR)
-        assert same(got, want), k
-        assert not same(before[1:2], got[1:2]) or k == 0, k  # pattern / up-sampler change the result
+    for mode in (bm.BATCH_GRAPH, bm.BATCH_MULTI):
+        run(mode)                                        # captured with the initial configuration
+        c = batch.ctx[1]
+        t = c.tuning()
+        changes = [lambda: c.set_tuning(brief_near=16 if t.brief_near != 16 else 24, strip_rows=64),
+                   lambda: c.set_pattern(np.random.default_rng(3 + mode).integers(-12, 13, 4 * c.params.N)
+                                         .astype(np.int16)),
+                   lambda: c.set_upsampler(bm.UPSAMPLE_LANCZOS3),   # the multi mode's per-pair path for this pair
+                   lambda: c.set_upsampler(bm.UPSAMPLE_BILINEAR)]
+        for k, change in enumerate(changes):
+            before = run(mode)
+            change()
+            got = run(mode)
+            want = run(bm.BATCH_PER_PAIR)
+            assert same(got, want), (mode, k)
+            assert not same(before[1:2], got[1:2]) or k == 0, (mode, k)  # pattern / up-sampler change the result
+
+
+def test_batch_schedules_and_autotune():
+    """Every schedule of the multi-pair and graph modes (job order, resident
+    work-groups, branches) gives the per-pair launches' results bit for bit;
+    the batch autotuner returns a valid schedule that BATCH_AUTO then runs."""
+    say("batch schedules")
+    wl = _batch_workload(5, 13)
+    batch = bm.Batch([p for p, _ in wl], seed=3, streams=4)
+    r1 = [bm.DeviceBuffer(array=pr.frame1) for _, pr in wl]
+    r2 = [bm.DeviceBuffer(array=pr.frame2) for _, pr in wl]
+    Us = [bm.DeviceBuffer(p.sx_in * p.sy_in * 4) for p, _ in wl]
+    Vs = [bm.DeviceBuffer(p.sx_in * p.sy_in * 4) for p, _ in wl]
+
+    def run(mode):
+        batch.flow(r2, r1, Us, Vs, mode=bm.BATCH_PER_PAIR)   # other contents in every buffer first
+        batch.flow(r1, r2, Us, Vs, mode=mode)
+        batch.sync()
+        return [(batch.ctx[i].field(), Us[i].download((p.sy_in, p.sx_in), np.float32),
+                 Vs[i].download((p.sy_in, p.sx_in), np.float32)) for i, (p, _) in enumerate(wl)]
+
+    def same(a, b):
+        return all(np.array_equal(x[0][0], y[0][0]) and np.array_equal(x[0][1], y[0][1]) and
+                   np.array_equal(x[1], y[1]) and np.array_equal(x[2], y[2]) for x, y in zip(a, b))
+
+    want = run(bm.BATCH_PER_PAIR)
+    for order in (0, 1):
+        for persist in (0, 1, 4):
+            batch.set_schedule(order=order, persist=persist)
+            assert same(run(bm.BATCH_MULTI), want), (order, persist)
+        for branches in (1, 4):
+            batch.set_schedule(order=order, branches=branches)
+            assert same(run(bm.BATCH_GRAPH), want), (order, branches)
+    s = batch.autotune(r1, r2, Us, Vs)
+    assert s.mode in (bm.BATCH_GRAPH, bm.BATCH_MULTI) and 1 <= s.branches <= 4
+    assert batch.schedule().as_dict() == s.as_dict()
+    assert same(run(bm.BATCH_AUTO), want)
+    with pytest.raises(bm.BriefMatchError):
+        batch.set_schedule(branches=5)
+    with pytest.raises(bm.BriefMatchError):
+        batch.set_schedule(mode=7)
