@@ -352,8 +352,10 @@ def bench_batch(a, bm, n_pairs, specs=None, frames=None):
     tune_s = time.perf_counter() - t0
     modes.append(("autotuned", bm.BATCH_AUTO))
     for name, mode in modes:
-        if name in ("graph", "multi"):              # these with the default schedule
+        if name == "graph":                         # the modes with their default schedules
             batch.set_schedule(mode=mode, branches=8, order=0, persist=0)
+        elif name == "multi":
+            batch.set_schedule(mode=mode, branches=1, order=0, persist=0)
         elif name == "autotuned":
             batch.set_schedule(**sched.as_dict())
         for _ in range(max(1, a.warmup)):

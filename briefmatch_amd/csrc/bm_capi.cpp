@@ -1531,9 +1531,9 @@ struct bm_batch {
     std::vector<uintptr_t> key, mkey;
     // multi-pair plan: launches, their job arrays on the device, the field
     // buffer each pair ends in, and the pairs that take the per-pair path
-    std::vector<MultiOp> ops;
+    std::vector<std::vector<MultiOp>> lanes;  // launches of each lane (a subset of the pairs)
     void *plan_dev = nullptr;
-    std::vector<int> final_field, fallback;
+    std::vector<int> final_field, fallback, lane_of;
     std::vector<uintptr_t> pkey;
     // the field buffer each pair's result lands in when the captured graph of
     // the graph mode / the multi-pair mode replays (host state the replay
@@ -1903,12 +1903,18 @@ MultiOp group_op(PlanBytes &pb, MultiOp::Kind kind, int a, int bk, const std::ve
     op.kind = kind;
     op.a = a;
     op.b = bk;
-    std::vector<int> start(tiles_per_job.size() + 1, 0);
-    for (size_t j = 0; j < tiles_per_job.size(); ++j) start[j + 1] = start[j] + tiles_per_job[j];
+    // slot starts padded to multiples of 8 (each job spread over all XCDs as
+    // in its own launch), then the jobs' tile counts
+    const size_t nj = tiles_per_job.size();
+    std::vector<int> start(2 * nj + 1, 0);
+    for (size_t j = 0; j < nj; ++j) {
+        start[j + 1] = start[j] + (tiles_per_job[j] + 7) / 8 * 8;
+        start[nj + 1 + j] = tiles_per_job[j];
+    }
     op.jobs = jobs_at == std::string::npos ? pb.put(jobs) : jobs_at;
     op.start = pb.put(start);
-    op.njobs = (int)tiles_per_job.size();
-    op.tiles = start.back();
+    op.njobs = (int)nj;
+    op.tiles = start[nj];
     op.lds = lds;
     return op;
 }
@@ -1924,21 +1930,10 @@ bool multi_eligible(const bm_ctx *c)
 // The plan of the multi-pair mode for these buffers: every stage of every
 // eligible pair as one launch per kernel variant, the job arrays in one
 // device buffer (re-made only when a buffer, a configuration or the schedule changes).
-int build_plan(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r2, float *const *U, float *const *V)
+// The launches of one lane: every stage of its pairs as one launch per kernel variant.
+int plan_lane(bm_batch *b, PlanBytes &pb, const std::vector<int> &pairs, const uint8_t *const *r1,
+              const uint8_t *const *r2, float *const *U, float *const *V, std::vector<MultiOp> &ops)
 {
-    const int n = (int)b->ctx.size();
-    const std::vector<int> order = pair_order(b);
-    PlanBytes pb;
-    std::vector<MultiOp> ops;
-    b->fallback.clear();
-    b->final_field.assign(n, 0);
-    std::vector<int> pairs;
-    for (int i : order) {
-        if (multi_eligible(b->ctx[i]))
-            pairs.push_back(i);
-        else
-            b->fallback.push_back(i);
-    }
     // gray: both frames of every pair
     {
         std::vector<bm::GrayJob> jobs;
@@ -2098,6 +2093,41 @@ int build_plan(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r2, 
         }
         ops.push_back(group_op(pb, MultiOp::TRILATERAL, tg, 0, jobs, tiles, lds));
     }
+    return BM_OK;
+}
+
+// The plan of the multi-pair mode for these buffers: the eligible pairs split
+// into `branches` lanes (largest first onto the least-loaded lane, or
+// round-robin in index order), each lane's stages as multi-pair launches on its
+// own branch, so the lanes' different stages overlap; the job arrays in one
+// device buffer (re-made only when a buffer, a configuration or the schedule
+// changes).
+int build_plan(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r2, float *const *U, float *const *V)
+{
+    const int n = (int)b->ctx.size();
+    const int L = std::max(1, std::min(b->sched.branches, (int)b->streams.size()));
+    b->fallback.clear();
+    b->final_field.assign(n, 0);
+    b->lane_of.assign(n, 0);
+    std::vector<std::vector<int>> members(L);
+    std::vector<double> load(L, 0.0);
+    int rr = 0;
+    for (int i : pair_order(b)) {
+        const int l = b->sched.order == 1 ? (int)(std::min_element(load.begin(), load.end()) - load.begin())
+                                           : rr++ % L;
+        load[l] += pair_work(b->ctx[i]);
+        b->lane_of[i] = l;
+        if (multi_eligible(b->ctx[i]))
+            members[l].push_back(i);
+        else
+            b->fallback.push_back(i);
+    }
+    PlanBytes pb;
+    std::vector<std::vector<MultiOp>> lanes(L);
+    for (int l = 0; l < L; ++l) {
+        const int r = plan_lane(b, pb, members[l], r1, r2, U, V, lanes[l]);
+        if (r != BM_OK) return r;
+    }
     // the job arrays on the device (the previous plan's buffer may still be read)
     hipError_t e = hipStreamSynchronize(b->origin);
     if (e == hipSuccess && b->plan_dev) {
@@ -2111,7 +2141,7 @@ int build_plan(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r2, 
         (void)hipGetLastError();
         return batch_fail(b, BM_E_ALLOC, std::string("batch plan: ") + hipGetErrorString(e));
     }
-    b->ops.swap(ops);
+    b->lanes.swap(lanes);
     return BM_OK;
 }
 
@@ -2126,9 +2156,11 @@ int multi_grid(const bm_batch *b, int tiles)
 // pairs the plan does not cover.
 int run_plan(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r2, float *const *U, float *const *V)
 {
-    const hipStream_t st = b->origin;
     const unsigned char *base = static_cast<const unsigned char *>(b->plan_dev);
-    for (const MultiOp &o : b->ops) {
+    const bool branched = b->lanes.size() > 1;
+    for (size_t l = 0; l < b->lanes.size(); ++l) {
+    const hipStream_t st = branched ? b->streams[l] : b->origin;
+    for (const MultiOp &o : b->lanes[l]) {
         const int *start = reinterpret_cast<const int *>(base + o.start);
         const void *jobs = base + o.jobs;
         const int grid = multi_grid(b, o.tiles);
@@ -2164,11 +2196,12 @@ int run_plan(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r2, fl
         }
         if (e != hipSuccess) return batch_fail(b, BM_E_LAUNCH, std::string("multi-pair launch: ") + hipGetErrorString(e));
     }
+    }
     for (int i : b->fallback) {
         bm_ctx *c = b->ctx[i];
         hipStream_t own = c->stream;
         const bool prof = c->prof;
-        c->stream = st;
+        c->stream = branched ? b->streams[b->lane_of[i]] : b->origin;
         c->prof = false;
         const int r = bm_flow_pair(c, r1[i], r2[i], (uint32_t)i, U[i], V[i]);
         c->stream = own;
@@ -2281,7 +2314,7 @@ int bm_batch_flow(bm_batch *b, const uint8_t *const *r1, const uint8_t *const *r
             if (r != BM_OK) return r;
             b->pkey = key;
         }
-        r = capture(b, false, &b->mgraph, &b->mexec, [&] { return run_plan(b, r1, r2, U, V); });
+        r = capture(b, b->lanes.size() > 1, &b->mgraph, &b->mexec, [&] { return run_plan(b, r1, r2, U, V); });
         if (r != BM_OK) return r;
         // the plan's pairs end in final_field; the per-pair path of the
         // others set theirs while it was captured
@@ -2325,7 +2358,9 @@ int bm_batch_autotune(bm_batch *b, const uint8_t *const *r1, const uint8_t *cons
     if (S / 2 >= 1 && S / 2 != S) branch_counts.push_back(S / 2);
     for (int order = 0; order <= 1; ++order) {
         for (int br : branch_counts) cands.push_back({BM_BATCH_GRAPH, br, order, 0});
-        for (int persist : {0, 2, 4, 8}) cands.push_back({BM_BATCH_MULTI, S, order, persist});
+        for (int lanes : {1, 2, 4})
+            if (lanes <= S)
+                for (int persist : {0, 4}) cands.push_back({BM_BATCH_MULTI, lanes, order, persist});
     }
     float best_ms = -1.0f;
     bm_batch_schedule best = keep;

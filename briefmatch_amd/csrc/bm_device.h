@@ -87,32 +87,28 @@ __device__ __forceinline__ void strip_block(int b, int gx, int rows_total, int s
     by = strip * strip_rows + (rem - bx * rows);
 }
 
-// Multi-pair launches: the tile a work-group takes in iteration `it` of a
-// launch over T tiles with G work-groups, or -1 when it has none left.  With
-// G >= T (one work-group per tile) work-group b takes tile xcd_remap(b, T);
-// with fewer (G a multiple of 8, the work-groups stay resident) the
-// work-groups of XCD k walk the k-th contiguous eighth of the tiles.
-__device__ __forceinline__ int tile_of(int b, int G, int T, int it)
-{
-    if (G >= T) return (it == 0 && b < T) ? xcd_remap(b, T) : -1;
-    const int k = b & 7, r = b >> 3, per = G >> 3;
-    const int q = T >> 3, rem = T & 7;
-    const int lo = k * q + min(k, rem), hi = lo + q + (k < rem ? 1 : 0);
-    const int t = lo + r + it * per;
-    return t < hi ? t : -1;
-}
-
-// The job owning tile t: start[j] <= t < start[j + 1] (start[0] = 0).
-__device__ __forceinline__ int job_of(const int *__restrict__ start, int njobs, int t)
+// Multi-pair launches.  Job j owns the work-group slots [start[j], start[j + 1])
+// of the launch, a multiple of 8 slots (start[] is padded so), of which the
+// first count[j] = start[njobs + 1 + j] are its tiles; slot s of the job is
+// taken by a work-group on XCD s mod 8, and takes the job's tile
+// xcd_remap(s, count[j]), the tile the same work-group would take in the
+// job's own launch: every pair is spread over all XCDs, each XCD holding a
+// contiguous part of it.  With fewer work-groups than slots (a multiple of 8,
+// resident), work-group b walks the slots b, b + G, b + 2G, ... (all on its XCD).
+// Returns the job and sets *tile, or -1 for an idle slot.
+__device__ __forceinline__ int job_of(const int *__restrict__ start, int njobs, int slot, int *tile)
 {
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= t)
+        if (start[mid] <= slot)
             lo = mid;
         else
             hi = mid - 1;
     }
+    const int s = slot - start[lo], count = start[njobs + 1 + lo];
+    if (s >= count) return -1;
+    *tile = xcd_remap(s, count);
     return lo;
 }
 

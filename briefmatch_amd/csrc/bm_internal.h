@@ -88,11 +88,13 @@ hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
 // ---------------------------------------------------------------------------
 // Multi-pair launches (batches of uneven pairs).  A job is one pair's (or one
 // frame's) arguments of a stage, kept in device memory; one launch covers the
-// tiles of all its jobs, job j owning tiles [start[j], start[j + 1]) of the
-// launch, with the same tile shapes and in-pair tile order as the single-pair
-// kernels, so results are identical.  grid == tiles gives one work-group per
-// tile (XCD k takes the k-th contiguous slice of the tiles); a smaller grid (a
-// multiple of 8) keeps its work-groups resident and each walks its XCD's slice.
+// tiles of all its jobs with the single-pair kernels' tile shapes, so results
+// are identical.  `start` holds njobs + 1 work-group slot offsets, each job's
+// padded to a multiple of 8, then the njobs tile counts; `tiles` is the slot
+// total.  Each job's tiles are spread over the 8 XCDs as in its own launch
+// (job_of in bm_device.h).  grid == slots gives one work-group per slot; a
+// smaller grid (a multiple of 8) keeps its work-groups resident, each walking
+// the slots of its XCD.
 
 struct GrayJob {
     const uint8_t *rgb;

@@ -1198,17 +1198,18 @@ __global__ void __launch_bounds__(kThreads) k_copy(const uint4 *__restrict__ src
 // its jobs (see tile_of / job_of); a work-group may take several tiles, and
 // kernels with LDS tiles wait for the previous tile's readers before reusing it.
 
-#define BM_MULTI_TILES(...)                                                   \
-    for (int it_ = 0;; ++it_) {                                               \
-        const int t_ = tile_of((int)blockIdx.x, (int)gridDim.x, tiles, it_);  \
-        if (t_ < 0) break;                                                    \
-        const int j_ = job_of(start, njobs, t_);                              \
-        const int lt_ = t_ - start[j_];                                       \
-        __VA_ARGS__                                                           \
+// `slots` is start[njobs]; it_ counts the tiles this work-group has taken
+#define BM_MULTI_TILES(...)                                                              \
+    for (int slot_ = (int)blockIdx.x, it_ = 0; slot_ < slots; slot_ += (int)gridDim.x) { \
+        int lt_ = 0;                                                                     \
+        const int j_ = job_of(start, njobs, slot_, &lt_);                               \
+        if (j_ < 0) continue;                                                            \
+        __VA_ARGS__                                                                      \
+        ++it_;                                                                           \
     }
 
 __global__ void __launch_bounds__(kThreads) k_gray_multi(const GrayJob *__restrict__ jobs,
-                                                         const int *__restrict__ start, int njobs, int tiles)
+                                                         const int *__restrict__ start, int njobs, int slots)
 {
     BM_MULTI_TILES({
         const GrayJob &J = jobs[j_];
@@ -1218,7 +1219,7 @@ __global__ void __launch_bounds__(kThreads) k_gray_multi(const GrayJob *__restri
 
 template <int NSP, int RN, int PY>
 __global__ void __launch_bounds__(kThreads) k_brief_multi(const BriefJob *__restrict__ jobs,
-                                                          const int *__restrict__ start, int njobs, int tiles)
+                                                          const int *__restrict__ start, int njobs, int slots)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     BM_MULTI_TILES({
@@ -1242,7 +1243,7 @@ __device__ __forceinline__ RSArgs job_rs(const MatchJob &J, int rs_iteration)
 // zero initialisation into off[0] / cost[0]
 template <int NSP, bool RS>
 __global__ void __launch_bounds__(kThreads) k_init_multi(const MatchJob *__restrict__ jobs,
-                                                         const int *__restrict__ start, int njobs, int tiles,
+                                                         const int *__restrict__ start, int njobs, int slots,
                                                          int rs_iteration)
 {
     BM_MULTI_TILES({
@@ -1255,7 +1256,7 @@ __global__ void __launch_bounds__(kThreads) k_init_multi(const MatchJob *__restr
 // one level from off[src] / cost[src] into the other buffer
 template <int NSP, bool RS, int PX>
 __global__ void __launch_bounds__(kThreads) k_level_multi(const MatchJob *__restrict__ jobs,
-                                                          const int *__restrict__ start, int njobs, int tiles, int p,
+                                                          const int *__restrict__ start, int njobs, int slots, int p,
                                                           int pd, int src, int rs_iteration)
 {
     BM_MULTI_TILES({
@@ -1267,7 +1268,7 @@ __global__ void __launch_bounds__(kThreads) k_level_multi(const MatchJob *__rest
 }
 
 __global__ void __launch_bounds__(kThreads) k_downsample_multi(const DsJob *__restrict__ jobs,
-                                                               const int *__restrict__ start, int njobs, int tiles,
+                                                               const int *__restrict__ start, int njobs, int slots,
                                                                int tiled)
 {
     extern __shared__ uint32_t win[];
@@ -1286,7 +1287,7 @@ __global__ void __launch_bounds__(kThreads) k_downsample_multi(const DsJob *__re
 // a job's tiles are its first component's row blocks, then its second's
 template <int FS>
 __global__ void __launch_bounds__(kThreads) k_median_multi(const MedJob *__restrict__ jobs,
-                                                           const int *__restrict__ start, int njobs, int tiles,
+                                                           const int *__restrict__ start, int njobs, int slots,
                                                            int dir)
 {
     BM_MULTI_TILES({
@@ -1300,7 +1301,7 @@ __global__ void __launch_bounds__(kThreads) k_median_multi(const MedJob *__restr
 // groups = 0: VALU sums; 2, 4: MFMA sums with that many 4-row groups
 template <int GROUPS>
 __global__ void __launch_bounds__(kThreads) k_trilateral_multi(const TriJob *__restrict__ jobs,
-                                                               const int *__restrict__ start, int njobs, int tiles)
+                                                               const int *__restrict__ start, int njobs, int slots)
 {
     extern __shared__ float4 band[];
     BM_MULTI_TILES({

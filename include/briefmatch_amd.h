@@ -272,22 +272,24 @@ int bm_batch_ctx(bm_batch *b, int i, bm_ctx **ctx);
  * fused kernel, jfMax 0) run their own launches inside the same graph. */
 int bm_batch_flow(bm_batch *b, const uint8_t *const *rgb1_dev, const uint8_t *const *rgb2_dev, float *const *U_dev,
                   float *const *V_dev, int mode);
-/* Result-invariant schedule of a batch: which mode BM_BATCH_AUTO runs, the
- * graph mode's branch count and pair-to-branch assignment, the multi mode's job
- * order and resident work-groups per CU. */
+/* Result-invariant schedule of a batch: which mode BM_BATCH_AUTO runs, how
+ * pairs are spread over parallel branches, their order, and the multi mode's
+ * resident work-groups per CU. */
 typedef struct bm_batch_schedule {
     int mode;      /* BM_BATCH_PER_PAIR, BM_BATCH_GRAPH or BM_BATCH_MULTI */
-    int branches;  /* graph mode: parallel branches, 1 .. the streams of bm_batch_create */
-    int order;     /* 0: pairs in index order (graph: round-robin over branches);
-                      1: largest first (graph: each onto the least-loaded branch) */
+    int branches;  /* parallel branches, 1 .. the streams of bm_batch_create: the graph mode's per-pair
+                      branches; the multi mode's lanes, each a subset of the pairs with its own
+                      multi-pair launches, so different stages of different lanes overlap */
+    int order;     /* 0: pairs in index order, round-robin over branches;
+                      1: largest first, each onto the least-loaded branch */
     int persist;   /* multi mode: resident work-groups per CU walking the tiles; 0 = one per tile */
 } bm_batch_schedule;
 int bm_batch_get_schedule(const bm_batch *b, bm_batch_schedule *out);
 int bm_batch_set_schedule(bm_batch *b, const bm_batch_schedule *s);
 /* Times the batch on these buffers under candidate schedules (both orders;
- * the graph mode with half and all branches; the multi mode one work-group per
- * tile and 2, 4, 8 resident per CU), keeps the fastest and returns it in *out
- * (may be NULL).  Results do not depend on the schedule. */
+ * the graph mode with half and all branches; the multi mode with 1, 2 and 4
+ * lanes, one work-group per tile or 4 resident per CU), keeps the fastest and
+ * returns it in *out (may be NULL).  Results do not depend on the schedule. */
 int bm_batch_autotune(bm_batch *b, const uint8_t *const *rgb1_dev, const uint8_t *const *rgb2_dev,
                       float *const *U_dev, float *const *V_dev, bm_batch_schedule *out);
 int bm_batch_sync(bm_batch *b);

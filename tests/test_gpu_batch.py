@@ -200,11 +200,12 @@ def test_batch_schedules_and_autotune():
 
     want = run(bm.BATCH_PER_PAIR)
     for order in (0, 1):
-        for persist in (0, 1, 4):
-            batch.set_schedule(order=order, persist=persist)
-            assert same(run(bm.BATCH_MULTI), want), (order, persist)
+        for lanes in (1, 2, 4):
+            for persist in (0, 1, 4):
+                batch.set_schedule(order=order, branches=lanes, persist=persist)
+                assert same(run(bm.BATCH_MULTI), want), (order, lanes, persist)
         for branches in (1, 4):
-            batch.set_schedule(order=order, branches=branches)
+            batch.set_schedule(order=order, branches=branches, persist=0)
             assert same(run(bm.BATCH_GRAPH), want), (order, branches)
     s = batch.autotune(r1, r2, Us, Vs)
     assert s.mode in (bm.BATCH_GRAPH, bm.BATCH_MULTI) and 1 <= s.branches <= 4

@@ -35,9 +35,13 @@ def c4():
     Us = [bm.DeviceBuffer(p.sx_in * p.sy_in * 4) for p in params]
     Vs = [bm.DeviceBuffer(p.sx_in * p.sy_in * 4) for p in params]
     runs = {}
-    for name, mode in (("per_pair", bm.BATCH_PER_PAIR), ("graph", bm.BATCH_GRAPH), ("graph_replay", bm.BATCH_GRAPH),
-                       ("multi", bm.BATCH_MULTI), ("multi_replay", bm.BATCH_MULTI)):
-        say("c4 workload: mode", name)
+    for name, mode, sched in (("per_pair", bm.BATCH_PER_PAIR, {}), ("graph", bm.BATCH_GRAPH, {}),
+                              ("graph_replay", bm.BATCH_GRAPH, {}), ("multi", bm.BATCH_MULTI, {"branches": 1}),
+                              ("multi_replay", bm.BATCH_MULTI, {"branches": 1}),
+                              ("multi_lanes", bm.BATCH_MULTI, {"branches": 4, "order": 1, "persist": 4})):
+        say("c4 workload: mode", name, sched)
+        if sched:
+            batch.set_schedule(**sched)
         if name != "per_pair":   # the swapped pairs first: no mode can pass on a buffer left behind
             batch.flow(r2, r1, Us, Vs, mode=bm.BATCH_PER_PAIR)
         batch.flow(r1, r2, Us, Vs, mode=mode)
@@ -51,7 +55,7 @@ def c4():
 
 def test_c4_modes_bit_exact(c4):
     base = c4["runs"]["per_pair"]
-    for name in ("graph", "graph_replay", "multi", "multi_replay"):
+    for name in ("graph", "graph_replay", "multi", "multi_replay", "multi_lanes"):
         for i, (a, b) in enumerate(zip(base, c4["runs"][name])):
             assert np.array_equal(a[0][0], b[0][0]) and np.array_equal(a[0][1], b[0][1]), (name, i)
             assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), (name, i)
