@@ -28,7 +28,7 @@ import time
 
 import numpy as np
 
-from . import Context, DeviceBuffer, Net, derive_params, epe, NET_PARAMS
+from . import Context, DeviceBuffer, Net, derive_params, epe, NET_PARAMS, H2D, _download_raw, _check, lib
 from . import synth
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -61,6 +61,11 @@ def init_weights(seed=0):
 
 def _offset(buf, nbytes):
     return C.c_void_p(buf.ptr.value + int(nbytes))
+
+
+def _upload_raw(ptr, array):
+    a = np.ascontiguousarray(array)
+    _check(lib().bm_memcpy(ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, H2D))
 
 
 FAMILIES = ("rects", "mixed")
@@ -112,18 +117,40 @@ class Dataset:
         return self.label.download((self.rows, self.ch, self.cw, 2), np.float32)
 
 
+LABELS = ("plain", "seeded", "gt")
+
+
+def _complement(label, feat_row, tau):
+    """Complement target: in cells whose cost-volume argmin (feature channels
+    123, 124, in eighths of a cell) is within tau cells of the label in both
+    axes, the argmin itself, elsewhere the label.  The network, whose output
+    is that argmin plus a residual, then learns where the argmin is wrong and
+    repeats it elsewhere, so its candidates add options only there."""
+    am = (feat_row[..., 123:125].astype(np.uint32) << 16).view(np.float32) * 8.0
+    ok = np.isfinite(label).all(-1) & (np.abs(am - label) <= tau).all(-1)
+    out = label.copy()
+    out[ok] = am[ok]
+    return out, float(ok.mean())
+
+
 def build_dataset(pairs, seed0=1000, W=640, H=480, up=1.0, N=128, iterations=4, workers=16, labels="plain",
-                  family="rects", regions_share=0.5, log=print):
+                  family="rects", regions_share=0.5, complement=None, log=print):
     """Generate `pairs` seeded pairs (seeds seed0..), label them and store
     features and labels on the device.  labels="plain": the plain method's
-    flow (`iterations` propagation iterations); labels="gt": the pairs' known
-    flow.  Either is reduced to 8x8 block means in coarse pixels."""
-    if labels not in ("plain", "gt"):
-        raise ValueError("labels must be 'plain' or 'gt'")
+    flow (`iterations` propagation iterations); "seeded": the same method
+    started from the block-matching proposal alone (the learned path with an
+    all-zero network: cost-volume argmin and far match, then `iterations`
+    iterations); "gt": the pairs' known flow.  Each is reduced to 8x8 block
+    means in coarse pixels; with complement = tau the targets become
+    _complement's."""
+    if labels not in LABELS:
+        raise ValueError("labels must be one of %s" % (LABELS,))
     with_gt = labels == "gt"
     params = derive_params(W, H, up, up, N=N)
     params.iterations = iterations
     ctx = Context(params, seed=0)
+    zero = Net() if labels == "seeded" else None
+    kept = []
     cw, ch = ctx.proposal_dims()
     ds = Dataset(pairs, ch, cw)
     r1, r2 = ctx.rgb_buffer(), ctx.rgb_buffer()
@@ -143,18 +170,30 @@ def build_dataset(pairs, seed0=1000, W=640, H=480, up=1.0, N=128, iterations=4, 
                 ctx.gray(r2, g8[1])
                 U.upload(np.ascontiguousarray(job[2][..., 0]))
                 V.upload(np.ascontiguousarray(job[2][..., 1]))
+            elif zero is not None:
+                ctx.flow_pair_learned(zero, r1, r2, U, V, pair=0)
             else:
                 ctx.flow_pair(r1, r2, U, V, pair=0)
             ctx.coarse_flow(U, V, _offset(ds.label, i * ds.label_row))
             ctx.proposal_features(g8[0], g8[1], _offset(ds.feat, i * ds.feat_row))
+            if complement is not None:
+                ctx.sync()
+                lab = _download_raw(_offset(ds.label, i * ds.label_row), (ch, cw, 2), np.float32)
+                feat = _download_raw(_offset(ds.feat, i * ds.feat_row), (ch, cw, 128), np.uint16)
+                lab, frac = _complement(lab, feat, complement)
+                kept.append(frac)
+                _upload_raw(_offset(ds.label, i * ds.label_row), lab)
             if log and ((i + 1) % 1000 == 0 or i + 1 == pairs):
-                log("[data] %d/%d pairs, %.1f s" % (i + 1, pairs, time.time() - t0))
+                extra = " (argmin kept in %.1f %% of the cells)" % (100 * np.mean(kept)) if kept else ""
+                log("[data] %d/%d pairs, %.1f s%s" % (i + 1, pairs, time.time() - t0, extra))
         ctx.sync()
     finally:
         if pool:
             pool.close()
             pool.join()
     ctx.close()
+    if zero is not None:
+        zero.close()
     return ds
 
 
@@ -249,15 +288,19 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--workers", type=int, default=16)
     t.add_argument("--seed", type=int, default=0)
-    t.add_argument("--labels", choices=["plain", "gt"], default="plain",
-                   help="label flow: the plain method's output or the pairs' known flow")
+    t.add_argument("--labels", choices=LABELS, default="plain",
+                   help="label flow: the plain method's output, the same started from the block-matching "
+                        "proposal, or the pairs' known flow")
+    t.add_argument("--complement", type=float, default=None, metavar="TAU",
+                   help="targets: the argmin where it is within TAU cells of the label, else the label")
     t.add_argument("--family", choices=FAMILIES, default="rects",
                    help="training pairs: moving rectangles, or rectangles and large-motion regions")
     t.add_argument("--regions-share", type=float, default=0.5, help="share of region pairs in the mixed family")
     t.add_argument("--label-iterations", type=int, default=4,
                    help="propagation iterations of the plain method that labels the pairs (--labels plain)")
     t.add_argument("--out", required=True)
-    t.add_argument("--eval", action="store_true", help="evaluate on the 1920x1080 workload afterwards")
+    t.add_argument("--eval", action="store_true", help="evaluate on the 1920x1080 and 1280x720 workloads afterwards")
+    t.add_argument("--eval-seeds", type=int, default=8)
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
     e.add_argument("--seeds", type=int, default=4)
@@ -266,7 +309,8 @@ def main(argv=None):
     log = lambda m: print(m, flush=True)
     if a.cmd == "train":
         ds = build_dataset(a.pairs, workers=a.workers, labels=a.labels, family=a.family,
-                           regions_share=a.regions_share, iterations=a.label_iterations, log=log)
+                           regions_share=a.regions_share, iterations=a.label_iterations,
+                           complement=a.complement, log=log)
         net = Net(weights=init_weights(a.seed))
         rows = np.arange(a.pairs - a.val)
         val = np.arange(a.pairs - a.val, a.pairs)
@@ -276,7 +320,9 @@ def main(argv=None):
         net.save(a.out)
         log("[train] saved %s" % a.out)
         if a.eval:
-            log(str(evaluate(net, log=log)))
+            log(str(evaluate(net, seeds=tuple(range(a.eval_seeds)), log=log)))
+            log(str(evaluate(net, workload="c3_1280x720_u4.9_n512", seeds=tuple(range(max(1, a.eval_seeds // 2))),
+                             log=log)))
     else:
         net = Net(path=a.weights)
         log(str(evaluate(net, workload=a.workload, seeds=tuple(range(a.seeds)), log=log)))
