@@ -284,30 +284,35 @@ bool plan_brief(bm_ctx *ctx, int rmax, int rn_req)
     return true;
 }
 
-// Test records of the fused BRIEF, 64 per descriptor word: first one record per
-// bit of the word in descending bit order (LDS offsets of both samples for a near
-// test, whose offsets all lie within the near halo; both offsets 0 for a far
-// test, which then shifts in a 0), then the far tests (packed int16 offsets and
-// the bit position).  Counts per word: bits, far tests.
+// Test records of the fused BRIEF.  Near part: per descriptor word 32 records
+// {o1, o2} (two int32) for bits 31 .. 0, the LDS offsets of both samples for a
+// near test (all offsets within the near halo), both 0 for a far test or a
+// bit beyond the descriptor length (the kernel then shifts in a 0).  Far part,
+// after the NSP * 64 near words: per descriptor word up to 32 records {packed
+// int16 offsets of both samples, bit position, 0}.  Counts per word: bits,
+// far tests.  NSP * 192 words in all, within the NSP * 256 of the buffers.
 void build_brief_plan(bm_ctx *ctx)
 {
     const bm::Geometry &g = ctx->g;
     const int NSP = g.NSP, RN = g.brief_near, pitch = 64 + 2 * RN;
     ctx->plan.assign((size_t)NSP * 64 * 4, 0u);
     ctx->plan_count.assign((size_t)NSP * 2, 0);
+    uint32_t *near_rec = ctx->plan.data();
+    uint32_t *far_rec = ctx->plan.data() + (size_t)NSP * 64;
     for (int w = 0; w < g.NS; ++w) {
         const int nbits = std::min(32, g.N - 32 * w);
-        int r = 0, nfar = 0;
-        for (int b = nbits - 1; b >= 0; --b) {
+        int nfar = 0;
+        for (int i = 0; i < 32; ++i) {
+            const int b = 31 - i;
+            if (b >= nbits) continue;  // shifts in a 0 above the last bit
             const int16_t *o = &ctx->pattern[4 * (32 * w + b)];
             const int m = std::max(std::max(std::abs((int)o[0]), std::abs((int)o[1])),
                                    std::max(std::abs((int)o[2]), std::abs((int)o[3])));
-            uint32_t *rec = &ctx->plan[((size_t)64 * w + r++) * 4];
             if (m <= RN) {
-                rec[0] = (uint32_t)(o[1] * pitch + o[0]);
-                rec[1] = (uint32_t)(o[3] * pitch + o[2]);
+                near_rec[(size_t)64 * w + 2 * i] = (uint32_t)(o[1] * pitch + o[0]);
+                near_rec[(size_t)64 * w + 2 * i + 1] = (uint32_t)(o[3] * pitch + o[2]);
             } else {
-                uint32_t *fr = &ctx->plan[((size_t)64 * w + 32 + nfar++) * 4];
+                uint32_t *fr = far_rec + ((size_t)32 * w + nfar++) * 4;
                 fr[0] = (uint32_t)(uint16_t)o[0] | ((uint32_t)(uint16_t)o[1] << 16);
                 fr[1] = (uint32_t)(uint16_t)o[2] | ((uint32_t)(uint16_t)o[3] << 16);
                 fr[2] = (uint32_t)b;

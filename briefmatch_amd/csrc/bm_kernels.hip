@@ -272,6 +272,7 @@ __device__ __forceinline__ uint32_t bil_lds(const uint8_t *gt, uint32_t xv, uint
 
 // One 64 x TH tile b (row-major tile index) of the fused BRIEF; smem is the
 // work-group's dynamic LDS.
+
 template <int NSP, int RN, int PY>
 __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned char *smem)
 {
@@ -324,43 +325,45 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
 #pragma unroll
         for (int k = 0; k < PY; ++k) words[w][k] = 0u;
 
-    // Per word the plan holds one record per bit in descending bit order, then
-    // the far tests.  A bit record {o1, o2} gives the LDS offsets (row * PITCH +
-    // col) of the two samples; the bit is the sign of b - a on the FP32 bit
-    // patterns (samples are non-negative: gray levels below 256, such floats
-    // order like their bit patterns) and is shifted in at the bottom,
-    // word = word << 1 | sign = alignbit(word, b - a, 31), so after the last
-    // record each bit sits at its position.  A far test's bit record has both
-    // offsets 0 and shifts in a 0; the far records {packed int16 offsets, bit}
-    // then OR their bits into place.
+    // The plan holds, per word, 32 near records {o1, o2} (int2) for bits 31 ..
+    // 0: the LDS offsets (row * PITCH + col) of the two samples, both 0 for a
+    // bit beyond the descriptor length or a far test; after all words, up to
+    // 32 far records per word {packed int16 offsets, bit} (uint4).  A near bit
+    // is the sign of b - a on the FP32 bit patterns (samples are non-negative:
+    // gray levels below 256, such floats order like their bit patterns),
+    // shifted in at the bottom: word = word << 1 | sign = alignbit(word, b - a,
+    // 31); after the 32nd record every bit sits at its position, the (0, 0)
+    // records having shifted in zeros.  The far records then OR their bits into
+    // place.  The near records are taken STEP at a time (one scalar load), so
+    // each pass of the loop has 2 * STEP * PY samples in flight from LDS:
+    // 4 records per pass for 16-row tiles, 2 for 32-row tiles (measured best).
+    constexpr int STEP = PY >= 8 ? 2 : 4;
+    const int2 *near_rec = reinterpret_cast<const int2 *>(a.plan);
+    const uint4 *far_rec = reinterpret_cast<const uint4 *>(a.plan + 64 * NSP);
 #pragma unroll
     for (int w = 0; w < NSP; ++w) {
-        const uint4 *rec = reinterpret_cast<const uint4 *>(a.plan) + 64 * w;
-        const int nbits = a.plan_count[2 * w], nfar = a.plan_count[2 * w + 1];
-        int t = 0;
-        for (; t + 2 <= nbits; t += 2) {
-            const uint4 r0 = rec[t], r1 = rec[t + 1];
-            const uint32_t *p1 = reinterpret_cast<const uint32_t *>(lbase + (int)r0.x);
-            const uint32_t *p2 = reinterpret_cast<const uint32_t *>(lbase + (int)r0.y);
-            const uint32_t *p3 = reinterpret_cast<const uint32_t *>(lbase + (int)r1.x);
-            const uint32_t *p4 = reinterpret_cast<const uint32_t *>(lbase + (int)r1.y);
+#pragma unroll 1
+        for (int i = 0; i < 32; i += STEP) {
+            int2 r[STEP];
 #pragma unroll
-            for (int k = 0; k < PY; ++k) {
-                const uint32_t d0 = p2[4 * k * PITCH] - p1[4 * k * PITCH];
-                const uint32_t d1 = p4[4 * k * PITCH] - p3[4 * k * PITCH];
-                words[w][k] = __builtin_amdgcn_alignbit(__builtin_amdgcn_alignbit(words[w][k], d0, 31), d1, 31);
+            for (int j = 0; j < STEP; ++j) r[j] = near_rec[32 * w + i + j];
+#pragma unroll
+            for (int j = 0; j < STEP; j += 2) {
+                const uint32_t *p1 = reinterpret_cast<const uint32_t *>(lbase + r[j].x);
+                const uint32_t *p2 = reinterpret_cast<const uint32_t *>(lbase + r[j].y);
+                const uint32_t *p3 = reinterpret_cast<const uint32_t *>(lbase + r[j + 1].x);
+                const uint32_t *p4 = reinterpret_cast<const uint32_t *>(lbase + r[j + 1].y);
+#pragma unroll
+                for (int k = 0; k < PY; ++k) {
+                    const uint32_t d0 = p2[4 * k * PITCH] - p1[4 * k * PITCH];
+                    const uint32_t d1 = p4[4 * k * PITCH] - p3[4 * k * PITCH];
+                    words[w][k] = __builtin_amdgcn_alignbit(__builtin_amdgcn_alignbit(words[w][k], d0, 31), d1, 31);
+                }
             }
         }
-        if (t < nbits) {
-            const uint4 r0 = rec[t];
-            const uint32_t *p1 = reinterpret_cast<const uint32_t *>(lbase + (int)r0.x);
-            const uint32_t *p2 = reinterpret_cast<const uint32_t *>(lbase + (int)r0.y);
-#pragma unroll
-            for (int k = 0; k < PY; ++k)
-                words[w][k] = __builtin_amdgcn_alignbit(words[w][k], p2[4 * k * PITCH] - p1[4 * k * PITCH], 31);
-        }
+        const int nfar = a.plan_count[2 * w + 1];
         for (int f = 0; f < nfar; ++f) {
-            const uint4 r0 = rec[32 + f];
+            const uint4 r0 = far_rec[32 * w + f];
             const int ox1 = (int16_t)(r0.x & 0xffffu), oy1 = (int16_t)(r0.x >> 16);
             const int ox2 = (int16_t)(r0.y & 0xffffu), oy2 = (int16_t)(r0.y >> 16);
             const uint32_t xa = xrow[ox1], xb = xrow[ox2];
