@@ -250,24 +250,43 @@ __global__ void __launch_bounds__(kThreads) k_brief(const float *__restrict__ L,
 // BRIEF with the up-sampling fused, everything in LDS.  Per work-group (64 x TH
 // matched pixels, TH = 4 PY, thread (lx, ly) owns rows ly + 4k):
 //   gt  the input-resolution gray tile covering the tile plus the FAR halo Rf
-//       (the largest pattern offset), bytes;
-//   xt, yt  per matched column / row of that halo: the bilinear source index
-//       (relative to gt), its clamped neighbour step and the 8-bit weight;
+//       (the largest pattern offset) as horizontal pairs: 16 bits per source
+//       pixel x, the gray level at x (low byte) and at its clamped right
+//       neighbour min(x + 1, W - 1) (high byte);
+//   xt  per matched column of that halo: the bilinear source column (relative
+//       to gt) and the 8-bit weight a;
+//   yt  per matched row: the byte offsets in gt of the source row and of its
+//       clamped lower neighbour (16 bits each), and the weight pair
+//       (256 - a, a);
 //   lt  the up-sampled tile plus a NEAR halo RN <= Rf, FP32, computed from gt.
 // Tests whose offsets lie within RN read lt (two LDS loads per pixel, the
 // per-pixel offsets are immediates because the pitch is a template constant);
 // the rest evaluate the bilinear samples from gt.  No test reads global memory.
 
-__device__ __forceinline__ uint32_t bil_lds(const uint8_t *gt, uint32_t xv, uint32_t yv, int gw)
+typedef unsigned short bm_u16x2 __attribute__((ext_vector_type(2)));
+
+// Weight pair (256 - a, a) as two 16-bit lanes.
+__device__ __forceinline__ uint32_t weight_pair(uint32_t a) { return (256u - a) | (a << 16); }
+
+// u16 lanes (lo, hi) . (wlo, whi) + 0 on the 16-bit dot-product unit.
+__device__ __forceinline__ uint32_t dot_u16x2(uint32_t v, uint32_t w)
 {
-    const int x0 = (int)(xv & 0xfffu), dx1 = (int)((xv >> 12) & 1u);
-    const uint32_t ax = xv >> 16;
-    const int r0 = (int)(yv & 0x7fffu), dr1 = (int)((yv >> 15) & 1u) * gw;
-    const uint32_t ay = yv >> 16;
-    const uint8_t *p0 = gt + r0 + x0, *p1 = p0 + dr1;
-    const uint32_t top = __umul24(p0[0], 256u - ax) + __umul24(p0[dx1], ax);
-    const uint32_t bot = __umul24(p1[0], 256u - ax) + __umul24(p1[dx1], ax);
-    return __umul24(top, 256u - ay) + __umul24(bot, ay);
+    return __builtin_amdgcn_udot2(__builtin_bit_cast(bm_u16x2, v), __builtin_bit_cast(bm_u16x2, w), 0u, false);
+}
+
+// Bilinear sample in 16.16 fixed point (exact: sum of g_ij wx_j wy_i, below
+// 2^24) from the pair tile at column x0 with weight pair wx, source rows
+// {upper, lower} offsets and weight pair yv: two 16-bit loads, the byte pairs
+// spread to 16-bit lanes and weighted horizontally, then the two row values
+// (each below 2^16) weighted vertically, three dot products in all.
+__device__ __forceinline__ uint32_t bil_lds(const uint16_t *gt, int x0, uint32_t wx, uint2 yv)
+{
+    const unsigned char *p = reinterpret_cast<const unsigned char *>(gt + x0);
+    const uint32_t t = *reinterpret_cast<const uint16_t *>(p + (yv.x & 0xffffu));
+    const uint32_t b = *reinterpret_cast<const uint16_t *>(p + (yv.x >> 16));
+    const uint32_t top = dot_u16x2(__builtin_amdgcn_perm(0u, t, 0x0c010c00u), wx);
+    const uint32_t bot = dot_u16x2(__builtin_amdgcn_perm(0u, b, 0x0c010c00u), wx);
+    return dot_u16x2(top | (bot << 16), yv.y);
 }
 
 // One 64 x TH tile b (row-major tile index) of the fused BRIEF; smem is the
@@ -280,8 +299,8 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
     float *lt = reinterpret_cast<float *>(smem);
     const int Rf = a.Rf, XT = kBriefTW + 2 * Rf, YT = TH + 2 * Rf;
     uint32_t *xt = reinterpret_cast<uint32_t *>(lt + PITCH * ROWS);
-    uint32_t *yt = xt + XT;
-    uint8_t *gt = reinterpret_cast<uint8_t *>(yt + YT);
+    uint2 *yt = reinterpret_cast<uint2 *>(xt + XT + (XT & 1));
+    uint16_t *gt = reinterpret_cast<uint16_t *>(yt + YT);
     const int sx = a.sx, sy = a.sy, W = a.W, H = a.H;
 
     const int gxb = (sx + kBriefTW - 1) / kBriefTW;
@@ -296,29 +315,50 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
 
     for (int i = threadIdx.x; i < XT; i += kThreads) {
         const uint32_t u = a.up_x[clampi(tx0 - Rf + i, 0, sx - 1)];
-        const int x0 = (int)(u >> 8), x1 = min(x0 + 1, W - 1);
-        xt[i] = (uint32_t)(x0 - gx0) | ((uint32_t)(x1 - x0) << 12) | ((u & 255u) << 16);
+        xt[i] = (uint32_t)((int)(u >> 8) - gx0) | ((u & 255u) << 16);
     }
     for (int i = threadIdx.x; i < YT; i += kThreads) {
         const uint32_t u = a.up_y[clampi(ty0 - Rf + i, 0, sy - 1)];
         const int y0 = (int)(u >> 8), y1 = min(y0 + 1, H - 1);
-        yt[i] = (uint32_t)((y0 - gy0) * gw) | ((uint32_t)(y1 - y0) << 15) | ((u & 255u) << 16);
+        yt[i] = make_uint2((uint32_t)((y0 - gy0) * gw * 2) | ((uint32_t)((y1 - gy0) * gw * 2) << 16),
+                           weight_pair(u & 255u));
     }
-    for (int r = ly; r < gh; r += 4) {
-        const uint8_t *src = a.g8 + (size_t)(gy0 + r) * W + gx0;
-        for (int c = lx; c < gw; c += 64) gt[r * gw + c] = src[c];
+    // The pair tile: each gray byte is read from memory once, FB rows at a time
+    // (their loads in flight together: the fill is latency-bound, 8 rows
+    // measured best against 2, 4, 12 and 16); the right neighbour comes from
+    // the next lane, except in lane 63 and past the tile's last column.
+    constexpr int FB = 8;
+    for (int r0 = ly; r0 < gh; r0 += 4 * FB) {
+        for (int c0 = 0; c0 < gw; c0 += 64) {
+            const int c = c0 + lx, x = gx0 + min(c, gw - 1);
+            uint32_t v[FB], n[FB];
+#pragma unroll
+            for (int q = 0; q < FB; ++q) v[q] = a.g8[(size_t)(gy0 + min(r0 + 4 * q, gh - 1)) * W + x];
+#pragma unroll
+            for (int q = 0; q < FB; ++q) n[q] = (uint32_t)__shfl_down((int)v[q], 1, 64);
+            if (lx == 63 || c + 1 >= gw) {
+#pragma unroll
+                for (int q = 0; q < FB; ++q)
+                    n[q] = a.g8[(size_t)(gy0 + min(r0 + 4 * q, gh - 1)) * W + min(x + 1, W - 1)];
+            }
+#pragma unroll
+            for (int q = 0; q < FB; ++q)
+                if (c < gw && r0 + 4 * q < gh) gt[(r0 + 4 * q) * gw + c] = (uint16_t)(v[q] | (n[q] << 8));
+        }
     }
     __syncthreads();
-    for (int r = ly; r < ROWS; r += 4) {
-        const uint32_t yv = yt[r - RN + Rf];
-        for (int c = lx; c < PITCH; c += 64)
-            lt[r * PITCH + c] = (float)bil_lds(gt, xt[c - RN + Rf], yv, gw) * (1.0f / 65536.0f);
+    for (int c = lx; c < PITCH; c += 64) {
+        const uint32_t xv = xt[c - RN + Rf];
+        const int x0 = (int)(xv & 0xffffu);
+        const uint32_t wx = weight_pair(xv >> 16);
+        for (int r = ly; r < ROWS; r += 4)
+            lt[r * PITCH + c] = (float)bil_lds(gt, x0, wx, yt[r - RN + Rf]) * (1.0f / 65536.0f);
     }
     __syncthreads();
 
     const float *lbase = lt + (ly + RN) * PITCH + lx + RN;
     const uint32_t *xrow = xt + lx + Rf;
-    const uint32_t *ycol = yt + ly + Rf;
+    const uint2 *ycol = yt + ly + Rf;
     uint32_t words[NSP][PY];
 #pragma unroll
     for (int w = 0; w < NSP; ++w)
@@ -367,10 +407,12 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
             const int ox1 = (int16_t)(r0.x & 0xffffu), oy1 = (int16_t)(r0.x >> 16);
             const int ox2 = (int16_t)(r0.y & 0xffffu), oy2 = (int16_t)(r0.y >> 16);
             const uint32_t xa = xrow[ox1], xb = xrow[ox2];
+            const int xa0 = (int)(xa & 0xffffu), xb0 = (int)(xb & 0xffffu);
+            const uint32_t wxa = weight_pair(xa >> 16), wxb = weight_pair(xb >> 16);
 #pragma unroll
             for (int k = 0; k < PY; ++k) {
-                const uint32_t va = bil_lds(gt, xa, ycol[4 * k + oy1], gw);
-                const uint32_t vb = bil_lds(gt, xb, ycol[4 * k + oy2], gw);
+                const uint32_t va = bil_lds(gt, xa0, wxa, ycol[4 * k + oy1]);
+                const uint32_t vb = bil_lds(gt, xb0, wxb, ycol[4 * k + oy2]);
                 words[w][k] |= ((vb - va) >> 31) << r0.z;
             }
         }
@@ -1334,9 +1376,10 @@ int brief_max_radius()
 size_t brief_lds_bytes(const Geometry &g, int RN, int TH)
 {
     const size_t lt = (size_t)(kBriefTW + 2 * RN) * (TH + 2 * RN) * sizeof(float);
-    const size_t tabs = (size_t)(kBriefTW + 2 * g.brief_far + TH + 2 * g.brief_far) * sizeof(uint32_t);
+    const size_t xt = (size_t)(kBriefTW + 2 * g.brief_far + 1) / 2 * 2;  // padded to 8 bytes
+    const size_t tabs = xt * sizeof(uint32_t) + (size_t)(TH + 2 * g.brief_far) * sizeof(uint2);
     const int which = TH == 32 ? 1 : 0;
-    return lt + tabs + (size_t)g.brief_gw[which] * g.brief_gh[which];
+    return lt + tabs + (size_t)g.brief_gw[which] * g.brief_gh[which] * sizeof(uint16_t);
 }
 
 int brief_tile_rows(int NSP) { return NSP <= 4 ? 32 : 16; }
