@@ -32,10 +32,11 @@ from . import Context, DeviceBuffer, Net, derive_params, epe, NET_PARAMS, H2D, _
 from . import synth
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-# trained on 16,000 640x480 pairs, half moving rectangles and half large-motion
-# regions, labelled by the plain method with 8 propagation iterations:
-#   python -m briefmatch_amd.learned train --pairs 16000 --epochs 30 --lr 5e-4 --labels plain \
-#       --label-iterations 8 --family mixed --regions-share 0.5 --out proposal_640x480.bnet
+# trained on 12,000 640x480 pairs, half moving rectangles and half large-motion
+# regions, labelled by the method started from the block-matching proposal (8
+# propagation iterations), the argmin kept as target within half a cell:
+#   python -m briefmatch_amd.learned train --pairs 12000 --epochs 30 --lr 5e-4 --labels seeded \
+#       --label-iterations 8 --family mixed --regions-share 0.5 --complement 0.5 --out proposal_640x480.bnet
 DEFAULT_WEIGHTS = os.path.join(_HERE, "weights", "proposal_640x480.bnet")
 
 # (cin, cout) per layer, as in the C ABI's parameter layout
@@ -297,7 +298,7 @@ def main(argv=None):
                    help="training pairs: moving rectangles, or rectangles and large-motion regions")
     t.add_argument("--regions-share", type=float, default=0.5, help="share of region pairs in the mixed family")
     t.add_argument("--label-iterations", type=int, default=4,
-                   help="propagation iterations of the plain method that labels the pairs (--labels plain)")
+                   help="propagation iterations of the method that labels the pairs (--labels plain, seeded)")
     t.add_argument("--out", required=True)
     t.add_argument("--eval", action="store_true", help="evaluate on the 1920x1080 and 1280x720 workloads afterwards")
     t.add_argument("--eval-seeds", type=int, default=8)

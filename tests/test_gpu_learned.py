@@ -289,14 +289,30 @@ def test_learned_path_equals_composed_oracle(motion, W, H, ux, uy):
 
 @pytest.mark.skipif(not os.path.exists(learned.DEFAULT_WEIGHTS), reason="no trained proposal weights")
 def test_trained_proposal_gain():
-    """1920x1080 / u2.9 / 128-bit pairs: the learned path with one iteration
-    reaches the plain two-iteration EPE within 2% and improves on the plain
-    method with one iteration."""
-    say("learned gain c2")
+    """What the learned path gains, with one propagation iteration, over the
+    plain method and over the same seeded path with an all-zero network (its
+    proposal is then the cost-volume argmin alone), on the 1920x1080 / u2.9 /
+    128-bit pairs (seeds 0-7) and the 1280x720 / u4.9 / 512-bit pairs moving up
+    to 60 px (seeds 0-3).  Measured with the committed weights: EPE 0.772
+    against 1.224 (plain, two iterations), 11.58 (plain, one) and 0.769 (zero
+    network) at 1080p; 4.90 against 30.4, 34.4 and 4.93 on the large-motion
+    pairs.  The gain over the plain method comes from the seeded
+    initialisation; the network itself gains 0.7 % on the large-motion pairs
+    and loses 0.4 % at 1080p.  Asserted: the target (within 2 % of plain with
+    two iterations), the gain over the zero network on the large-motion pairs,
+    and at most 1 % loss against it at 1080p."""
     net = bm.Net(path=learned.DEFAULT_WEIGHTS)
-    r = learned.evaluate(net, seeds=(0, 1), fewer=1, log=lambda m: print(m, flush=True))
-    assert r["learned_fewer"] <= 1.02 * r["plain"]
-    assert r["learned_fewer"] < r["plain_fewer"]
+    log = lambda m: print(m, flush=True)
+    say("learned gain c2")
+    c2 = learned.evaluate(net, seeds=tuple(range(8)), fewer=1, log=log)
+    say("learned gain c3")
+    c3 = learned.evaluate(net, workload="c3_1280x720_u4.9_n512", seeds=(0, 1, 2, 3), fewer=1, log=log)
+    print("[gain] c2", c2, "\n[gain] c3", c3, flush=True)
+    assert c2["learned_fewer"] <= 1.02 * c2["plain"]
+    assert c2["learned_fewer"] < 0.1 * c2["plain_fewer"]
+    assert c2["learned_fewer"] <= 1.01 * c2["argmin_fewer"]
+    assert c3["learned_fewer"] <= 0.2 * c3["plain"]
+    assert c3["learned_fewer"] < c3["argmin_fewer"]
 
 
 def _staged_learned_epe(ctx, net, p, pr, with_far):
@@ -338,9 +354,9 @@ def _staged_learned_epe(ctx, net, p, pr, with_far):
 def test_far_match_gain():
     """With one iteration, the far match lowers the learned path's EPE by at
     least 10 % on 1280x720 pairs moving up to 60 px (beyond the cost volume's
-    40 px; measured 56 % with the committed weights, 17 % with weights trained
-    on that motion's known flow) and costs at most 3 % on the 1920x1080 pairs; the
-    composed stages with the far match are bm_flow_pair_learned itself."""
+    40 px; measured 43 % with the committed weights on these two seeds) and
+    costs at most 3 % on the 1920x1080 pairs (measured 1.2 %); the composed
+    stages with the far match are bm_flow_pair_learned itself."""
     net = bm.Net(path=learned.DEFAULT_WEIGHTS)
     for name, seeds in (("c3_1280x720_u4.9_n512", (0, 1)), ("c2_1920x1080_u2.9_n128", (0, 1))):
         wl = synth.WORKLOADS[name]
