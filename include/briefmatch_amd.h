@@ -233,7 +233,8 @@ int bm_flow_pair_static(const bm_params *params, uint64_t seed, const uint8_t *r
 /* Launch-shape choices of one context.  Every valid tuning computes identical
  * results; only the speed changes. */
 typedef struct bm_tuning {
-    int level_px;     /* pixels per thread of the jump-flooding kernels: 1, 2 (N <= 256) or 4 (N <= 128) */
+    int level_px;     /* pixels per thread of the jump-flooding kernels: 1, 2 (N <= 256) or 4 (N <= 128);
+                         above 256 bits they run two threads per pixel whatever the value */
     int strip_rows;   /* rows per strip of the matching kernels' work-group order: 16..4096 */
     int brief_near;   /* LDS near halo of the fused BRIEF: 8 .. 48 in steps of 8 (40, 48 need > 64 KiB of LDS
                          per work-group, as on gfx950); 0 = kernel with global far taps */
