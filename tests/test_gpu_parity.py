@@ -389,10 +389,11 @@ def test_flow_pair_c1(filt):
     assert e_all < (0.5 if filt else 0.95) * z_all
 
 
-def test_sequence_temporal_init():
-    """Sequence mode: three frames, temporal init from the previous field."""
-    say("stream 3 frames")
-    W, H, up, N = 120, 90, 2.0, 64
+@pytest.mark.parametrize("W,H,up,N", [(120, 90, 2.0, 64), (80, 60, 2.5, 512)])
+def test_sequence_temporal_init(W, H, up, N):
+    """Sequence mode: three frames, temporal init from the previous field (at
+    512 bits the initialisation and level kernels run two lanes per pixel)."""
+    say("stream 3 frames", W, H, up, N)
     base = _pair(W, H, seed=31, max_motion=5.0)
     f3 = _pair(W, H, seed=32, max_motion=5.0).frame2
     frames = [base.frame1, base.frame2, f3]
