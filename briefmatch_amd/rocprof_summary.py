@@ -33,7 +33,7 @@ def short_name(name):
 
 
 STAGE_OF = OrderedDict([("k_gray", "gray"), ("k_upsample", "upsample"), ("k_brief", "brief"),
-                        ("k_init_proposal", "proposal"), ("k_init", "init"), ("k_level<", "level"), ("k_random_search", "random_search"),
+                        ("k_init_proposal", "proposal"), ("k_init", "init"), ("k_level<", "level"), ("k_level_lanes<", "level"), ("k_random_search", "random_search"),
                         ("k_downsample", "downsample"), ("k_median", "median"), ("k_trilateral", "trilateral"),
                         ("k_copy", "copy"), ("k_features", "proposal"),
                         ("k_block_sums", "proposal"), ("k_far_search", "proposal"), ("k_conv", "proposal"),
@@ -43,7 +43,7 @@ STAGE_OF = OrderedDict([("k_gray", "gray"), ("k_upsample", "upsample"), ("k_brie
 def stage_of(short):
     for k, v in STAGE_OF.items():
         if short.startswith(k):
-            if v == "level" and short.endswith("true>"):
+            if v == "level" and ", true" in short:
                 return "level_rs"
             return v
     return short
