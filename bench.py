@@ -19,10 +19,13 @@ Prints one JSON line.  Besides the driver's fields it carries:
                 stage's own ceilings (HBM, LDS, VALU issue, FP32);
   streaming     sequence mode (one BRIEF per frame, temporal init), serial and
                 pipelined (each frame's refinement tail beside the next frame);
-  learned       the same pair through the learned-proposal path with one
-                propagation iteration: throughput, executed propagation levels
-                per pair for both paths, EPE next to the plain method's, and
-                the zero-network ablation's EPE (see --no-learned);
+  learned       the same pair through the learned-proposal path (seeded
+                initialisation, a local search around the seeds, one
+                propagation iteration over the jump range 8): throughput,
+                executed propagation levels per pair for both paths, EPE next
+                to the plain method's and to the zero-network ablation's; the
+                same with the full jump range and with no propagation
+                (see --no-learned);
   parity        the GPU match field against the CPU oracle on the same pair;
   cpu_baseline  the CPU oracle on the box's cores: one warm-up, the median of
                 three timed runs, a 1-thread figure, the CPU model (--no-cpu);
@@ -276,56 +279,79 @@ def _level_launches(prof, steps):
     return (prof.get("level", (0, 0))[1] + prof.get("level_rs", (0, 0))[1]) / steps
 
 
+# The learned path's configurations: (iterations, jump range, local search
+# radius in input pixels, samples).  "learned" is the path bench.py reports as
+# the learned leg: one propagation iteration over the jump range 8 (4 level
+# launches) after the seeded initialisation and a local search of 6 samples
+# around the seeds; the others show the full jump range (9 levels, no local
+# search) and no propagation at all.
+LEARNED_CONFIGS = (("learned", 1, 8, 16.0, 6), ("full_jump_range", 1, 256, 0.0, 0),
+                   ("without_propagation", 0, 256, 0.0, 0))
+
+
 def bench_learned(a, bm, s, plain_out):
-    """The pair through bm_flow_pair_learned with one propagation iteration
-    (the committed network), timed like the plain path; EPE next to the plain
+    """The pair through bm_flow_pair_learned (the committed network) in each
+    of LEARNED_CONFIGS, timed like the plain path; EPE next to the plain
     method's and next to the same path with an all-zero network (whose
-    proposal is the cost-volume argmin alone)."""
+    proposal is the cost-volume argmin alone), so the ratio is what the
+    trained network itself gains; executed level launches per pair against
+    the plain method's."""
     from briefmatch_amd import learned
     if not os.path.exists(learned.DEFAULT_WEIGHTS):
         return None
     p, pr, W, H = s["p"], s["pr"], s["W"], s["H"]
-    pf = p.copy()
-    pf.iterations = max(1, p.iterations // 2)
-    res = {}
-    for label, net in (("trained", bm.Net(path=learned.DEFAULT_WEIGHTS)),
-                       ("zero_network", bm.Net(weights=np.zeros(bm.NET_PARAMS, np.float32)))):
-        ctx = bm.Context(pf, seed=1)
-        ctx.set_tuning(**s["tuning"])
-        U, V = ctx.image_buffer(), ctx.image_buffer()
-        for _ in range(a.warmup):
-            ctx.flow_pair_learned(net, s["rgb1"], s["rgb2"], U, V)
-        ctx.sync()
-        ctx.profile_enable(True)
-        ctx.profile_read()
-        ctx.timer_start()
-        for _ in range(a.steps):
-            ctx.flow_pair_learned(net, s["rgb1"], s["rgb2"], U, V)
-        total_ms = ctx.timer_stop()
-        prof = ctx.profile_read()
-        ctx.profile_enable(False)
-        ms = total_ms / a.steps
-        e_all, e_noc = bm.epe(U.download((H, W), np.float32), V.download((H, W), np.float32), pr.gt, pr.occ)
-        res[label] = dict(ms=ms, prof=prof, e_all=e_all, e_noc=e_noc)
-        ctx.close()
-        net.close()
-    t = res["trained"]
+    nets = {"trained": bm.Net(path=learned.DEFAULT_WEIGHTS),
+            "zero_network": bm.Net(weights=np.zeros(bm.NET_PARAMS, np.float32))}
     plain_levels = _level_launches(plain_out["_prof"], a.steps)
-    return {"iterations": pf.iterations, "plain_iterations": p.iterations,
+    out = {}
+    for name, iters, jf, ls_px, ls_n in LEARNED_CONFIGS:
+        pf = p.copy()
+        pf.iterations = iters
+        pf.jfMax = jf
+        ls_r = learned.local_search_radius(pf, ls_px) if ls_n else 0
+        res = {}
+        for label, net in nets.items():
+            ctx = bm.Context(pf, seed=1)
+            ctx.set_tuning(**s["tuning"])
+            ctx.set_local_search(ls_r, ls_n)
+            U, V = ctx.image_buffer(), ctx.image_buffer()
+            for _ in range(a.warmup):
+                ctx.flow_pair_learned(net, s["rgb1"], s["rgb2"], U, V)
+            ctx.sync()
+            ctx.profile_enable(True)
+            ctx.profile_read()
+            ctx.timer_start()
+            for _ in range(a.steps):
+                ctx.flow_pair_learned(net, s["rgb1"], s["rgb2"], U, V)
+            total_ms = ctx.timer_stop()
+            prof = ctx.profile_read()
+            ctx.profile_enable(False)
+            e_all, e_noc = bm.epe(U.download((H, W), np.float32), V.download((H, W), np.float32), pr.gt, pr.occ)
+            res[label] = dict(ms=total_ms / a.steps, prof=prof, e_all=e_all, e_noc=e_noc)
+            ctx.close()
+        t, z = res["trained"], res["zero_network"]
+        out[name] = {
+            "iterations": iters, "jfMax": jf,
+            "local_search": {"radius_input_px": ls_px, "radius_matched_px": ls_r, "samples": ls_n},
             "ms_per_step": round(t["ms"], 4), "value": round(W * H / (t["ms"] * 1e3), 3), "unit": "Mpix/s",
-            "propagation_level_launches_per_pair": {"learned": _level_launches(t["prof"], a.steps),
-                                                    "plain": plain_levels},
-            "note": "both paths execute one propagation pass of 9 levels: the plain method's first iteration "
-                    "is the identity after its zero initialisation and runs as its random search inside the "
-                    "initialisation kernel",
+            "propagation_level_launches_per_pair": _level_launches(t["prof"], a.steps),
             "epe": {"all": round(t["e_all"], 4), "noc": round(t["e_noc"], 4)},
-            "plain_epe": plain_out["epe"],
             "epe_ratio_vs_plain": round(t["e_all"] / plain_out["epe"]["all"], 4),
-            "zero_network_epe": {"all": round(res["zero_network"]["e_all"], 4),
-                                 "noc": round(res["zero_network"]["e_noc"], 4)},
-            "epe_ratio_vs_zero_network": round(t["e_all"] / res["zero_network"]["e_all"], 4),
-            "stages_ms_per_step": {k: round(v[0] / a.steps, 4) for k, v in t["prof"].items() if v[1]},
-            "weights": os.path.relpath(learned.DEFAULT_WEIGHTS, ROOT)}
+            "zero_network_epe": {"all": round(z["e_all"], 4), "noc": round(z["e_noc"], 4)},
+            "epe_ratio_vs_zero_network": round(t["e_all"] / z["e_all"], 4),
+            "stages_ms_per_step": {k: round(v[0] / a.steps, 4) for k, v in t["prof"].items() if v[1]}}
+    for net in nets.values():
+        net.close()
+    head = out.pop("learned")
+    head.update({"plain_iterations": p.iterations, "plain_epe": plain_out["epe"],
+                 "plain_propagation_level_launches_per_pair": plain_levels,
+                 "note": "the plain method executes one propagation pass of 9 levels (its first iteration is the "
+                         "identity after its zero initialisation and runs as its random search inside the "
+                         "initialisation kernel); the learned path seeds the field, searches around the seeds and "
+                         "propagates over the jump range 8 (4 levels)",
+                 "weights": os.path.relpath(learned.DEFAULT_WEIGHTS, ROOT)})
+    head.update(out)
+    return head
 
 
 def bench_batch(a, bm, n_pairs, specs=None, frames=None):

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(_HERE, "libbriefmatch_amd.so")
 BM_OK, BM_E_PARAM, BM_E_ALLOC, BM_E_LAUNCH, BM_E_IO, BM_E_STATE = range(6)
 # stage ids of bm_profile_read (BM_STAGE_* in briefmatch_amd.h)
 STAGES = ("gray", "upsample", "brief", "init", "level", "level_rs", "random_search", "downsample", "median",
-          "trilateral", "copy", "proposal")
+          "trilateral", "copy", "proposal", "local_search")
 H2D, D2H, D2D, H2H = range(4)
 
 
@@ -159,6 +159,9 @@ def lib():
         "bm_proposal_far": (I, [V, V, V, V]),
         "bm_proposal_offsets": (I, [V, V, V, V, V]),
         "bm_init_proposal": (I, [V, V, V, I, V, V, V, V]),
+        "bm_set_local_search": (I, [V, I, I]),
+        "bm_get_local_search": (I, [V, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "bm_local_search": (I, [V, V, V, V, V, U32, U32]),
         "bm_flow_pair_learned": (I, [V, V, V, V, U32, V, V]),
     }
     for name, (res, args) in sig.items():
@@ -502,6 +505,20 @@ class Context:
     def init_proposal(self, F1, F2, coff, off, cost, first=True, prev_off=None):
         self._c(lib().bm_init_proposal(self.h, _ptr(F1), _ptr(F2), 1 if first else 0, _ptr(prev_off), _ptr(coff),
                                        _ptr(off), _ptr(cost)))
+
+    def set_local_search(self, radius, samples):
+        """Local search around the seeds of the learned path: `samples`
+        candidates per matched pixel around the running best offset, the
+        square's half side halving from `radius` (0 samples: off)."""
+        self._c(lib().bm_set_local_search(self.h, int(radius), int(samples)))
+
+    def local_search_config(self):
+        r, n = C.c_int(), C.c_int()
+        self._c(lib().bm_get_local_search(self.h, C.byref(r), C.byref(n)))
+        return r.value, n.value
+
+    def local_search(self, F1, F2, off, cost, iteration, pair=0):
+        self._c(lib().bm_local_search(self.h, _ptr(F1), _ptr(F2), _ptr(off), _ptr(cost), iteration, pair))
 
     def flow_pair_learned(self, net, rgb1, rgb2, U, V, pair=0):
         self._c(lib().bm_flow_pair_learned(self.h, net.h, _ptr(rgb1), _ptr(rgb2), pair, _ptr(U), _ptr(V)))

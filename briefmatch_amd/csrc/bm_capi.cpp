@@ -92,6 +92,7 @@ struct bm_ctx {
     int32_t *d_coff = nullptr;             // [ch][cw][4] Q8
     int32_t *d_bsum = nullptr;             // [2][ch][cw] block sums
     int32_t *d_far = nullptr;              // [ch / 2][cw / 2][2] far block match (frames of 16x16 and up)
+    int ls_radius = 0, ls_samples = 0;     // bm_set_local_search (0 samples: off)
     // second stream (own hardware queue): the proposal network beside the
     // descriptor kernels, and the refinement tail of a pipelined sequence
     hipStream_t side = nullptr;
@@ -1301,6 +1302,42 @@ int bm_init_proposal(bm_ctx *ctx, const uint32_t *F1, const uint32_t *F2, int fi
     return BM_OK;
 }
 
+int bm_set_local_search(bm_ctx *ctx, int radius, int samples)
+{
+    if (!ctx) return fail(nullptr, BM_E_PARAM, "null context");
+    if (samples < 0 || samples > BM_LOCAL_SEARCH_MAX_SAMPLES)
+        return fail(ctx, BM_E_PARAM, "local search samples must be in [0, " +
+                                         std::to_string(BM_LOCAL_SEARCH_MAX_SAMPLES) + "]");
+    if (samples > 0 && (radius < 1 || radius > BM_LOCAL_SEARCH_MAX_RADIUS))
+        return fail(ctx, BM_E_PARAM, "local search radius must be in [1, " +
+                                         std::to_string(BM_LOCAL_SEARCH_MAX_RADIUS) + "]");
+    ctx->ls_radius = samples > 0 ? radius : 0;
+    ctx->ls_samples = samples;
+    return BM_OK;
+}
+
+int bm_get_local_search(const bm_ctx *ctx, int *radius, int *samples)
+{
+    if (!ctx || !radius || !samples) return fail(nullptr, BM_E_PARAM, "null argument");
+    *radius = ctx->ls_radius;
+    *samples = ctx->ls_samples;
+    return BM_OK;
+}
+
+int bm_local_search(bm_ctx *ctx, const uint32_t *F1, const uint32_t *F2, uint32_t *off, uint16_t *cost,
+                    uint32_t iteration, uint32_t pair)
+{
+    int r = check(ctx, F1, "F1");
+    if (!r) r = check(ctx, F2, "F2");
+    if (!r) r = check(ctx, off, "off");
+    if (!r) r = check(ctx, cost, "cost");
+    if (r) return r;
+    BM_LAUNCH(ctx, BM_STAGE_LOCAL_SEARCH,
+              bm::launch_local_search(ctx->g, F1, F2, ctx->ls_radius, ctx->ls_samples, iteration, pair, off, cost,
+                                      ctx->stream));
+    return BM_OK;
+}
+
 int bm_flow_pair_learned(bm_ctx *ctx, bm_net *net, const uint8_t *rgb1, const uint8_t *rgb2, uint32_t pair, float *U,
                          float *V)
 {
@@ -1341,6 +1378,11 @@ int bm_flow_pair_learned(bm_ctx *ctx, bm_net *net, const uint8_t *rgb1, const ui
     BM_LAUNCH(ctx, BM_STAGE_PROPOSAL,
               bm::launch_init_proposal(g, ctx->d_F[0], ctx->d_F[1], 1, nullptr, ctx->d_coff, cw, ch, g.iterations,
                                        pair, ctx->d_off[dst], ctx->d_cost[dst], ctx->stream));
+    if (ctx->ls_samples > 0)
+        BM_LAUNCH(ctx, BM_STAGE_LOCAL_SEARCH,
+                  bm::launch_local_search(g, ctx->d_F[0], ctx->d_F[1], ctx->ls_radius, ctx->ls_samples,
+                                          (uint32_t)g.iterations, pair, ctx->d_off[dst], ctx->d_cost[dst],
+                                          ctx->stream));
     int pd = -1;
     for (int i = 0; i < g.iterations; ++i) {
         r = run_iteration(ctx, ctx->d_F[0], ctx->d_F[1], pd, i, pair, true);

@@ -75,6 +75,10 @@ hipError_t launch_random_search(const Geometry &g, const uint32_t *F1, const uin
 hipError_t launch_init_proposal(const Geometry &g, const uint32_t *F1, const uint32_t *F2, int first,
                                 const uint32_t *prev, const int32_t *coff, int cw, int ch, int rs_iteration,
                                 uint32_t pair, uint32_t *off, uint16_t *cost, hipStream_t st);
+// Local search around the current offsets (in place): `samples` candidates,
+// sample s uniform in [-r, r]^2 around the running best, r = max(radius >> s, 1).
+hipError_t launch_local_search(const Geometry &g, const uint32_t *F1, const uint32_t *F2, int radius, int samples,
+                               uint32_t iteration, uint32_t pair, uint32_t *off, uint16_t *cost, hipStream_t st);
 hipError_t launch_downsample(const Geometry &g, const uint32_t *off, float *U, float *V, hipStream_t st);
 // Output tile of the LDS-tiled down-sampling (columns x rows of input pixels).
 constexpr int kDsTileW = 64, kDsTileH = 4, kDsLdsMax = 64 * 1024;

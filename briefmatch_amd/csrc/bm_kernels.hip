@@ -1069,15 +1069,73 @@ __global__ void __launch_bounds__(kThreads) k_init_proposal(const uint32_t *__re
     cost[i] = (uint16_t)best;
 }
 
+// Local search around each pixel's current offset (the learned path's
+// refinement of its seeds; the reference's random search samples around zero
+// instead).  Sample s = 0 .. samples - 1 draws a displacement uniformly from
+// the square [-r, r]^2, r = max(radius >> s, 1): Philox4x32-10(ctr = {pixel,
+// iteration, pair, 1 + s}, key = seed) words 0 and 1 give d = floor(word *
+// (2r + 1) / 2^32) - r (counter word 3 is 0 in the random search, so the two
+// streams never share a counter).  The displacement is added to the current
+// offset, clamped to the frame like a random-search candidate and inserted
+// (skip a repeat of the current offset, accept a strictly lower cost), so
+// each sample searches around the best offset so far.  A pixel reads and
+// writes only its own entry: the field is updated in place.
+template <int NSP>
+__global__ void __launch_bounds__(kThreads) k_local_search(const uint32_t *__restrict__ F1,
+                                                           const uint32_t *__restrict__ F2, int sx, int sy,
+                                                           int strip_rows, int radius, int samples, uint32_t iteration,
+                                                           uint32_t pair, uint32_t key0, uint32_t key1,
+                                                           uint32_t *__restrict__ off, uint16_t *__restrict__ cost)
+{
+    const int gx = (sx + kThreads - 1) / kThreads;
+    int bx, y;
+    strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, strip_rows, bx, y);
+    const int x = bx * kThreads + threadIdx.x;
+    if (x >= sx) return;
+    const int i = y * sx + x;
+    uint32_t best = off[i];
+    int bcost = cost[i];
+    uint32_t f1[NSP];
+    load_desc<NSP>(F1, (uint32_t)i, f1);
+    for (int s = 0; s < samples; ++s) {
+        const int r = max(radius >> s, 1);
+        uint32_t c0 = (uint32_t)i, c1 = iteration, c2 = pair, c3 = 1u + (uint32_t)s;
+        philox4x32_10(c0, c1, c2, c3, key0, key1);
+        const uint32_t n = 2u * (uint32_t)r + 1u;
+        const int dx = (int)__umulhi(c0, n) - r, dy = (int)__umulhi(c1, n) - r;
+        const int nx = clampi(off_dx(best) + dx, -x, sx - 1 - x);
+        const int ny = clampi(off_dy(best) + dy, -y, sy - 1 - y);
+        const uint32_t cand = pack_off(nx, ny);
+        if (cand != best) {
+            uint32_t f2[NSP];
+            load_desc<NSP>(F2, (uint32_t)(i + ny * sx + nx), f2);
+            const int c = hamming<NSP>(f1, f2);
+            if (c < bcost) {
+                bcost = c;
+                best = cand;
+            }
+        }
+    }
+    off[i] = best;
+    cost[i] = (uint16_t)bcost;
+}
+
 // ---------------------------------------------------------------------------
 // down-sampling: Gaussian (horizontal then vertical, as filterConst twice)
-// evaluated only at the nearest-neighbour sample point, then scaled.
+// evaluated only at the nearest-neighbour sample point, then scaled.  Every
+// product and sum is rounded on its own (contraction into fused multiply-adds
+// is off in these functions), in the
+// filter's order (rows j, then taps i within a row), so the result is the
+// same float as the separable filter's: the trilateral filter's weights
+// depend on flow differences, and a 1-ulp change of a 60-pixel flow moves an
+// output whose weight sum sits at the 1e-5 floor by up to 4e-3 pixels.
 
 __device__ __forceinline__ void ds_direct_tile(const uint32_t *__restrict__ off, const uint32_t *__restrict__ nn_x,
                                                const uint32_t *__restrict__ nn_y, const float *__restrict__ gauss,
                                                int P, int sx, int sy, int W, int H, float scx, float scy,
                                                int identity, float *__restrict__ U, float *__restrict__ V, int b)
 {
+#pragma clang fp contract(off)
     const int gx = (W + kThreads - 1) / kThreads;
     const int x = (b % gx) * kThreads + threadIdx.x, y = b / gx;
     if (x >= W) return;
@@ -1126,6 +1184,7 @@ __device__ __forceinline__ void ds_tile(const uint32_t *__restrict__ off, const 
                                         int sx, int sy, int W, int H, float scx, float scy, float *__restrict__ U,
                                         float *__restrict__ V, int blk, uint32_t *win)
 {
+#pragma clang fp contract(off)
     const int gxb = (W + kDsTileW - 1) / kDsTileW;
     const int X0 = (blk % gxb) * kDsTileW, Y0 = (blk / gxb) * kDsTileH;
     const int P2 = P / 2;
@@ -1864,6 +1923,18 @@ hipError_t launch_init_proposal(const Geometry &g, const uint32_t *F1, const uin
         BM_NSP_DISPATCH(g.NSP, BM_INIT_PROPOSAL(false));
     }
 #undef BM_INIT_PROPOSAL
+    return hipGetLastError();
+}
+
+hipError_t launch_local_search(const Geometry &g, const uint32_t *F1, const uint32_t *F2, int radius, int samples,
+                               uint32_t iteration, uint32_t pair, uint32_t *off, uint16_t *cost, hipStream_t st)
+{
+    if (samples <= 0) return hipSuccess;
+    if (radius < 1) return hipErrorInvalidValue;
+    const dim3 grid(blocks_rows(g.sx, g.sy));
+    BM_NSP_DISPATCH(g.NSP, hipLaunchKernelGGL((k_local_search<NSP>), grid, dim3(kThreads), 0, st, F1, F2, g.sx, g.sy,
+                                              g.strip_rows, radius, samples, iteration, pair, (uint32_t)g.seed,
+                                              (uint32_t)(g.seed >> 32), off, cost));
     return hipGetLastError();
 }
 

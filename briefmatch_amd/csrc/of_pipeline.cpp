@@ -3,6 +3,7 @@
 
 #include <strings.h>
 
+#include <algorithm>
 #include <cctype>
 #include <cerrno>
 #include <cmath>
@@ -34,7 +35,7 @@ bool has_extension(const std::string &f, const char *ext)
 OFPipeline::OFPipeline()
     : m_doWriteMOT(false), m_doWriteFLO(false), m_doWriteLDR(false), m_verbose(false), m_doPrintError(false),
       m_havePrev(false), m_endFrame(0), m_frame(0), m_stepFrame(1), m_startFrame(0), m_sx(0), m_sy(0), m_sx_in(0),
-      m_sy_in(0), m_c(0), m_seed(0), m_upsampler("bilinear"), m_net(nullptr), m_bm(nullptr), m_bmP(nullptr), m_timerCtx(nullptr),
+      m_sy_in(0), m_c(0), m_seed(0), m_upsampler("bilinear"), m_lsRadius(0), m_lsSamples(0), m_net(nullptr), m_bm(nullptr), m_bmP(nullptr), m_timerCtx(nullptr),
       dev_rgb(nullptr), dev_g8(nullptr), dev_rgbPrev(nullptr), m_cur(0)
 {
     dev_Lin[0] = dev_Lin[1] = dev_L[0] = dev_L[1] = nullptr;
@@ -160,6 +161,9 @@ bool OFPipeline::setParams(int argc, char *argv[])
          &m_upsampler, 0, 0},
         {"--proposal", "-pp", "Learned-proposal weights (.bnet): every pair through the proposal-seeded path",
          Option::STR, &m_proposal, 0, 0},
+        {"--local-search", "-ls",
+         "With --proposal: RADIUS:SAMPLES local search around the seeds (radius in input pixels)", Option::STR,
+         &m_localSearch, 0, 0},
     };
     auto usage = [&]() {
         std::printf("briefmatch -- computes the optical flow of a sequence of frames on an AMD MI355X\n\n"
@@ -216,6 +220,21 @@ bool OFPipeline::setParams(int argc, char *argv[])
                                       "and %% escapes only.");
     if (m_upsampler != "bilinear" && m_upsampler != "lanczos3")
         throw BriefMatchException("Argument '--upsampler' expects bilinear or lanczos3, got '" + m_upsampler + "'");
+    if (!m_localSearch.empty()) {
+        char *e = nullptr;
+        const char *c = m_localSearch.c_str();
+        const double r = std::strtod(c, &e);
+        const bool sep = e != c && *e == ':';
+        char *e2 = nullptr;
+        const long n = sep ? std::strtol(e + 1, &e2, 10) : -1;
+        if (!sep || e2 == e + 1 || *e2 || !(r > 0 && r <= 1024) || n < 1 || n > BM_LOCAL_SEARCH_MAX_SAMPLES)
+            throw BriefMatchException("Argument '--local-search' expects RADIUS:SAMPLES with a radius in (0, 1024] "
+                                      "input pixels and 1 to " + std::to_string(BM_LOCAL_SEARCH_MAX_SAMPLES) +
+                                      " samples, got '" + m_localSearch + "'");
+        if (m_proposal.empty()) throw BriefMatchException("Argument '--local-search' needs '--proposal'");
+        m_lsRadius = (float)r;
+        m_lsSamples = (int)n;
+    }
     m_seed = seed;
     if (!m_output.empty()) {
         if (has_extension(m_output, ".flo"))
@@ -280,6 +299,13 @@ void OFPipeline::setup()
             throw BriefMatchException(std::string("Cannot load the proposal weights.\n\t") + bm_net_last_error(m_net));
         if (m_upsampler == "lanczos3" && bm_set_upsampler(m_timerCtx, BM_UPSAMPLE_LANCZOS3) != BM_OK)
             throw BriefMatchException(bm_last_error(m_timerCtx));
+        if (m_lsSamples > 0) {
+            // the radius in matched pixels: input pixels times the larger up-sampling factor
+            const double f = std::max((double)m_sx / m_sx_in, (double)m_sy / m_sy_in);
+            const int r = std::max(1, std::min(BM_LOCAL_SEARCH_MAX_RADIUS, (int)std::lround(m_lsRadius * f)));
+            if (bm_set_local_search(m_timerCtx, r, m_lsSamples) != BM_OK)
+                throw BriefMatchException(bm_last_error(m_timerCtx));
+        }
     }
     const size_t Win = (size_t)m_sx_in * m_sy_in, M = (size_t)m_sx * m_sy;
     void *p = nullptr;

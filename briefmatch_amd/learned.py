@@ -243,25 +243,49 @@ def validation_loss(net, ds, rows, batch=32):
     return tot / max(n, 1)
 
 
-def evaluate(net, workload="c2_1920x1080_u2.9_n128", seeds=(0, 1, 2, 3), fewer=1, log=print):
+def local_search_radius(params, radius_px):
+    """A local-search radius in input pixels as matched pixels: times the
+    larger up-sampling factor, rounded half up, at least 1 (as the command
+    line's --local-search)."""
+    return max(1, int(math.floor(radius_px * max(params.sx / params.sx_in, params.sy / params.sy_in) + 0.5)))
+
+
+def evaluate(net, workload="c2_1920x1080_u2.9_n128", seeds=(0, 1, 2, 3), fewer=1, jf_max=None, local_search=None,
+             log=print):
     """EPE (all pixels) of plain(default iterations), plain(fewer),
     learned(fewer), learned(default iterations) and, as the method without
     the trained network, the seeded path with an all-zero network (its
     proposal is then the cost-volume argmin alone) with fewer iterations, on
-    seeded pairs of a benchmark workload."""
+    seeded pairs of a benchmark workload.  fewer = 0 runs the seeded
+    initialisation and its random search only: the flow is then the seeds'
+    alone.  jf_max (default: the workload's) is the learned and zero-network
+    runs' jump range; local_search = (radius in input pixels, samples) turns
+    on the local search around their seeds."""
     wl = synth.WORKLOADS[workload]
     p = derive_params(wl["W"], wl["H"], wl["up"], wl["up"], N=wl["N"])
     default_it = p.iterations
     pf = p.copy()
     pf.iterations = fewer
+    ps = p.copy()
+    if jf_max is not None:
+        pf.jfMax = jf_max
+        ps.jfMax = jf_max
+    pp = p.copy()
+    pp.iterations = fewer
     c_plain = Context(p, seed=0)
+    c_plain_few = Context(pp, seed=0)
     c_few = Context(pf, seed=0)
+    c_learn = Context(ps, seed=0)
+    if local_search:
+        r = local_search_radius(pf, local_search[0])
+        c_few.set_local_search(r, local_search[1])
+        c_learn.set_local_search(r, local_search[1])
     zero = Net()
     res = {"plain": [], "plain_fewer": [], "learned_fewer": [], "learned": [], "argmin_fewer": []}
     for s in seeds:
         pair = synth.workload_pair(workload, seed=s)
-        for key, ctx, learned in (("plain", c_plain, None), ("plain_fewer", c_few, None),
-                                  ("learned_fewer", c_few, net), ("learned", c_plain, net),
+        for key, ctx, learned in (("plain", c_plain, None), ("plain_fewer", c_plain_few, None),
+                                  ("learned_fewer", c_few, net), ("learned", c_learn, net),
                                   ("argmin_fewer", c_few, zero)):
             if learned is not None:
                 U, V = ctx.flow_numpy_learned(learned, pair.frame1, pair.frame2)
@@ -273,8 +297,12 @@ def evaluate(net, workload="c2_1920x1080_u2.9_n128", seeds=(0, 1, 2, 3), fewer=1
                 "argmin only(%d) %.4f"
                 % (s, default_it, res["plain"][-1], fewer, res["plain_fewer"][-1], fewer, res["learned_fewer"][-1],
                    default_it, res["learned"][-1], fewer, res["argmin_fewer"][-1]))
+    for c in (c_plain, c_plain_few, c_few, c_learn):
+        c.close()
+    zero.close()
     out = {k: float(np.mean(v)) for k, v in res.items()}
-    out.update(iterations=default_it, fewer=fewer, seeds=list(seeds), workload=workload)
+    out.update(iterations=default_it, fewer=fewer, seeds=list(seeds), workload=workload, jf_max=pf.jfMax,
+               local_search=list(local_search) if local_search else None)
     return out
 
 
@@ -306,6 +334,12 @@ def main(argv=None):
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
     e.add_argument("--seeds", type=int, default=4)
     e.add_argument("--workload", default="c2_1920x1080_u2.9_n128", choices=sorted(synth.WORKLOADS))
+    e.add_argument("--fewer", type=int, default=1,
+                   help="propagation iterations of the learned, zero-network and shortened plain runs (0: the "
+                        "seeded initialisation and its random search only)")
+    e.add_argument("--jf-max", type=int, default=None, help="jump range of the learned and zero-network runs")
+    e.add_argument("--local-search", default=None, metavar="RADIUS:SAMPLES",
+                   help="local search around the seeds (radius in input pixels)")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     if a.cmd == "train":
@@ -326,7 +360,12 @@ def main(argv=None):
                              log=log)))
     else:
         net = Net(path=a.weights)
-        log(str(evaluate(net, workload=a.workload, seeds=tuple(range(a.seeds)), log=log)))
+        ls = None
+        if a.local_search:
+            r, n = a.local_search.split(":")
+            ls = (float(r), int(n))
+        log(str(evaluate(net, workload=a.workload, seeds=tuple(range(a.seeds)), fewer=a.fewer, jf_max=a.jf_max,
+                         local_search=ls, log=log)))
     return 0
 
 

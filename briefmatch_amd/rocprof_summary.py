@@ -33,7 +33,7 @@ def short_name(name):
 
 
 STAGE_OF = OrderedDict([("k_gray", "gray"), ("k_upsample", "upsample"), ("k_brief", "brief"),
-                        ("k_init_proposal", "proposal"), ("k_init", "init"), ("k_level<", "level"), ("k_level_lanes<", "level"), ("k_random_search", "random_search"),
+                        ("k_init_proposal", "proposal"), ("k_local_search", "local_search"), ("k_init", "init"), ("k_level<", "level"), ("k_level_lanes<", "level"), ("k_random_search", "random_search"),
                         ("k_downsample", "downsample"), ("k_median", "median"), ("k_trilateral", "trilateral"),
                         ("k_copy", "copy"), ("k_features", "proposal"),
                         ("k_block_sums", "proposal"), ("k_far_search", "proposal"), ("k_conv", "proposal"),

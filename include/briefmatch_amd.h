@@ -388,6 +388,26 @@ int bm_init_proposal(bm_ctx *ctx, const uint32_t *F1_dev, const uint32_t *F2_dev
  * when the frames are at least 16x16 pixels. */
 int bm_flow_pair_learned(bm_ctx *ctx, bm_net *net, const uint8_t *rgb1_dev, const uint8_t *rgb2_dev, uint32_t pair,
                          float *U_dev, float *V_dev);
+/* Local search around the seeds (no counterpart in the reference, whose random
+ * search samples around offset zero; PatchMatch's search around the current
+ * offset with a shrinking radius).  With samples > 0, bm_flow_pair_learned
+ * runs it after the seeded initialisation's random search, before the
+ * propagation iterations: per matched pixel, sample s = 0 .. samples - 1 is
+ * the running best offset plus a displacement uniform in [-r, r]^2, r =
+ * max(radius >> s, 1) matched pixels, from Philox4x32-10(ctr = {pixel,
+ * iteration, pair, 1 + s}, key = seed) (d = floor(word * (2r + 1) / 2^32) -
+ * r, words 0 and 1), clamped to the frame, inserted like a random-search
+ * candidate.  bm_flow_pair_learned passes iteration = `iterations`, the seeded
+ * initialisation's random-search index.  0 samples (the default) turns it
+ * off. */
+#define BM_LOCAL_SEARCH_MAX_SAMPLES 16
+#define BM_LOCAL_SEARCH_MAX_RADIUS 4096
+int bm_set_local_search(bm_ctx *ctx, int radius, int samples);
+int bm_get_local_search(const bm_ctx *ctx, int *radius, int *samples);
+/* The local search alone on a field (off/cost updated in place) with the
+ * context's radius and samples. */
+int bm_local_search(bm_ctx *ctx, const uint32_t *F1_dev, const uint32_t *F2_dev, uint32_t *off_dev,
+                    uint16_t *cost_dev, uint32_t iteration, uint32_t pair);
 
 /* ---- timing (CudaTimer, util_cuda.h:131-141) ----------------------------- */
 
@@ -412,7 +432,8 @@ int bm_timer_stop(bm_ctx *ctx, float *ms);
 #define BM_STAGE_TRILATERAL 9
 #define BM_STAGE_COPY 10
 #define BM_STAGE_PROPOSAL 11      /* learned proposal: features, network, seeded init */
-#define BM_STAGE_COUNT 12
+#define BM_STAGE_LOCAL_SEARCH 12  /* learned path: local search around the seeds */
+#define BM_STAGE_COUNT 13
 int bm_profile_enable(bm_ctx *ctx, int on);
 int bm_profile_read(bm_ctx *ctx, double *ms, unsigned *launches);
 

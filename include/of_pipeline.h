@@ -40,6 +40,9 @@ private:
     uint64_t m_seed;
     std::string m_upsampler;  // "bilinear" or "lanczos3"
     std::string m_proposal;   // weights file of the learned proposal, or empty
+    std::string m_localSearch;  // --local-search RADIUS:SAMPLES (learned path)
+    float m_lsRadius;         // its radius in input pixels
+    int m_lsSamples;          // its samples (0: off)
     bm_net *m_net;
     std::string m_input, m_flowGT, m_output;
 

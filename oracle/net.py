@@ -264,6 +264,62 @@ def init_proposal(F1, F2, coff, W, H, first=True, prev_off=None):
     return bo.astype(np.uint32), best.astype(np.uint16)
 
 
+def philox_np(c0, c1, c2, c3, k0, k1):
+    """Philox4x32-10 (Salmon et al., SC'11) on arrays of counters: ten rounds
+    of the two 32x32 -> 64-bit products with the published multipliers and
+    Weyl key increments; the element-wise equal of oracle.philox (checked
+    against it on random counters in tests/test_net_oracle.py)."""
+    M0, M1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57)
+    W0, W1 = 0x9E3779B9, 0xBB67AE85
+    mask = np.uint64(0xFFFFFFFF)
+    c = [np.asarray(v, np.uint64) & mask for v in (c0, c1, c2, c3)]
+    c = np.broadcast_arrays(*c)
+    c0, c1, c2, c3 = (v.copy() for v in c)
+    k0, k1 = int(k0) & 0xFFFFFFFF, int(k1) & 0xFFFFFFFF
+    for _ in range(10):
+        p0 = M0 * c0
+        p1 = M1 * c2
+        n0 = (p1 >> np.uint64(32)) ^ c1 ^ np.uint64(k0)
+        n2 = (p0 >> np.uint64(32)) ^ c3 ^ np.uint64(k1)
+        c1 = p1 & mask
+        c3 = p0 & mask
+        c0, c2 = n0, n2
+        k0 = (k0 + W0) & 0xFFFFFFFF
+        k1 = (k1 + W1) & 0xFFFFFFFF
+    return tuple(v.astype(np.uint32) for v in (c0, c1, c2, c3))
+
+
+def local_search(F1, F2, off, cost, radius, samples, iteration, pair, seed):
+    """Local search around each pixel's offset (the learned path's refinement
+    of its seeds); returns (off, cost).  Sample s = 0 .. samples - 1: r =
+    max(radius >> s, 1); Philox4x32-10(ctr = {pixel, iteration, pair, 1 + s},
+    key = (seed low, seed high)) words 0 and 1 give d = (word * (2r + 1)) >>
+    32 - r; the candidate is the running best plus (dx, dy), clamped to the
+    frame per axis; a repeat of the running best is skipped, a strictly lower
+    Hamming cost is taken."""
+    sy, sx = F1.shape[:2]
+    yy, xx = np.mgrid[0:sy, 0:sx].astype(np.int64)
+    idx = (yy * sx + xx).astype(np.uint64)
+    bo = np.asarray(off, np.uint32).copy()
+    best = np.asarray(cost, np.int64).copy()
+    k0, k1 = int(seed) & 0xFFFFFFFF, (int(seed) >> 32) & 0xFFFFFFFF
+    for s in range(samples):
+        r = max(radius >> s, 1)
+        w0, w1, _, _ = philox_np(idx, iteration, pair, 1 + s, k0, k1)
+        n = np.uint64(2 * r + 1)
+        dx = ((w0.astype(np.uint64) * n) >> np.uint64(32)).astype(np.int64) - r
+        dy = ((w1.astype(np.uint64) * n) >> np.uint64(32)).astype(np.int64) - r
+        bx, by = _unpack(bo)
+        nx = np.clip(bx + dx, -xx, sx - 1 - xx)
+        ny = np.clip(by + dy, -yy, sy - 1 - yy)
+        po = _pack(nx, ny)
+        c = _popcount_rows(F1 ^ F2[yy + ny, xx + nx])
+        take = (po != bo) & (c < best)
+        best = np.where(take, c, best)
+        bo = np.where(take, po, bo)
+    return bo.astype(np.uint32), best.astype(np.uint16)
+
+
 # ---------------------------------------------------------------------------
 # the network
 

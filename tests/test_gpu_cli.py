@@ -99,6 +99,29 @@ def test_cli_proposal_equals_flow_pair_learned(tmp_path):
         assert np.array_equal(U, pU) and np.array_equal(V, pV), (a, b)
 
 
+def test_cli_proposal_local_search_equals_flow_pair_learned(tmp_path):
+    """--proposal with --local-search 16:6 and --jf-max 8 (the benchmark's
+    learned configuration) equals bm_flow_pair_learned with the local search
+    radius 16 input pixels times the up-sampling factor."""
+    from briefmatch_amd import learned
+    if not os.path.exists(learned.DEFAULT_WEIGHTS):
+        pytest.skip("no trained proposal weights")
+    _run(["-i", os.path.join(GOLD, "frame%02d.png"), "-f", "10:11", "-ux", "2.9", "-uy", "2.9", "-fl", "128",
+          "-it", "1", "-jm", "8", "--proposal", learned.DEFAULT_WEIGHTS, "--local-search", "16:6",
+          "-o", str(tmp_path / "flow%02d.flo")], cwd=tmp_path)
+    p = bm.derive_params(584, 388, 2.9, 2.9, N=128)
+    p.iterations = 1
+    p.jfMax = 8
+    ctx = bm.Context(p, seed=0)
+    ctx.set_local_search(learned.local_search_radius(p, 16.0), 6)
+    assert ctx.local_search_config() == (46, 6)
+    net = bm.Net(path=learned.DEFAULT_WEIGHTS)
+    frames = [bio.read_image(os.path.join(GOLD, "frame%02d.png" % k)) for k in (10, 11)]
+    U, V = bio.read_flo(str(tmp_path / "flow11.flo"))
+    pU, pV = ctx.flow_numpy_learned(net, frames[0], frames[1], pair=0)
+    assert np.array_equal(U, pU) and np.array_equal(V, pV)
+
+
 def test_cli_proposal_missing_weights(tmp_path):
     r = subprocess.run([bio.CLI_PATH, "-i", os.path.join(GOLD, "frame%02d.png"), "-f", "10:11",
                         "--proposal", str(tmp_path / "none.bnet"), "-o", str(tmp_path / "f%02d.flo")],

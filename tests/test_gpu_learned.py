@@ -6,9 +6,10 @@ forward output and gradients are compared with the oracle's float64 network
 evaluated with the same BF16 rounding points: the device accumulates in FP32
 in another order, so they agree to a relative 2e-2 per layer (gradients) and
 2e-2 of the output range (forward).  Training lowers the loss and is
-bit-reproducible.  With the committed weights, the learned path with one
-iteration reaches the plain method's two-iteration EPE within 2% and beats
-the plain method with one iteration."""
+bit-reproducible.  The local search around the seeds matches the oracle bit
+for bit.  With the committed weights, the learned path with a local search and
+4 level launches beats the plain method's two-iteration EPE at 1080p, and the
+trained network beats the all-zero network."""
 import os
 
 import numpy as np
@@ -226,25 +227,71 @@ def test_training_lowers_loss_and_is_reproducible():
     assert np.array_equal(nets[0].weights(), nets[1].weights())
 
 
-@pytest.mark.parametrize("motion,W,H,ux,uy", [("c1", 320, 240, 1.5, 1.5), ("far", 320, 240, 1.5, 1.5),
-                                             ("small", 56, 14, 1.5, 1.5), ("c1", 200, 150, 0.0, 0.0)])
-def test_learned_path_equals_composed_oracle(motion, W, H, ux, uy):
+@pytest.mark.parametrize("W,H,up,N,radius,samples", [(320, 240, 1.5, 32, 16, 5), (200, 136, 2.9, 128, 24, 6),
+                                                      (130, 75, 1.0, 512, 3, 4), (97, 43, 2.2, 64, 200, 16)])
+def test_local_search_bit_exact(W, H, up, N, radius, samples):
+    """bm_local_search on a converged field of a real pair (and on its zero
+    field, where most samples are taken) against the oracle; the last case's
+    radius exceeds the frame, so the clamp acts on most samples."""
+    say("local search", W, H, up, N, radius, samples)
+    ctx, pr, _ = _flow_ctx(W, H, up, N)
+    ctx.set_local_search(radius, samples)
+    assert ctx.local_search_config() == (radius, samples)
+    g8, F, Fs, gs = _frames(ctx)
+    oF1, oF2 = Fs[0][..., :ctx.NS].copy(), Fs[1][..., :ctx.NS].copy()
+    off0, cost0 = ctx.field()
+    zo, zc = oracle.init(oF1, oF2, first=True)
+    for k, (o_in, c_in) in enumerate(((off0, cost0), (zo, zc))):
+        off = bm.DeviceBuffer(array=np.ascontiguousarray(o_in))
+        cost = bm.DeviceBuffer(array=np.ascontiguousarray(c_in))
+        ctx.local_search(F[0], F[1], off, cost, iteration=2 + k, pair=9)
+        ctx.sync()
+        eo, ec = onet.local_search(oF1, oF2, o_in, c_in, radius, samples, 2 + k, 9, 3)
+        go = off.download((ctx.sy, ctx.sx), np.uint32)
+        gc = cost.download((ctx.sy, ctx.sx), np.uint16)
+        assert np.array_equal(go, eo) and np.array_equal(gc, ec)
+        if k == 1:
+            assert (go != zo).mean() > 0.02     # the search moved pixels off zero
+    ctx.close()
+
+
+def test_local_search_errors():
+    p = bm.derive_params(64, 48, 1.0, 1.0, N=32)
+    ctx = bm.Context(p, seed=1)
+    assert ctx.local_search_config() == (0, 0)
+    for r, n in ((0, 3), (5, -1), (5, 17), (4097, 2)):
+        with pytest.raises(bm.BriefMatchError):
+            ctx.set_local_search(r, n)
+    ctx.set_local_search(0, 0)
+    ctx.set_local_search(4096, 16)
+    assert ctx.local_search_config() == (4096, 16)
+    ctx.close()
+
+
+@pytest.mark.parametrize("motion,W,H,ux,uy,ls", [("c1", 320, 240, 1.5, 1.5, None), ("far", 320, 240, 1.5, 1.5, None),
+                                                ("small", 56, 14, 1.5, 1.5, None), ("c1", 200, 150, 0.0, 0.0, None),
+                                                ("c1", 320, 240, 1.5, 1.5, (12, 5)),
+                                                ("far", 320, 240, 0.0, 0.0, (30, 6))])
+def test_learned_path_equals_composed_oracle(motion, W, H, ux, uy, ls):
     """bm_flow_pair_learned = features + far match -> network -> offsets ->
     seeded init -> random search -> propagation + random search ->
     refinement, each stage on the oracle except the network (its device output
     is the oracle's input); "far": regions moving up to 60 pixels, where the
     far match overrides the network in many cells; "small": 56 x 14 frames,
     one coarse row and no 16-pixel block row, so there is no far match; the
-    last case runs the reference's default anisotropic up-sampling (2.740 x
+    fourth case runs the reference's default anisotropic up-sampling (2.740 x
     3.052, of_pipeline.cpp:178-181), where the coarse-to-matched offset scales
-    differ per axis."""
-    say("learned path", motion, W, H, ux, uy)
+    differ per axis; the last two add the local search around the seeds
+    (radius, samples) after the seeded initialisation's random search."""
+    say("learned path", motion, W, H, ux, uy, ls)
     N = 32
     p = bm.derive_params(W, H, ux, uy, N=N)
     if (ux, uy) == (0.0, 0.0):
         assert p.sx * H != p.sy * W  # different scale factors per axis
     p.iterations = 1
     ctx = bm.Context(p, seed=11)
+    if ls:
+        ctx.set_local_search(*ls)
     path = learned.DEFAULT_WEIGHTS
     net = bm.Net(path=path) if os.path.exists(path) else bm.Net(weights=_random_weights(1))
     if motion == "c1":
@@ -274,6 +321,8 @@ def test_learned_path_equals_composed_oracle(motion, W, H, ux, uy):
     oF1, oF2 = Fs[0][..., :ctx.NS].copy(), Fs[1][..., :ctx.NS].copy()
     eo, ec = onet.init_proposal(oF1, oF2, coff, W, H)
     eo, ec = oracle.random_search(oF1, oF2, 1, 6, 11, eo, ec)     # after the seeded init: index = iterations
+    if ls:
+        eo, ec = onet.local_search(oF1, oF2, eo, ec, ls[0], ls[1], 1, 6, 11)
     eo, ec = oracle.propagate(oF1, oF2, -1, p.jfMax, True, eo, ec)
     eo, ec = oracle.random_search(oF1, oF2, 0, 6, 11, eo, ec)
     assert np.array_equal(off, eo) and np.array_equal(cost, ec)
@@ -289,30 +338,62 @@ def test_learned_path_equals_composed_oracle(motion, W, H, ux, uy):
 
 @pytest.mark.skipif(not os.path.exists(learned.DEFAULT_WEIGHTS), reason="no trained proposal weights")
 def test_trained_proposal_gain():
-    """What the learned path gains, with one propagation iteration, over the
-    plain method and over the same seeded path with an all-zero network (its
-    proposal is then the cost-volume argmin alone), on the 1920x1080 / u2.9 /
-    128-bit pairs (seeds 0-7) and the 1280x720 / u4.9 / 512-bit pairs moving up
-    to 60 px (seeds 0-3).  Measured with the committed weights: EPE 0.772
-    against 1.224 (plain, two iterations), 11.58 (plain, one) and 0.769 (zero
-    network) at 1080p; 4.90 against 30.4, 34.4 and 4.93 on the large-motion
-    pairs.  The gain over the plain method comes from the seeded
-    initialisation; the network itself gains 0.7 % on the large-motion pairs
-    and loses 0.4 % at 1080p.  Asserted: the target (within 2 % of plain with
-    two iterations), the gain over the zero network on the large-motion pairs,
-    and at most 1 % loss against it at 1080p."""
+    """What the learned path gains, and what the trained network itself gains
+    over the same path with an all-zero network (its proposal is then the
+    cost-volume argmin and far match alone), in the configuration the
+    benchmark reports: the seeded initialisation, a local search of 6 samples
+    around the seeds (radius 16 input pixels), one propagation iteration over
+    the jump range 8 (4 level launches; the plain method executes 9).
+    Measured with the committed weights, mean EPE: 1920x1080 / u2.9 / 128-bit
+    pairs (seeds 0-7) 1.013 against 1.224 (plain, two iterations) and 1.173
+    (zero network); 1280x720 / u4.9 / 512-bit pairs moving up to 60 px (seeds
+    0-3) 4.71 against 30.4 and 6.11; 320x240 / u1.5 / 32-bit pairs (seeds 0-3)
+    0.646 against 0.361 and 0.876 (motion below 8 px: the plain method's zero
+    start is already close, and the short jump range costs more than the seeds
+    give).  Asserted: the target on the 1080p pairs (within 2 % of the plain
+    method; here below it), the large-motion gain, and the network's own gain
+    over the zero network on all three workloads."""
     net = bm.Net(path=learned.DEFAULT_WEIGHTS)
     log = lambda m: print(m, flush=True)
+    cfg = dict(fewer=1, jf_max=8, local_search=(16.0, 6), log=log)
     say("learned gain c2")
-    c2 = learned.evaluate(net, seeds=tuple(range(8)), fewer=1, log=log)
+    c2 = learned.evaluate(net, seeds=tuple(range(8)), **cfg)
     say("learned gain c3")
-    c3 = learned.evaluate(net, workload="c3_1280x720_u4.9_n512", seeds=(0, 1, 2, 3), fewer=1, log=log)
-    print("[gain] c2", c2, "\n[gain] c3", c3, flush=True)
+    c3 = learned.evaluate(net, workload="c3_1280x720_u4.9_n512", seeds=(0, 1, 2, 3), **cfg)
+    say("learned gain c1")
+    c1 = learned.evaluate(net, workload="c1_320x240_u1.5_n32", seeds=(0, 1, 2, 3), **cfg)
+    print("[gain] c2", c2, "\n[gain] c3", c3, "\n[gain] c1", c1, flush=True)
     assert c2["learned_fewer"] <= 1.02 * c2["plain"]
     assert c2["learned_fewer"] < 0.1 * c2["plain_fewer"]
-    assert c2["learned_fewer"] <= 1.01 * c2["argmin_fewer"]
+    assert c2["learned_fewer"] <= 0.92 * c2["argmin_fewer"]
     assert c3["learned_fewer"] <= 0.2 * c3["plain"]
-    assert c3["learned_fewer"] < c3["argmin_fewer"]
+    assert c3["learned_fewer"] <= 0.85 * c3["argmin_fewer"]
+    assert c1["learned_fewer"] <= 0.85 * c1["argmin_fewer"]
+    net.close()
+
+
+@pytest.mark.skipif(not os.path.exists(learned.DEFAULT_WEIGHTS), reason="no trained proposal weights")
+def test_network_gain_without_propagation():
+    """What the trained network itself gains over the all-zero network (the
+    method without it: the seeded initialisation with the cost-volume argmin
+    and far match alone) when no propagation iteration follows the seeded
+    initialisation, so the flow is set by the seeds and one random search.
+    Measured with the committed weights: mean EPE 3.72 against 7.33 on the
+    1920x1080 / u2.9 / 128-bit pairs (seeds 0-7), 12.2 against 18.0 on the
+    1280x720 pairs moving up to 60 px (seeds 0-3) and 1.66 against 2.46 on the
+    320x240 pairs (seeds 0-3).  With one iteration the propagation spreads
+    either set of seeds over the regions and the two are level
+    (test_trained_proposal_gain)."""
+    net = bm.Net(path=learned.DEFAULT_WEIGHTS)
+    log = lambda m: print(m, flush=True)
+    bounds = (("c2_1920x1080_u2.9_n128", tuple(range(8)), 0.6), ("c3_1280x720_u4.9_n512", (0, 1, 2, 3), 0.8),
+              ("c1_320x240_u1.5_n32", (0, 1, 2, 3), 0.8))
+    for workload, seeds, bound in bounds:
+        say("network gain without propagation", workload)
+        r = learned.evaluate(net, workload=workload, seeds=seeds, fewer=0, log=log)
+        print("[gain0]", workload, r, flush=True)
+        assert r["learned_fewer"] <= bound * r["argmin_fewer"], workload
+    net.close()
 
 
 def _staged_learned_epe(ctx, net, p, pr, with_far):

@@ -123,6 +123,11 @@ def test_cli_parameter_errors(tmp_path):
         (["-i", os.path.join(GOLD, "frame%02lu.png"), "-f", "10:11"], "Frame pattern"),
         (["-i", os.path.join(GOLD, "frame%02lld.png"), "-f", "10:11"], "Frame pattern"),
         (["-i", os.path.join(GOLD, "frame%02hhhd.png"), "-f", "10:11"], "Frame pattern"),
+        (["-i", frames, "-f", "10:11", "-ls", "16:6"], "needs '--proposal'"),
+        (["-i", frames, "-f", "10:11", "-pp", "w.bnet", "-ls", "16"], "expects RADIUS:SAMPLES"),
+        (["-i", frames, "-f", "10:11", "-pp", "w.bnet", "-ls", "16:0"], "expects RADIUS:SAMPLES"),
+        (["-i", frames, "-f", "10:11", "-pp", "w.bnet", "-ls", "0:4"], "expects RADIUS:SAMPLES"),
+        (["-i", frames, "-f", "10:11", "-pp", "w.bnet", "-ls", "16:6x"], "expects RADIUS:SAMPLES"),
     ]
     for args, msg in cases:
         r = _cli(*args)

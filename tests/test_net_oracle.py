@@ -339,3 +339,97 @@ def test_training_families():
                               learned.train_pair(s, 96, 64, True, "rects")[2]) for s in range(1000, 1004))
     with pytest.raises(ValueError):
         learned.train_pair(1, family="other")
+
+
+def test_philox_np_equals_oracle_philox():
+    """The vectorised Philox of the local search against the oracle's scalar
+    one (itself pinned by the Random123 known-answer vectors)."""
+    import oracle
+    rng = np.random.default_rng(12)
+    ctr = rng.integers(0, 2 ** 32, (64, 4), dtype=np.uint64).astype(np.uint32)
+    key = rng.integers(0, 2 ** 32, (64, 2), dtype=np.uint64).astype(np.uint32)
+    ctr[0] = [0, 0, 0, 0]
+    ctr[1] = [0xFFFFFFFF] * 4
+    for c, k in zip(ctr, key):
+        got = onet.philox_np(c[0], c[1], c[2], c[3], k[0], k[1])
+        assert [int(v) for v in got] == [int(v) for v in oracle.philox(c, k)]
+
+
+def _local_search_loops(F1, F2, off, cost, radius, samples, iteration, pair, seed):
+    import oracle
+    sy, sx = F1.shape[:2]
+    key = np.array([seed & 0xFFFFFFFF, seed >> 32], np.uint32)
+    off, cost = off.copy(), cost.astype(np.int64)
+    pop = lambda a: int(sum(bin(int(v)).count("1") for v in a))
+    for y in range(sy):
+        for x in range(sx):
+            bo, best = int(off[y, x]), int(cost[y, x])
+            for s in range(samples):
+                r = max(radius >> s, 1)
+                w = oracle.philox(np.array([y * sx + x, iteration, pair, 1 + s], np.uint32), key)
+                dx = (int(w[0]) * (2 * r + 1) >> 32) - r
+                dy = (int(w[1]) * (2 * r + 1) >> 32) - r
+                cx = ((bo & 0xFFFF) ^ 0x8000) - 0x8000
+                cy = ((bo >> 16) ^ 0x8000) - 0x8000
+                nx = min(max(cx + dx, -x), sx - 1 - x)
+                ny = min(max(cy + dy, -y), sy - 1 - y)
+                po = (nx & 0xFFFF) | ((ny & 0xFFFF) << 16)
+                if po == bo:
+                    continue
+                c = pop(F1[y, x] ^ F2[y + ny, x + nx])
+                if c < best:
+                    best, bo = c, po
+            off[y, x], cost[y, x] = bo, best
+    return off, cost.astype(np.uint16)
+
+
+@pytest.mark.parametrize("radius,samples", [(6, 4), (1, 3), (40, 7)])
+def test_local_search_matches_loops(radius, samples):
+    rng = np.random.default_rng(radius)
+    sx, sy = 37, 23
+    # few distinct descriptors so that candidates tie; offsets near the border
+    pal = rng.integers(0, 2 ** 32, (5, 2), dtype=np.uint64).astype(np.uint32)
+    F1 = pal[rng.integers(0, 5, (sy, sx))]
+    F2 = pal[rng.integers(0, 5, (sy, sx))]
+    yy, xx = np.mgrid[0:sy, 0:sx]
+    dx = np.clip(rng.integers(-9, 10, (sy, sx)), -xx, sx - 1 - xx)
+    dy = np.clip(rng.integers(-9, 10, (sy, sx)), -yy, sy - 1 - yy)
+    off = ((dx & 0xFFFF) | ((dy & 0xFFFF) << 16)).astype(np.uint32)
+    pop = np.vectorize(lambda v: bin(int(v)).count("1"))
+    cost = pop(F1 ^ F2[yy + dy, xx + dx]).sum(-1).astype(np.uint16)
+    seed = 0x1234567890AB
+    got = onet.local_search(F1, F2, off, cost, radius, samples, 3, 5, seed)
+    want = _local_search_loops(F1, F2, off, cost, radius, samples, 3, 5, seed)
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+
+
+def test_local_search_hit_rate_closed_form():
+    """Random 64-bit descriptors, frame 2 = frame 1 moved by (3, -2) (a torus,
+    so every pixel has its match), zero start offsets given cost 1: only the
+    exact offset (cost 0) is strictly cheaper, so a pixel's offset changes
+    only when a sample hits (3, -2) exactly.  From zero, sample s hits it with
+    probability 1 / (2 r_s + 1)^2 when max(|3|, |-2|) <= r_s, else 0; after a
+    hit the best is the translation and later samples cannot beat cost 0.
+    The share of pixels that find it matches 1 - prod(1 - p_s) within four
+    standard errors; the offsets found are exactly the translation."""
+    rng = np.random.default_rng(21)
+    sx, sy = 200, 150
+    F2 = rng.integers(0, 2 ** 32, (sy, sx, 2), dtype=np.uint64).astype(np.uint32)
+    F1 = np.roll(F2, shift=(2, -3), axis=(0, 1))      # F1[y, x] = F2[y - 2, x + 3]
+    off = np.zeros((sy, sx), np.uint32)
+    cost = np.ones((sy, sx), np.uint16)
+    radius, samples = 8, 5                           # r = 8, 4, 2, 1, 1
+    o, c = onet.local_search(F1, F2, off, cost, radius, samples, 0, 0, 77)
+    # interior pixels only: near the border the clamp changes the odds
+    inner = (slice(10, sy - 10), slice(10, sx - 10))
+    hit = o[inner] == ((3 & 0xFFFF) | ((-2 & 0xFFFF) << 16))
+    assert np.all(c[inner][hit] == 0)
+    p_miss = 1.0
+    for s in range(samples):
+        r = max(radius >> s, 1)
+        if r >= 3:
+            p_miss *= 1.0 - 1.0 / (2 * r + 1) ** 2
+    p = 1.0 - p_miss
+    n = hit.size
+    assert abs(hit.mean() - p) <= 4.0 * np.sqrt(p * (1 - p) / n)
+    assert np.all(o[inner][~hit] == 0) and np.all(c[inner][~hit] == 1)
