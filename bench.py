@@ -15,8 +15,11 @@ Prints one JSON line.  Besides the driver's fields it carries:
                 rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload;
                 "pipeline": the sum over stages of work at peak against the
                 measured step;
-  stages        per-stage event times and roofline fractions against each
-                stage's own ceilings (HBM, LDS, VALU issue, FP32);
+  stages        per-stage event times (a second pass of the K steps with HIP
+                events around every kernel) and roofline fractions against
+                each stage's own ceilings (HBM, LDS, VALU issue, FP32);
+  graph_replay  the same pair captured once as a hipGraph and replayed,
+                bit-exact against bm_flow_pair;
   streaming     sequence mode (one BRIEF per frame, temporal init), serial and
                 pipelined (each frame's refinement tail beside the next frame);
   learned       the same pair through the learned-proposal path (seeded
@@ -204,16 +207,23 @@ def measure_pair(a, bm, name, tune_cache=None):
     for _ in range(a.warmup):
         ctx.flow_pair(rgb1, rgb2, U, V)
     ctx.sync()
-    # timed region: K steps, HIP events around every kernel on the context stream
+    # timed region: K steps of the isolated pair
+    ctx.timer_start()
+    for _ in range(a.steps):
+        ctx.flow_pair(rgb1, rgb2, U, V)
+    total_ms = ctx.timer_stop()
+    ms_per_step = total_ms / a.steps
+    # stage times: the same K steps again with HIP events around every kernel
+    # on the context stream (the events cost a few microseconds per launch,
+    # which matters only for small frames)
     ctx.profile_enable(True)
     ctx.profile_read()
     ctx.timer_start()
     for _ in range(a.steps):
         ctx.flow_pair(rgb1, rgb2, U, V)
-    total_ms = ctx.timer_stop()
+    prof_ms = ctx.timer_stop() / a.steps
     prof = ctx.profile_read()
     ctx.profile_enable(False)
-    ms_per_step = total_ms / a.steps
 
     s_tri = int(np.ceil(2.5 * float(np.sqrt(np.float32(p.sS) * np.float32(p.sS)))))
     work = rl.stage_work(p, ctx.NSP, s_tri)
@@ -237,13 +247,36 @@ def measure_pair(a, bm, name, tune_cache=None):
     ctx.sync()
     gU, gV = U.download((H, W), np.float32), V.download((H, W), np.float32)
     e_all, e_noc = bm.epe(gU, gV, pr.gt, pr.occ)
+    replay = graph_replay(a, bm, p, tuning, rgb1, rgb2, gU, gV, W, H)
     out = {"value": round(W * H / (ms_per_step * 1e3), 3), "unit": "Mpix/s", "ms_per_step": round(ms_per_step, 4),
+           "ms_per_step_with_stage_events": round(prof_ms, 4), "graph_replay": replay,
            "config": {"workload": name, "input": "%dx%d RGB uint8" % (W, H), "matched": "%dx%d" % (p.sx, p.sy),
                       "feature_bits": p.N, "iterations": p.iterations, "jfMax": p.jfMax, "median": p.fsize,
                       "trilateral_window": 2 * s_tri, "mode": "isolated pair, inputs resident in HBM",
                       "tuning": tuning},
            "roofline": roof, "stages": stages, "epe": {"all": round(e_all, 4), "noc": round(e_noc, 4)}}
     return out, dict(ctx=ctx, p=p, pr=pr, rgb1=rgb1, rgb2=rgb2, U=U, V=V, gU=gU, gV=gV, tuning=tuning, W=W, H=H)
+
+
+def graph_replay(a, bm, p, tuning, rgb1, rgb2, gU, gV, W, H):
+    """The same isolated pair captured once as a hipGraph (a one-pair batch in
+    graph mode, same seed and launch shapes) and replayed K times: what the
+    launch sequence costs beyond its kernels.  Checked bit for bit against
+    bm_flow_pair's output."""
+    b = bm.Batch([p], seed=1, streams=1)
+    b.ctx[0].set_tuning(**tuning)
+    U, V = b.ctx[0].image_buffer(), b.ctx[0].image_buffer()
+    for _ in range(max(1, a.warmup)):
+        b.flow([rgb1], [rgb2], [U], [V], mode=bm.BATCH_GRAPH)
+    b.sync()
+    b.timer_start()
+    for _ in range(a.steps):
+        b.flow([rgb1], [rgb2], [U], [V], mode=bm.BATCH_GRAPH)
+    ms = b.timer_stop() / a.steps
+    same = bool(np.array_equal(U.download((H, W), np.float32), gU) and
+                np.array_equal(V.download((H, W), np.float32), gV))
+    b.close()
+    return {"ms_per_step": round(ms, 4), "value": round(W * H / (ms * 1e3), 3), "bit_exact_vs_flow_pair": same}
 
 
 def bench_streaming(a, s):
@@ -318,12 +351,14 @@ def bench_learned(a, bm, s, plain_out):
             for _ in range(a.warmup):
                 ctx.flow_pair_learned(net, s["rgb1"], s["rgb2"], U, V)
             ctx.sync()
-            ctx.profile_enable(True)
-            ctx.profile_read()
             ctx.timer_start()
             for _ in range(a.steps):
                 ctx.flow_pair_learned(net, s["rgb1"], s["rgb2"], U, V)
             total_ms = ctx.timer_stop()
+            ctx.profile_enable(True)
+            ctx.profile_read()
+            for _ in range(a.steps):
+                ctx.flow_pair_learned(net, s["rgb1"], s["rgb2"], U, V)
             prof = ctx.profile_read()
             ctx.profile_enable(False)
             e_all, e_noc = bm.epe(U.download((H, W), np.float32), V.download((H, W), np.float32), pr.gt, pr.occ)
@@ -480,6 +515,8 @@ def main():
     out["config"] = main_out["config"]
     out["roofline"] = main_out["roofline"]
     out["stages"] = main_out["stages"]
+    out["ms_per_step_with_stage_events"] = main_out["ms_per_step_with_stage_events"]
+    out["graph_replay"] = main_out["graph_replay"]
     out["epe"] = main_out["epe"]
     if not a.no_streaming:
         out["streaming"] = bench_streaming(a, s)
