@@ -37,7 +37,8 @@ class BriefMatchError(RuntimeError):
 
 class Tuning(C.Structure):
     """bm_tuning: launch-shape choices; results do not depend on them."""
-    _fields_ = [(n, C.c_int) for n in ("level_px", "strip_rows", "brief_near", "tri_band", "tri_groups")]
+    _fields_ = [(n, C.c_int) for n in ("level_px", "strip_rows", "brief_near", "tri_band", "tri_groups",
+                                       "brief_frame")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -438,11 +438,15 @@ int lanczos_ready(bm_ctx *ctx, bool frame)
 }
 
 // Descriptors of one gray frame with the context's up-sampler: bilinear fused
-// into BRIEF, or Lanczos-3 into the context frame followed by BRIEF on it.
+// into BRIEF, bilinear into the context frame followed by BRIEF on it
+// (brief_frame), or Lanczos-3 into the context frame followed by BRIEF on it.
 int describe(bm_ctx *ctx, const uint8_t *g8, uint32_t *F)
 {
     if (ctx->upsampler == BM_UPSAMPLE_LANCZOS3) {
         BM_LAUNCH(ctx, BM_STAGE_UPSAMPLE, bm::launch_upsample_lanczos(ctx->g, g8, ctx->d_L, ctx->stream));
+        BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(ctx->g, ctx->d_L, nullptr, F, ctx->stream));
+    } else if (ctx->g.brief_frame) {
+        BM_LAUNCH(ctx, BM_STAGE_UPSAMPLE, bm::launch_upsample(ctx->g, g8, ctx->d_L, ctx->stream));
         BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(ctx->g, ctx->d_L, nullptr, F, ctx->stream));
     } else {
         BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief(ctx->g, nullptr, g8, F, ctx->stream));
@@ -602,6 +606,7 @@ int prepare_host(bm_ctx *ctx, const bm_params &p, uint64_t seed, HostTables &t)
     g.seed = seed;
     g.level_px = NSP <= 4 ? 2 : 1;
     g.strip_rows = 256;
+    g.brief_frame = 0;
     const float rs_sigma = std::sqrt(std::ceil((float)p.sx * 0.5f / 3.0f));
     g.rs_sigma_q16 = (int64_t)std::llround((double)rs_sigma * 65536.0);
 
@@ -1602,6 +1607,8 @@ int tuning_valid(const bm_ctx *ctx, const bm_tuning &t, std::string &why)
     const int maxb = ctx->p.filter ? bm::tri_max_band(ctx->g.tri_s) : 16;
     if (t.tri_band < 1 || t.tri_band > std::max(1, maxb)) return why = "tri_band outside the LDS limit", 0;
     if (t.tri_groups != 0 && t.tri_groups != 2 && t.tri_groups != 4) return why = "tri_groups must be 0, 2 or 4", 0;
+    if (t.brief_frame != 0 && t.brief_frame != 1) return why = "brief_frame must be 0 or 1", 0;
+    if (t.brief_frame && t.brief_near == 0) return why = "brief_frame 1 needs brief_near > 0", 0;
     return 1;
 }
 
@@ -1617,10 +1624,15 @@ int apply_tuning(bm_ctx *ctx, const bm_tuning &t)
         const int r = set_pattern_internal(ctx, pat, t.brief_near);
         if (r != BM_OK) return r;
     }
+    if (t.brief_frame && !ctx->d_L) {
+        const int r = dev_alloc(ctx, &ctx->d_L, (size_t)ctx->g.sx * ctx->g.sy);
+        if (r != BM_OK) return r;
+    }
     ctx->g.level_px = t.level_px;
     ctx->g.strip_rows = t.strip_rows;
     ctx->g.tri_band = t.tri_band;
     ctx->g.tri_groups = t.tri_groups;
+    ctx->g.brief_frame = t.brief_frame;
     return BM_OK;
 }
 
@@ -1632,6 +1644,7 @@ bm_tuning current_tuning(const bm_ctx *ctx)
     t.brief_near = ctx->g.brief_near;
     t.tri_band = ctx->g.tri_band;
     t.tri_groups = ctx->g.tri_groups;
+    t.brief_frame = ctx->g.brief_frame;
     return t;
 }
 
@@ -1684,7 +1697,8 @@ int bm_autotune(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, const cha
             std::istringstream ls(line);
             std::string k;
             bm_tuning t;
-            if (ls >> k >> t.level_px >> t.strip_rows >> t.brief_near >> t.tri_band >> t.tri_groups && k == key) {
+            if (ls >> k >> t.level_px >> t.strip_rows >> t.brief_near >> t.tri_band >> t.tri_groups >> t.brief_frame &&
+                k == key) {
                 std::string why;
                 if (tuning_valid(ctx, t, why) && apply_tuning(ctx, t) == BM_OK) {
                     if (out) *out = t;
@@ -1730,19 +1744,46 @@ int bm_autotune(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, const cha
         std::initializer_list<int> stages;
     };
     const std::initializer_list<int> lv = {BM_STAGE_INIT, BM_STAGE_LEVEL, BM_STAGE_LEVEL_RS};
-    const std::initializer_list<int> br = {BM_STAGE_BRIEF};
+    const std::initializer_list<int> br = {BM_STAGE_BRIEF, BM_STAGE_UPSAMPLE};
     const std::initializer_list<int> tr = {BM_STAGE_TRILATERAL};
     std::vector<int> bands;
     const int maxb = ctx->p.filter ? bm::tri_max_band(ctx->g.tri_s) : 0;
     for (int v : {4, 8, 12, 16})
         if (v <= maxb) bands.push_back(v);
-    const Knob knobs[] = {{&bm_tuning::brief_near, {16, 24, 32, 40, 48}, br},
-                          {&bm_tuning::level_px, {1, 2, 4}, lv},
+    // the BRIEF knobs interact (with the frame in memory the far tests are
+    // cheap and a smaller near halo, with more work-groups per CU, wins), so
+    // they are timed jointly first
+    int e = BM_OK;
+    {
+        double best_ms = -1.0;
+        bm_tuning winner = best;
+        for (int frame : {0, 1})
+            for (int near : {16, 24, 32, 40, 48}) {
+                bm_tuning t = best;
+                t.brief_frame = frame;
+                t.brief_near = near;
+                std::string why;
+                if (!tuning_valid(ctx, t, why)) continue;
+                const int save = ctx->g.brief_near;  // check that the near halo fits before timing it
+                const bool fits = plan_brief(ctx, pattern_rmax(ctx), near);
+                plan_brief(ctx, pattern_rmax(ctx), save);
+                if (!fits) continue;
+                double ms = 0.0;
+                e = measure(t, br, ms);
+                if (e != BM_OK) break;
+                if (best_ms < 0 || ms < best_ms) {
+                    best_ms = ms;
+                    winner = t;
+                }
+            }
+        best = winner;
+    }
+    const Knob knobs[] = {{&bm_tuning::level_px, {1, 2, 4}, lv},
                           {&bm_tuning::strip_rows, {64, 128, 256, 512, 1024}, lv},
                           {&bm_tuning::tri_groups, {0, 2, 4}, tr},
                           {&bm_tuning::tri_band, bands, tr}};
-    int e = BM_OK;
     for (const Knob &kn : knobs) {
+        if (e != BM_OK) break;
         double best_ms = -1.0;
         bm_tuning winner = best;
         for (int v : kn.values) {
@@ -1777,7 +1818,7 @@ int bm_autotune(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, const cha
     if (cache_path) {
         std::ofstream o(cache_path, std::ios::app);
         o << key << " " << best.level_px << " " << best.strip_rows << " " << best.brief_near << " " << best.tri_band
-          << " " << best.tri_groups << "\n";
+          << " " << best.tri_groups << " " << best.brief_frame << "\n";
     }
     return BM_OK;
 }

@@ -48,6 +48,7 @@ struct Geometry {
     int strip_rows;               // strip height of the matching kernels' block order
     int tri_band;                 // max source rows per LDS band of the trilateral filter
     int tri_groups;               // trilateral kernel: 0 VALU sums, 2 / 4 MFMA sums with that many 4-row groups
+    int brief_frame;              // 1: the whole-path calls up-sample into a frame and BRIEF reads it
 };
 
 // Largest trilateral band (rows) that fits the LDS budget for half window s.
@@ -115,6 +116,7 @@ struct BriefJob {
     const int *plan_count;  // per word: bit records, far records
     int N, W, H, sx, sy, Rf, gw, gh;
     uint32_t *F;
+    const float *L = nullptr;  // up-sampled frame: tile and far tests read it (else interpolated from g8)
 };
 
 // Initialisation and propagation levels of one pair: the field ping-pongs

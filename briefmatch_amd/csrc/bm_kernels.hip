@@ -290,75 +290,104 @@ __device__ __forceinline__ uint32_t bil_lds(const uint16_t *gt, int x0, uint32_t
 }
 
 // One 64 x TH tile b (row-major tile index) of the fused BRIEF; smem is the
-// work-group's dynamic LDS.
+// work-group's dynamic LDS.  GL: the up-sampled frame is in global memory
+// (a.L, written by the up-sampler before): the tile and its near halo are
+// copied from it and the far tests read it directly, so the work-group needs
+// neither the pair tile nor the coordinate tables and no test interpolates.
 
-template <int NSP, int RN, int PY>
+template <int NSP, int RN, int PY, bool GL>
 __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned char *smem)
 {
     constexpr int TH = 4 * PY, PITCH = kBriefTW + 2 * RN, ROWS = TH + 2 * RN;
     float *lt = reinterpret_cast<float *>(smem);
-    const int Rf = a.Rf, XT = kBriefTW + 2 * Rf, YT = TH + 2 * Rf;
-    uint32_t *xt = reinterpret_cast<uint32_t *>(lt + PITCH * ROWS);
-    uint2 *yt = reinterpret_cast<uint2 *>(xt + XT + (XT & 1));
-    uint16_t *gt = reinterpret_cast<uint16_t *>(yt + YT);
-    const int sx = a.sx, sy = a.sy, W = a.W, H = a.H;
-
+    const int sx = a.sx, sy = a.sy;
     const int gxb = (sx + kBriefTW - 1) / kBriefTW;
     const int tx0 = (b % gxb) * kBriefTW, ty0 = (b / gxb) * TH;
     const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
-
-    const int gx0 = (int)(a.up_x[clampi(tx0 - Rf, 0, sx - 1)] >> 8);
-    const int gy0 = (int)(a.up_y[clampi(ty0 - Rf, 0, sy - 1)] >> 8);
-    const int gx1 = min((int)(a.up_x[clampi(tx0 + kBriefTW - 1 + Rf, 0, sx - 1)] >> 8) + 1, W - 1);
-    const int gy1 = min((int)(a.up_y[clampi(ty0 + TH - 1 + Rf, 0, sy - 1)] >> 8) + 1, H - 1);
-    const int gw = gx1 - gx0 + 1, gh = gy1 - gy0 + 1;
-
-    for (int i = threadIdx.x; i < XT; i += kThreads) {
-        const uint32_t u = a.up_x[clampi(tx0 - Rf + i, 0, sx - 1)];
-        xt[i] = (uint32_t)((int)(u >> 8) - gx0) | ((u & 255u) << 16);
-    }
-    for (int i = threadIdx.x; i < YT; i += kThreads) {
-        const uint32_t u = a.up_y[clampi(ty0 - Rf + i, 0, sy - 1)];
-        const int y0 = (int)(u >> 8), y1 = min(y0 + 1, H - 1);
-        yt[i] = make_uint2((uint32_t)((y0 - gy0) * gw * 2) | ((uint32_t)((y1 - gy0) * gw * 2) << 16),
-                           weight_pair(u & 255u));
-    }
-    // The pair tile: each gray byte is read from memory once, FB rows at a time
-    // (their loads in flight together: the fill is latency-bound, 8 rows
-    // measured best against 2, 4, 12 and 16); the right neighbour comes from
-    // the next lane, except in lane 63 and past the tile's last column.
-    constexpr int FB = 8;
-    for (int r0 = ly; r0 < gh; r0 += 4 * FB) {
-        for (int c0 = 0; c0 < gw; c0 += 64) {
-            const int c = c0 + lx, x = gx0 + min(c, gw - 1);
-            uint32_t v[FB], n[FB];
+    const uint32_t *xrow = nullptr;
+    const uint2 *ycol = nullptr;
+    const uint16_t *gt = nullptr;
+    if constexpr (GL) {
+        // the tile and its halo, clamped to the frame per axis (as the reference's
+        // per-axis offset clamp): element i of a thread is row ly + 4 (i / NC),
+        // column lx + 64 (i % NC); FB loads in flight before their stores (the
+        // copy is latency-bound)
+        constexpr int NC = (PITCH + 63) / 64, NR = (ROWS + 3) / 4, TOT = NC * NR, FB = 16;
+        for (int i0 = 0; i0 < TOT; i0 += FB) {
+            float v[FB];
 #pragma unroll
-            for (int q = 0; q < FB; ++q) v[q] = a.g8[(size_t)(gy0 + min(r0 + 4 * q, gh - 1)) * W + x];
-#pragma unroll
-            for (int q = 0; q < FB; ++q) n[q] = (uint32_t)__shfl_down((int)v[q], 1, 64);
-            if (lx == 63 || c + 1 >= gw) {
-#pragma unroll
-                for (int q = 0; q < FB; ++q)
-                    n[q] = a.g8[(size_t)(gy0 + min(r0 + 4 * q, gh - 1)) * W + min(x + 1, W - 1)];
+            for (int q = 0; q < FB; ++q) {
+                const int i = min(i0 + q, TOT - 1), r = ly + 4 * (i / NC), c = lx + 64 * (i % NC);
+                v[q] = a.L[(size_t)clampi(ty0 - RN + r, 0, sy - 1) * sx + clampi(tx0 - RN + c, 0, sx - 1)];
             }
 #pragma unroll
-            for (int q = 0; q < FB; ++q)
-                if (c < gw && r0 + 4 * q < gh) gt[(r0 + 4 * q) * gw + c] = (uint16_t)(v[q] | (n[q] << 8));
+            for (int q = 0; q < FB; ++q) {
+                const int i = i0 + q, r = ly + 4 * (i / NC), c = lx + 64 * (i % NC);
+                if (i < TOT && r < ROWS && c < PITCH) lt[r * PITCH + c] = v[q];
+            }
         }
+        __syncthreads();
+    } else {
+        const int Rf = a.Rf, XT = kBriefTW + 2 * Rf, YT = TH + 2 * Rf;
+        uint32_t *xt = reinterpret_cast<uint32_t *>(lt + PITCH * ROWS);
+        uint2 *yt = reinterpret_cast<uint2 *>(xt + XT + (XT & 1));
+        uint16_t *gtw = reinterpret_cast<uint16_t *>(yt + YT);
+        const int W = a.W, H = a.H;
+
+        const int gx0 = (int)(a.up_x[clampi(tx0 - Rf, 0, sx - 1)] >> 8);
+        const int gy0 = (int)(a.up_y[clampi(ty0 - Rf, 0, sy - 1)] >> 8);
+        const int gx1 = min((int)(a.up_x[clampi(tx0 + kBriefTW - 1 + Rf, 0, sx - 1)] >> 8) + 1, W - 1);
+        const int gy1 = min((int)(a.up_y[clampi(ty0 + TH - 1 + Rf, 0, sy - 1)] >> 8) + 1, H - 1);
+        const int gw = gx1 - gx0 + 1, gh = gy1 - gy0 + 1;
+
+        for (int i = threadIdx.x; i < XT; i += kThreads) {
+            const uint32_t u = a.up_x[clampi(tx0 - Rf + i, 0, sx - 1)];
+            xt[i] = (uint32_t)((int)(u >> 8) - gx0) | ((u & 255u) << 16);
+        }
+        for (int i = threadIdx.x; i < YT; i += kThreads) {
+            const uint32_t u = a.up_y[clampi(ty0 - Rf + i, 0, sy - 1)];
+            const int y0 = (int)(u >> 8), y1 = min(y0 + 1, H - 1);
+            yt[i] = make_uint2((uint32_t)((y0 - gy0) * gw * 2) | ((uint32_t)((y1 - gy0) * gw * 2) << 16),
+                               weight_pair(u & 255u));
+        }
+        // The pair tile: each gray byte is read from memory once, FB rows at a time
+        // (their loads in flight together: the fill is latency-bound, 8 rows
+        // measured best against 2, 4, 12 and 16); the right neighbour comes from
+        // the next lane, except in lane 63 and past the tile's last column.
+        constexpr int FB = 8;
+        for (int r0 = ly; r0 < gh; r0 += 4 * FB) {
+            for (int c0 = 0; c0 < gw; c0 += 64) {
+                const int c = c0 + lx, x = gx0 + min(c, gw - 1);
+                uint32_t v[FB], n[FB];
+    #pragma unroll
+                for (int q = 0; q < FB; ++q) v[q] = a.g8[(size_t)(gy0 + min(r0 + 4 * q, gh - 1)) * W + x];
+    #pragma unroll
+                for (int q = 0; q < FB; ++q) n[q] = (uint32_t)__shfl_down((int)v[q], 1, 64);
+                if (lx == 63 || c + 1 >= gw) {
+    #pragma unroll
+                    for (int q = 0; q < FB; ++q)
+                        n[q] = a.g8[(size_t)(gy0 + min(r0 + 4 * q, gh - 1)) * W + min(x + 1, W - 1)];
+                }
+    #pragma unroll
+                for (int q = 0; q < FB; ++q)
+                    if (c < gw && r0 + 4 * q < gh) gtw[(r0 + 4 * q) * gw + c] = (uint16_t)(v[q] | (n[q] << 8));
+            }
+        }
+        __syncthreads();
+        for (int c = lx; c < PITCH; c += 64) {
+            const uint32_t xv = xt[c - RN + Rf];
+            const int x0 = (int)(xv & 0xffffu);
+            const uint32_t wx = weight_pair(xv >> 16);
+            for (int r = ly; r < ROWS; r += 4)
+                lt[r * PITCH + c] = (float)bil_lds(gtw, x0, wx, yt[r - RN + Rf]) * (1.0f / 65536.0f);
+        }
+        __syncthreads();
+        xrow = xt + lx + Rf;
+        ycol = yt + ly + Rf;
+        gt = gtw;
     }
-    __syncthreads();
-    for (int c = lx; c < PITCH; c += 64) {
-        const uint32_t xv = xt[c - RN + Rf];
-        const int x0 = (int)(xv & 0xffffu);
-        const uint32_t wx = weight_pair(xv >> 16);
-        for (int r = ly; r < ROWS; r += 4)
-            lt[r * PITCH + c] = (float)bil_lds(gt, x0, wx, yt[r - RN + Rf]) * (1.0f / 65536.0f);
-    }
-    __syncthreads();
 
     const float *lbase = lt + (ly + RN) * PITCH + lx + RN;
-    const uint32_t *xrow = xt + lx + Rf;
-    const uint2 *ycol = yt + ly + Rf;
     uint32_t words[NSP][PY];
 #pragma unroll
     for (int w = 0; w < NSP; ++w)
@@ -402,6 +431,27 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
             }
         }
         const int nfar = a.plan_count[2 * w + 1];
+        if constexpr (GL) {
+            // a far sample is one load of the up-sampled frame; its FP32 bit
+            // pattern orders like the value (non-negative samples)
+            const int x = tx0 + lx;
+            for (int f = 0; f < nfar; ++f) {
+                const uint4 r0 = far_rec[32 * w + f];
+                const int ox1 = (int16_t)(r0.x & 0xffffu), oy1 = (int16_t)(r0.x >> 16);
+                const int ox2 = (int16_t)(r0.y & 0xffffu), oy2 = (int16_t)(r0.y >> 16);
+                const int xa = clampi(x + ox1, 0, sx - 1), xb = clampi(x + ox2, 0, sx - 1);
+                uint32_t va[PY], vb[PY];
+#pragma unroll
+                for (int k = 0; k < PY; ++k) {
+                    const int y = ty0 + ly + 4 * k;
+                    va[k] = __float_as_uint(a.L[(size_t)clampi(y + oy1, 0, sy - 1) * sx + xa]);
+                    vb[k] = __float_as_uint(a.L[(size_t)clampi(y + oy2, 0, sy - 1) * sx + xb]);
+                }
+#pragma unroll
+                for (int k = 0; k < PY; ++k) words[w][k] |= ((vb[k] - va[k]) >> 31) << r0.z;
+            }
+            continue;
+        }
         for (int f = 0; f < nfar; ++f) {
             const uint4 r0 = far_rec[32 * w + f];
             const int ox1 = (int16_t)(r0.x & 0xffffu), oy1 = (int16_t)(r0.x >> 16);
@@ -432,13 +482,13 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
     }
 }
 
-template <int NSP, int RN, int PY>
+template <int NSP, int RN, int PY, bool GL>
 __global__ void __launch_bounds__(kThreads) k_brief_lds(BriefJob a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int TH = 4 * PY;
     const int gxb = (a.sx + kBriefTW - 1) / kBriefTW, gyb = (a.sy + TH - 1) / TH;
-    brief_tile<NSP, RN, PY>(a, xcd_remap(blockIdx.x, gxb * gyb), smem);
+    brief_tile<NSP, RN, PY, GL>(a, xcd_remap(blockIdx.x, gxb * gyb), smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -1584,7 +1634,7 @@ __global__ void __launch_bounds__(kThreads) k_brief_multi(const BriefJob *__rest
     extern __shared__ __align__(16) unsigned char smem[];
     BM_MULTI_TILES({
         if (it_) __syncthreads();
-        brief_tile<NSP, RN, PY>(jobs[j_], lt_, smem);
+        brief_tile<NSP, RN, PY, false>(jobs[j_], lt_, smem);
     })
 }
 
@@ -1746,29 +1796,39 @@ static hipError_t launch_brief_t(const Geometry &g, const float *L, const uint8_
     return hipGetLastError();
 }
 
+// L != nullptr: the up-sampled frame is in memory (the GL kernel, LDS for
+// the near tile only); else the fused kernel interpolates from g8.
 template <int NSP, int RN>
-static hipError_t launch_brief_lds_t(const Geometry &g, const uint8_t *g8, uint32_t *F, hipStream_t st)
+static hipError_t launch_brief_lds_t(const Geometry &g, const float *L, const uint8_t *g8, uint32_t *F,
+                                     hipStream_t st)
 {
     constexpr int PY = NSP <= 4 ? 8 : 4;
     constexpr int TH = 4 * PY;
-    const size_t lds = brief_lds_bytes(g, RN, TH);
-    const BriefJob a{g8,  g.up_x, g.up_y, g.brief_plan, g.brief_plan_count, g.N, g.W, g.H, g.sx, g.sy, g.brief_far,
-                     g.brief_gw[PY == 8 ? 1 : 0], g.brief_gh[PY == 8 ? 1 : 0], F};
+    BriefJob a{g8,  g.up_x, g.up_y, g.brief_plan, g.brief_plan_count, g.N, g.W, g.H, g.sx, g.sy, g.brief_far,
+               g.brief_gw[PY == 8 ? 1 : 0], g.brief_gh[PY == 8 ? 1 : 0], F};
+    a.L = L;
     const int gxb = (g.sx + kBriefTW - 1) / kBriefTW, gyb = (g.sy + TH - 1) / TH;
-    hipLaunchKernelGGL((k_brief_lds<NSP, RN, PY>), dim3(gxb * gyb), dim3(kThreads), lds, st, a);
+    if (L) {
+        const size_t lds = (size_t)(kBriefTW + 2 * RN) * (TH + 2 * RN) * sizeof(float);
+        hipLaunchKernelGGL((k_brief_lds<NSP, RN, PY, true>), dim3(gxb * gyb), dim3(kThreads), lds, st, a);
+    } else {
+        const size_t lds = brief_lds_bytes(g, RN, TH);
+        hipLaunchKernelGGL((k_brief_lds<NSP, RN, PY, false>), dim3(gxb * gyb), dim3(kThreads), lds, st, a);
+    }
     return hipGetLastError();
 }
 
 template <int NSP>
-static hipError_t launch_brief_lds_n(const Geometry &g, const uint8_t *g8, uint32_t *F, hipStream_t st)
+static hipError_t launch_brief_lds_n(const Geometry &g, const float *L, const uint8_t *g8, uint32_t *F,
+                                     hipStream_t st)
 {
     switch (g.brief_near) {
-    case 8: return launch_brief_lds_t<NSP, 8>(g, g8, F, st);
-    case 16: return launch_brief_lds_t<NSP, 16>(g, g8, F, st);
-    case 24: return launch_brief_lds_t<NSP, 24>(g, g8, F, st);
-    case 32: return launch_brief_lds_t<NSP, 32>(g, g8, F, st);
-    case 40: return launch_brief_lds_t<NSP, 40>(g, g8, F, st);
-    case 48: return launch_brief_lds_t<NSP, 48>(g, g8, F, st);
+    case 8: return launch_brief_lds_t<NSP, 8>(g, L, g8, F, st);
+    case 16: return launch_brief_lds_t<NSP, 16>(g, L, g8, F, st);
+    case 24: return launch_brief_lds_t<NSP, 24>(g, L, g8, F, st);
+    case 32: return launch_brief_lds_t<NSP, 32>(g, L, g8, F, st);
+    case 40: return launch_brief_lds_t<NSP, 40>(g, L, g8, F, st);
+    case 48: return launch_brief_lds_t<NSP, 48>(g, L, g8, F, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1776,13 +1836,13 @@ static hipError_t launch_brief_lds_n(const Geometry &g, const uint8_t *g8, uint3
 hipError_t launch_brief(const Geometry &g, const float *L, const uint8_t *g8, uint32_t *F, hipStream_t st)
 {
     const bool fused = (L == nullptr);
-    if (fused && g.brief_near > 0) {
+    if (g.brief_near > 0) {
         switch (g.NSP) {
-        case 1: return launch_brief_lds_n<1>(g, g8, F, st);
-        case 2: return launch_brief_lds_n<2>(g, g8, F, st);
-        case 4: return launch_brief_lds_n<4>(g, g8, F, st);
-        case 8: return launch_brief_lds_n<8>(g, g8, F, st);
-        case 16: return launch_brief_lds_n<16>(g, g8, F, st);
+        case 1: return launch_brief_lds_n<1>(g, L, g8, F, st);
+        case 2: return launch_brief_lds_n<2>(g, L, g8, F, st);
+        case 4: return launch_brief_lds_n<4>(g, L, g8, F, st);
+        case 8: return launch_brief_lds_n<8>(g, L, g8, F, st);
+        case 16: return launch_brief_lds_n<16>(g, L, g8, F, st);
         default: return hipErrorInvalidValue;
         }
     }

@@ -241,6 +241,10 @@ typedef struct bm_tuning {
     int tri_band;     /* source rows per LDS band of the trilateral filter: 1..(LDS limit, <= 16) */
     int tri_groups;   /* trilateral kernel: 0 = weighted sums on the VALU, 4 output rows per thread;
                          2 or 4 = weighted sums on MFMA (4x4x1 f32), that many 4-row groups per wave */
+    int brief_frame;  /* bilinear up-sampling of the whole-path calls: 0 = fused into the BRIEF kernel
+                         (interpolated from the gray frame in LDS); 1 = written to a context frame first,
+                         the BRIEF kernel then copies its tile from that frame and reads its far tests
+                         from it (needs brief_near > 0) */
 } bm_tuning;
 
 int bm_get_tuning(const bm_ctx *ctx, bm_tuning *out);

@@ -31,11 +31,14 @@ def test_tuning_does_not_change_results(N):
     ref = _flow(ctx, pr)
     t0 = ctx.tuning()
     px = (1, 2, 4) if N <= 128 else ((1, 2) if N <= 256 else (1,))
-    for lp, strip, near, band in itertools.product(px, (16, 256), (0, 8, 32, 48), (1, t0.tri_band)):
-        ctx.set_tuning(level_px=lp, strip_rows=strip, brief_near=near, tri_band=band)
+    for lp, strip, near, band, frame in itertools.product(px, (16, 256), (0, 8, 32, 48), (1, t0.tri_band), (0, 1)):
+        if frame and not near:
+            continue
+        ctx.set_tuning(level_px=lp, strip_rows=strip, brief_near=near, tri_band=band, brief_frame=frame)
         got = _flow(ctx, pr)
         for a, b in zip(ref, got):
-            assert np.array_equal(a, b), (lp, strip, near, band)
+            assert np.array_equal(a, b), (lp, strip, near, band, frame)
+    ctx.set_tuning(brief_frame=0)
     # the trilateral kernels sum in different orders: equal within the flow tolerance
     for groups in (0, 2, 4):
         ctx.set_tuning(tri_groups=groups)
@@ -50,7 +53,8 @@ def test_tuning_rejects_invalid():
     p = bm.derive_params(64, 48, 2.0, 2.0, N=512)
     ctx = bm.Context(p)
     for kw in ({"level_px": 3}, {"level_px": 2}, {"level_px": 4}, {"strip_rows": 8}, {"brief_near": 12},
-               {"tri_band": 0}, {"tri_groups": 1}, {"tri_groups": 8}):
+               {"tri_band": 0}, {"tri_groups": 1}, {"tri_groups": 8}, {"brief_frame": 2},
+               {"brief_near": 0, "brief_frame": 1}):
         with pytest.raises(bm.BriefMatchError):
             ctx.set_tuning(**kw)
 

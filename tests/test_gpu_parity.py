@@ -104,7 +104,9 @@ def test_brief_pattern_halo_boundaries(N):
     it, at zero and far out, in words that mix them, words of near tests only
     and words of far tests only (and a partial last word for N = 72): the
     fused BRIEF (one bit record per bit in bit order, far bits through their
-    own records) equals the oracle for every near halo the tuner can pick."""
+    own records) equals the oracle for every near halo the tuner can pick,
+    and so does the BRIEF kernel reading an up-sampled frame (its tile and far
+    tests from the frame, the same records)."""
     say("brief pattern boundaries", N)
     W, H, up = 96, 64, 2.0
     pr = _pair(W, H, seed=N)
@@ -114,6 +116,8 @@ def test_brief_pattern_halo_boundaries(N):
     rgb = bm.DeviceBuffer(array=pr.frame1)
     g8 = bm.DeviceBuffer(W * H)
     ctx.gray(rgb, g8=g8)
+    L = bm.DeviceBuffer(p.sx * p.sy * 4)
+    ctx.upsample(g8, L)
     rng = np.random.default_rng(N)
     for rn in (8, 16, 24, 32, 40, 48):
         pat = np.zeros((N, 4), np.int16)
@@ -134,6 +138,10 @@ def test_brief_pattern_halo_boundaries(N):
         ctx.sync()
         oF = _pad(oracle.brief(oL, pat.ravel(), N), ctx.NSP)
         assert np.array_equal(ctx.download_desc(F), oF), (rn, ctx.tuning().brief_near)
+        Ff = ctx.desc_buffer()
+        ctx.brief(L, Ff)
+        ctx.sync()
+        assert np.array_equal(ctx.download_desc(Ff), oF), ("frame", rn)
         print("[brief halo]", N, rn, ctx.tuning().brief_near, flush=True)
 
 
