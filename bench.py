@@ -226,7 +226,7 @@ def measure_pair(a, bm, name, tune_cache=None):
     ctx.profile_enable(False)
 
     s_tri = int(np.ceil(2.5 * float(np.sqrt(np.float32(p.sS) * np.float32(p.sS)))))
-    work = rl.stage_work(p, ctx.NSP, s_tri)
+    work = rl.stage_work(p, ctx.NSP, s_tri, tuning.get("brief_frame", 0))
     stages = {}
     for st, (ms, n) in prof.items():
         if n == 0:

@@ -70,10 +70,11 @@ def median_minmax(fs):
     return minmax
 
 
-def stage_work(p, NSP, s_tri):
+def stage_work(p, NSP, s_tri, brief_frame=0):
     """Algorithmic work per launch of each stage for parameters p (a bm_params
     with sx, sy, sx_in, sy_in, N, fsize), descriptor stride NSP and trilateral
-    half window s_tri."""
+    half window s_tri; brief_frame 1: the up-sampled frame is written by its
+    own launch and read by BRIEF."""
     Win = p.sx_in * p.sy_in
     M = p.sx * p.sy
     b = p.N / 8.0
@@ -81,8 +82,12 @@ def stage_work(p, NSP, s_tri):
     return {
         # frame 1 writes gray8 + FP32, frame 2 gray8 only: the mean of the two launches
         "gray": {"hbm": Win * (3 + 1 + 2.0)},
-        # fused up-sampling + BRIEF of one frame: gray8 in, descriptors out
-        "brief": {"hbm": Win * 1.0 + M * NSP * 4.0, "lds": M * p.N * 8.0, "valu": M * p.N * BRIEF_VALU_PER_TEST},
+        # BRIEF of one frame: gray8 in (up-sampling fused) or the FP32 frame in,
+        # descriptors out
+        "brief": {"hbm": (M * 4.0 if brief_frame else Win * 1.0) + M * NSP * 4.0, "lds": M * p.N * 8.0,
+                  "valu": M * p.N * BRIEF_VALU_PER_TEST},
+        # bilinear up-sampling of one frame into memory: gray8 in, FP32 frame out
+        "upsample": {"hbm": Win * 1.0 + M * 4.0},
         "init": {"hbm": M * (2 * b + 6)},
         "level": {"hbm": M * (2 * b + 12)},          # F1, F2 once, field in + out
         "level_rs": {"hbm": M * (2 * b + 12)},
