@@ -244,7 +244,8 @@ typedef struct bm_tuning {
     int brief_frame;  /* bilinear up-sampling of the whole-path calls: 0 = fused into the BRIEF kernel
                          (interpolated from the gray frame in LDS); 1 = written to a context frame first,
                          the BRIEF kernel then copies its tile from that frame and reads its far tests
-                         from it (needs brief_near > 0) */
+                         from it (needs brief_near > 0); a new context takes 1 when its matched frame
+                         has at least 2^21 pixels */
 } bm_tuning;
 
 int bm_get_tuning(const bm_ctx *ctx, bm_tuning *out);
