@@ -388,6 +388,16 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
     }
 
     const float *lbase = lt + (ly + RN) * PITCH + lx + RN;
+    // GL: a tile whose far halo (a.Rf, the largest pattern offset) lies inside
+    // the frame, and the pixel index of each of the thread's rows
+    bool interior = false;
+    uint32_t pix[PY];
+    if constexpr (GL) {
+        interior = tx0 - a.Rf >= 0 && tx0 + kBriefTW - 1 + a.Rf <= sx - 1 && ty0 - a.Rf >= 0 &&
+                   ty0 + TH - 1 + a.Rf <= sy - 1;
+#pragma unroll
+        for (int k = 0; k < PY; ++k) pix[k] = (uint32_t)((ty0 + ly + 4 * k) * sx + tx0 + lx);
+    }
     uint32_t words[NSP][PY];
 #pragma unroll
     for (int w = 0; w < NSP; ++w)
@@ -433,8 +443,30 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
         const int nfar = a.plan_count[2 * w + 1];
         if constexpr (GL) {
             // a far sample is one load of the up-sampled frame; its FP32 bit
-            // pattern orders like the value (non-negative samples)
+            // pattern orders like the value (non-negative samples).  A tile
+            // whose far halo lies inside the frame needs no clamp: the sample
+            // of row k is the pixel's own index plus the test's offset
+            // oy * sx + ox, a work-group-uniform base, so the loads take their
+            // address from a scalar base and the row's index (no VALU)
             const int x = tx0 + lx;
+            if (interior) {
+                for (int f = 0; f < nfar; ++f) {
+                    const uint4 r0 = far_rec[32 * w + f];
+                    const int ox1 = (int16_t)(r0.x & 0xffffu), oy1 = (int16_t)(r0.x >> 16);
+                    const int ox2 = (int16_t)(r0.y & 0xffffu), oy2 = (int16_t)(r0.y >> 16);
+                    const uint32_t *b1 = reinterpret_cast<const uint32_t *>(a.L) + (ptrdiff_t)(oy1 * sx + ox1);
+                    const uint32_t *b2 = reinterpret_cast<const uint32_t *>(a.L) + (ptrdiff_t)(oy2 * sx + ox2);
+                    uint32_t va[PY], vb[PY];
+#pragma unroll
+                    for (int k = 0; k < PY; ++k) {
+                        va[k] = b1[pix[k]];
+                        vb[k] = b2[pix[k]];
+                    }
+#pragma unroll
+                    for (int k = 0; k < PY; ++k) words[w][k] |= ((vb[k] - va[k]) >> 31) << r0.z;
+                }
+                continue;
+            }
             for (int f = 0; f < nfar; ++f) {
                 const uint4 r0 = far_rec[32 * w + f];
                 const int ox1 = (int16_t)(r0.x & 0xffffu), oy1 = (int16_t)(r0.x >> 16);
