@@ -97,33 +97,16 @@ __global__ void __launch_bounds__(kThreads) k_gray(const uint8_t *__restrict__ r
 // ---------------------------------------------------------------------------
 // bilinear up-sampling (stand-alone; the matching path uses the fused BRIEF)
 
-// Four consecutive outputs per thread (one 16-byte store where the address
-// is 16-byte aligned), 1024 per work-group and row.
-constexpr int kUpPX = 4;
-
 __global__ void __launch_bounds__(kThreads) k_upsample(const uint8_t *__restrict__ g8,
                                                        const uint32_t *__restrict__ up_x,
                                                        const uint32_t *__restrict__ up_y, int W, int H,
                                                        int sx, int sy, float *__restrict__ L)
 {
-    constexpr int span = kThreads * kUpPX;
-    const int gx = (sx + span - 1) / span;
+    const int gx = (sx + kThreads - 1) / kThreads;
     const int b = xcd_remap(blockIdx.x, gx * sy);
-    const int x0 = (b % gx) * span + kUpPX * threadIdx.x, y = b / gx;
-    if (x0 >= sx) return;
-    const uint32_t uy = up_y[y];
-    float v[kUpPX];
-#pragma unroll
-    for (int q = 0; q < kUpPX; ++q)
-        v[q] = (float)bilinear_q16(g8, W, H, up_x[min(x0 + q, sx - 1)], uy) * (1.0f / 65536.0f);
-    float *out = L + (size_t)y * sx + x0;
-    if (((uintptr_t)out & 15u) == 0 && x0 + kUpPX <= sx) {
-        *reinterpret_cast<float4 *>(out) = make_float4(v[0], v[1], v[2], v[3]);
-    } else {
-#pragma unroll
-        for (int q = 0; q < kUpPX; ++q)
-            if (x0 + q < sx) out[q] = v[q];
-    }
+    const int x = (b % gx) * kThreads + threadIdx.x, y = b / gx;
+    if (x >= sx) return;
+    L[(size_t)y * sx + x] = (float)bilinear_q16(g8, W, H, up_x[x], up_y[y]) * (1.0f / 65536.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -1829,9 +1812,8 @@ hipError_t launch_upsample_lanczos(const Geometry &g, const uint8_t *g8, float *
 
 hipError_t launch_upsample(const Geometry &g, const uint8_t *g8, float *L, hipStream_t st)
 {
-    const unsigned gx = (unsigned)((g.sx + kThreads * kUpPX - 1) / (kThreads * kUpPX));
-    hipLaunchKernelGGL(k_upsample, dim3(gx * (unsigned)g.sy), dim3(kThreads), 0, st, g8, g.up_x, g.up_y, g.W, g.H,
-                       g.sx, g.sy, L);
+    hipLaunchKernelGGL(k_upsample, dim3(blocks_rows(g.sx, g.sy)), dim3(kThreads), 0, st, g8, g.up_x, g.up_y, g.W,
+                       g.H, g.sx, g.sy, L);
     return hipGetLastError();
 }
 
