@@ -32,10 +32,10 @@ from . import Context, DeviceBuffer, Net, derive_params, epe, NET_PARAMS, H2D, _
 from . import synth
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-# trained on 12,000 640x480 pairs, half moving rectangles and half large-motion
+# trained on 20,000 640x480 pairs, half moving rectangles and half large-motion
 # regions, labelled by the method started from the block-matching proposal (8
 # propagation iterations):
-#   python -m briefmatch_amd.learned train --pairs 12000 --epochs 100 --lr 1e-3 --labels seeded \
+#   python -m briefmatch_amd.learned train --pairs 20000 --epochs 100 --lr 1e-3 --labels seeded \
 #       --label-iterations 8 --family mixed --regions-share 0.5 --out proposal_640x480.bnet
 DEFAULT_WEIGHTS = os.path.join(_HERE, "weights", "proposal_640x480.bnet")
 

@@ -345,10 +345,10 @@ def test_trained_proposal_gain():
     around the seeds (radius 16 input pixels), one propagation iteration over
     the jump range 8 (4 level launches; the plain method executes 9).
     Measured with the committed weights, mean EPE: 1920x1080 / u2.9 / 128-bit
-    pairs (seeds 0-7) 0.979 against 1.224 (plain, two iterations) and 1.173
+    pairs (seeds 0-7) 0.970 against 1.224 (plain, two iterations) and 1.173
     (zero network); 1280x720 / u4.9 / 512-bit pairs moving up to 60 px (seeds
-    0-3) 4.66 against 30.4 and 6.11; 320x240 / u1.5 / 32-bit pairs (seeds 0-3)
-    0.580 against 0.361 and 0.876 (motion below 8 px: the plain method's zero
+    0-3) 4.58 against 30.4 and 6.11; 320x240 / u1.5 / 32-bit pairs (seeds 0-3)
+    0.569 against 0.361 and 0.876 (motion below 8 px: the plain method's zero
     start is already close, and the short jump range costs more than the seeds
     give).  Asserted: the target on the 1080p pairs (within 2 % of the plain
     method; here below it), the large-motion gain, and the network's own gain
@@ -378,9 +378,9 @@ def test_network_gain_without_propagation():
     method without it: the seeded initialisation with the cost-volume argmin
     and far match alone) when no propagation iteration follows the seeded
     initialisation, so the flow is set by the seeds and one random search.
-    Measured with the committed weights: mean EPE 2.99 against 7.33 on the
-    1920x1080 / u2.9 / 128-bit pairs (seeds 0-7), 12.1 against 18.0 on the
-    1280x720 pairs moving up to 60 px (seeds 0-3) and 1.21 against 2.46 on the
+    Measured with the committed weights: mean EPE 2.74 against 7.33 on the
+    1920x1080 / u2.9 / 128-bit pairs (seeds 0-7), 12.0 against 18.0 on the
+    1280x720 pairs moving up to 60 px (seeds 0-3) and 1.17 against 2.46 on the
     320x240 pairs (seeds 0-3).  With one iteration the propagation spreads
     either set of seeds over the regions and the two are level
     (test_trained_proposal_gain)."""
@@ -435,8 +435,8 @@ def _staged_learned_epe(ctx, net, p, pr, with_far):
 def test_far_match_gain():
     """With one iteration, the far match lowers the learned path's EPE by at
     least 10 % on 1280x720 pairs moving up to 60 px (beyond the cost volume's
-    40 px; measured 47 % with the committed weights on these two seeds) and
-    costs at most 3 % on the 1920x1080 pairs (measured 1.1 %); the composed
+    40 px; measured 11 % with the committed weights on these two seeds) and
+    costs at most 3 % on the 1920x1080 pairs (measured 0.8 %); the composed
     stages with the far match are bm_flow_pair_learned itself."""
     net = bm.Net(path=learned.DEFAULT_WEIGHTS)
     for name, seeds in (("c3_1280x720_u4.9_n512", (0, 1)), ("c2_1920x1080_u2.9_n128", (0, 1))):
