@@ -604,7 +604,7 @@ int prepare_host(bm_ctx *ctx, const bm_params &p, uint64_t seed, HostTables &t)
     g.iterations = (int)p.iterations;
     g.doubleProp = p.doubleProp ? 1 : 0;
     g.seed = seed;
-    g.level_px = NSP <= 4 ? 2 : 1;
+    g.level_px = 1;  // the autotuner's choice at 320x240, 1080p and 720p x4.9
     g.strip_rows = 256;
     g.brief_frame = 0;
     const float rs_sigma = std::sqrt(std::ceil((float)p.sx * 0.5f / 3.0f));
