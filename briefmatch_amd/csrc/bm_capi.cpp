@@ -364,6 +364,7 @@ int set_pattern_internal(bm_ctx *ctx, const std::vector<int16_t> &pat, int rn_re
     } else {
         ctx->plan.assign((size_t)ctx->g.NSP * 64 * 4, 0u);
         ctx->plan_count.assign((size_t)ctx->g.NSP * 2, 0);
+        ctx->g.brief_frame = 0;  // the frame kernel needs the near plan
     }
     ctx->gen++;
     if (ctx->d_pattern) return upload_pattern(ctx);

@@ -168,6 +168,22 @@ def test_set_pattern_after_unfit_plan():
     assert ctx.tuning().brief_near > 0
     F, _, _ = _descs(ctx, pr.frame1)
     assert np.array_equal(ctx.download_desc(F), _pad(oracle.brief(oL, small, N), ctx.NSP))
+    # the whole-path call in the frame mode, then the wide pattern again:
+    # without a near plan the context falls back to the fused mode, and its
+    # tuning stays settable
+    def pair_descs():
+        r1, r2 = bm.DeviceBuffer(array=pr.frame1), bm.DeviceBuffer(array=pr.frame2)
+        U, V = ctx.image_buffer(), ctx.image_buffer()
+        ctx.flow_pair(r1, r2, U, V)
+        ctx.sync()
+        return bm._download_raw(ctx.frames()[1][0], (ctx.sy, ctx.sx, ctx.NSP), np.uint32)
+    ctx.set_tuning(brief_frame=1)
+    assert np.array_equal(pair_descs(), _pad(oracle.brief(oL, small, N), ctx.NSP))
+    ctx.set_pattern(pat)
+    t = ctx.tuning()
+    assert t.brief_near == 0 and t.brief_frame == 0
+    ctx.set_tuning(**t.as_dict())
+    assert np.array_equal(pair_descs(), _pad(oracle.brief(oL, pat, N), ctx.NSP))
 
 
 @pytest.mark.parametrize("N,double_prop", [(32, True), (128, True), (128, False), (256, True), (512, True)])
