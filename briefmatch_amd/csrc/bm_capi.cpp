@@ -287,8 +287,11 @@ bool plan_brief(bm_ctx *ctx, int rmax, int rn_req)
 
 // Test records of the fused BRIEF.  Near part: per descriptor word 32 records
 // {o1, o2} (two int32) for bits 31 .. 0, the LDS offsets of both samples for a
-// near test (all offsets within the near halo), both 0 for a far test or a
-// bit beyond the descriptor length (the kernel then shifts in a 0).  Far part,
+// near test (all offsets within the near halo) from the thread's top-left
+// halo cell, (RN + oy) * pitch + RN + ox: the kernel's lane base then needs no
+// column constant that the row-pair LDS loads cannot take as an immediate;
+// both 0 for a far test or a bit beyond the descriptor length (the kernel
+// then shifts in a 0).  Far part,
 // after the NSP * 64 near words: per descriptor word up to 32 records {packed
 // int16 offsets of both samples, bit position, 0}.  Counts per word: bits,
 // far tests.  NSP * 192 words in all, within the NSP * 256 of the buffers.
@@ -310,8 +313,8 @@ void build_brief_plan(bm_ctx *ctx)
             const int m = std::max(std::max(std::abs((int)o[0]), std::abs((int)o[1])),
                                    std::max(std::abs((int)o[2]), std::abs((int)o[3])));
             if (m <= RN) {
-                near_rec[(size_t)64 * w + 2 * i] = (uint32_t)(o[1] * pitch + o[0]);
-                near_rec[(size_t)64 * w + 2 * i + 1] = (uint32_t)(o[3] * pitch + o[2]);
+                near_rec[(size_t)64 * w + 2 * i] = (uint32_t)((RN + o[1]) * pitch + RN + o[0]);
+                near_rec[(size_t)64 * w + 2 * i + 1] = (uint32_t)((RN + o[3]) * pitch + RN + o[2]);
             } else {
                 uint32_t *fr = far_rec + ((size_t)32 * w + nfar++) * 4;
                 fr[0] = (uint32_t)(uint16_t)o[0] | ((uint32_t)(uint16_t)o[1] << 16);

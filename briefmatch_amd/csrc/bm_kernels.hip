@@ -289,6 +289,14 @@ __device__ __forceinline__ uint32_t bil_lds(const uint16_t *gt, int x0, uint32_t
     return dot_u16x2(top | (bot << 16), yv.y);
 }
 
+// A raw buffer resource over n bytes from p (dword 3 as CDNA takes it: 32-bit
+// data format, no swizzle, no stride): a load through it takes the base from
+// SGPRs and an unsigned 32-bit offset from one VGPR, and reads 0 at or past n.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_rsrc(const float *p, uint32_t n)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), 0, (int)n, 0x00020000);
+}
+
 // One 64 x TH tile b (row-major tile index) of the fused BRIEF; smem is the
 // work-group's dynamic LDS.  GL: the up-sampled frame is in global memory
 // (a.L, written by the up-sampler before): the tile and its near halo are
@@ -387,16 +395,18 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
         gt = gtw;
     }
 
-    const float *lbase = lt + (ly + RN) * PITCH + lx + RN;
+    const float *lbase = lt + ly * PITCH + lx;  // the thread's cell in the halo-padded tile, less (RN, RN)
     // GL: a tile whose far halo (a.Rf, the largest pattern offset) lies inside
-    // the frame, and the pixel index of each of the thread's rows
+    // the frame, and the byte offset of the pixel of each of the thread's rows
+    // (below 2^32: sx, sy <= 32767), an unsigned 32-bit offset from a scalar
+    // base as the global loads take it
     bool interior = false;
     uint32_t pix[PY];
     if constexpr (GL) {
         interior = tx0 - a.Rf >= 0 && tx0 + kBriefTW - 1 + a.Rf <= sx - 1 && ty0 - a.Rf >= 0 &&
                    ty0 + TH - 1 + a.Rf <= sy - 1;
 #pragma unroll
-        for (int k = 0; k < PY; ++k) pix[k] = (uint32_t)((ty0 + ly + 4 * k) * sx + tx0 + lx);
+        for (int k = 0; k < PY; ++k) pix[k] = (uint32_t)((ty0 + ly + 4 * k) * sx + tx0 + lx) * 4u;
     }
     uint32_t words[NSP][PY];
 #pragma unroll
@@ -405,7 +415,9 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
         for (int k = 0; k < PY; ++k) words[w][k] = 0u;
 
     // The plan holds, per word, 32 near records {o1, o2} (int2) for bits 31 ..
-    // 0: the LDS offsets (row * PITCH + col) of the two samples, both 0 for a
+    // 0: the LDS offsets ((RN + oy) * PITCH + RN + ox) of the two samples from
+    // lbase (so a sample's address is lbase plus one scalar, and the rows k
+    // are immediate offsets of 4 k PITCH floats), both 0 for a
     // bit beyond the descriptor length or a far test; after all words, up to
     // 32 far records per word {packed int16 offsets, bit} (uint4).  A near bit
     // is the sign of b - a on the FP32 bit patterns (samples are non-negative:
@@ -447,20 +459,22 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
             // whose far halo lies inside the frame needs no clamp: the sample
             // of row k is the pixel's own index plus the test's offset
             // oy * sx + ox, a work-group-uniform base, so the loads take their
-            // address from a scalar base and the row's index (no VALU)
+            // address from a buffer resource over the frame from that offset
+            // on (SGPRs) and the row's byte offset (no VALU)
             const int x = tx0 + lx;
             if (interior) {
                 for (int f = 0; f < nfar; ++f) {
                     const uint4 r0 = far_rec[32 * w + f];
                     const int ox1 = (int16_t)(r0.x & 0xffffu), oy1 = (int16_t)(r0.x >> 16);
                     const int ox2 = (int16_t)(r0.y & 0xffffu), oy2 = (int16_t)(r0.y >> 16);
-                    const uint32_t *b1 = reinterpret_cast<const uint32_t *>(a.L) + (ptrdiff_t)(oy1 * sx + ox1);
-                    const uint32_t *b2 = reinterpret_cast<const uint32_t *>(a.L) + (ptrdiff_t)(oy2 * sx + ox2);
+                    const int M = sx * sy, o1 = oy1 * sx + ox1, o2 = oy2 * sx + ox2;
+                    const __amdgpu_buffer_rsrc_t r1 = frame_rsrc(a.L + o1, (uint32_t)(M - o1) * 4u);
+                    const __amdgpu_buffer_rsrc_t r2 = frame_rsrc(a.L + o2, (uint32_t)(M - o2) * 4u);
                     uint32_t va[PY], vb[PY];
 #pragma unroll
                     for (int k = 0; k < PY; ++k) {
-                        va[k] = b1[pix[k]];
-                        vb[k] = b2[pix[k]];
+                        va[k] = __builtin_amdgcn_raw_buffer_load_b32(r1, (int)pix[k], 0, 0);
+                        vb[k] = __builtin_amdgcn_raw_buffer_load_b32(r2, (int)pix[k], 0, 0);
                     }
 #pragma unroll
                     for (int k = 0; k < PY; ++k) words[w][k] |= ((vb[k] - va[k]) >> 31) << r0.z;
