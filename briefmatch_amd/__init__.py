@@ -36,9 +36,11 @@ class BriefMatchError(RuntimeError):
 
 
 class Tuning(C.Structure):
-    """bm_tuning: launch-shape choices; results do not depend on them."""
+    """bm_tuning: launch-shape choices; descriptors and match fields do not
+    depend on them, the flow only through the trilateral filter's summation
+    order (tri_groups, tri_split)."""
     _fields_ = [(n, C.c_int) for n in ("level_px", "strip_rows", "brief_near", "tri_band", "tri_groups",
-                                       "brief_frame")]
+                                       "brief_frame", "tri_split")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

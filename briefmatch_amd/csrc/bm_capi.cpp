@@ -85,6 +85,8 @@ struct bm_ctx {
     int *d_lz_x0 = nullptr, *d_lz_y0 = nullptr;
     float *d_lz_wx = nullptr, *d_lz_wy = nullptr;
     float *d_L = nullptr;  // up-sampled frame of the Lanczos mode
+    float *d_tri_part = nullptr;  // partial sums of the split trilateral filter
+    size_t tri_part_cap = 0;      // its size in floats
     // learned proposal (allocated on first use)
     uint16_t *d_feat = nullptr;            // [ch][cw][128] BF16
     uint16_t *d_nact[2] = {nullptr, nullptr};  // [ch][cw][64] BF16
@@ -611,6 +613,8 @@ int prepare_host(bm_ctx *ctx, const bm_params &p, uint64_t seed, HostTables &t)
     g.level_px = 1;  // the autotuner's choice at 320x240, 1080p and 720p x4.9
     g.strip_rows = 256;
     g.brief_frame = 0;
+    g.tri_split = 1;
+    g.tri_part = nullptr;
     const float rs_sigma = std::sqrt(std::ceil((float)p.sx * 0.5f / 3.0f));
     g.rs_sigma_q16 = (int64_t)std::llround((double)rs_sigma * 65536.0);
 
@@ -1623,6 +1627,8 @@ int tuning_valid(const bm_ctx *ctx, const bm_tuning &t, std::string &why)
     if (t.tri_groups != 0 && t.tri_groups != 2 && t.tri_groups != 4) return why = "tri_groups must be 0, 2 or 4", 0;
     if (t.brief_frame != 0 && t.brief_frame != 1) return why = "brief_frame must be 0 or 1", 0;
     if (t.brief_frame && t.brief_near == 0) return why = "brief_frame 1 needs brief_near > 0", 0;
+    if (t.tri_split != 1 && t.tri_split != 2 && t.tri_split != 4 && t.tri_split != 8)
+        return why = "tri_split must be 1, 2, 4 or 8", 0;
     return 1;
 }
 
@@ -1642,11 +1648,21 @@ int apply_tuning(bm_ctx *ctx, const bm_tuning &t)
         const int r = dev_alloc(ctx, &ctx->d_L, (size_t)ctx->g.sx * ctx->g.sy);
         if (r != BM_OK) return r;
     }
+    if (t.tri_split > 1) {  // three planes of W H partial sums per part
+        const size_t need = (size_t)3 * t.tri_split * ctx->g.W * ctx->g.H;
+        if (need > ctx->tri_part_cap) {
+            const int r = dev_alloc(ctx, &ctx->d_tri_part, need);
+            if (r != BM_OK) return r;
+            ctx->tri_part_cap = need;
+        }
+    }
     ctx->g.level_px = t.level_px;
     ctx->g.strip_rows = t.strip_rows;
     ctx->g.tri_band = t.tri_band;
     ctx->g.tri_groups = t.tri_groups;
     ctx->g.brief_frame = t.brief_frame;
+    ctx->g.tri_split = t.tri_split;
+    ctx->g.tri_part = ctx->d_tri_part;
     return BM_OK;
 }
 
@@ -1659,6 +1675,7 @@ bm_tuning current_tuning(const bm_ctx *ctx)
     t.tri_band = ctx->g.tri_band;
     t.tri_groups = ctx->g.tri_groups;
     t.brief_frame = ctx->g.brief_frame;
+    t.tri_split = ctx->g.tri_split;
     return t;
 }
 
@@ -1711,7 +1728,8 @@ int bm_autotune(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, const cha
             std::istringstream ls(line);
             std::string k;
             bm_tuning t;
-            if (ls >> k >> t.level_px >> t.strip_rows >> t.brief_near >> t.tri_band >> t.tri_groups >> t.brief_frame &&
+            if (ls >> k >> t.level_px >> t.strip_rows >> t.brief_near >> t.tri_band >> t.tri_groups >> t.brief_frame >>
+                    t.tri_split &&
                 k == key) {
                 std::string why;
                 if (tuning_valid(ctx, t, why) && apply_tuning(ctx, t) == BM_OK) {
@@ -1792,9 +1810,32 @@ int bm_autotune(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, const cha
             }
         best = winner;
     }
+    // the trilateral kernel and its split interact too (the split gives the
+    // device work where tiles are few, which is also where the MFMA kernel's
+    // smaller tiles help): timed jointly, unless the pair is not filtered
+    if (ctx->p.filter) {
+        double best_ms = -1.0;
+        bm_tuning winner = best;
+        for (int groups : {0, 2, 4})
+            for (int split : {1, 2, 4, 8}) {
+                if (e != BM_OK) break;
+                bm_tuning t = best;
+                t.tri_groups = groups;
+                t.tri_split = split;
+                std::string why;
+                if (!tuning_valid(ctx, t, why)) continue;
+                double ms = 0.0;
+                e = measure(t, tr, ms);
+                if (e != BM_OK) break;
+                if (best_ms < 0 || ms < best_ms) {
+                    best_ms = ms;
+                    winner = t;
+                }
+            }
+        best = winner;
+    }
     const Knob knobs[] = {{&bm_tuning::level_px, {1, 2, 4}, lv},
                           {&bm_tuning::strip_rows, {64, 128, 256, 512, 1024}, lv},
-                          {&bm_tuning::tri_groups, {0, 2, 4}, tr},
                           {&bm_tuning::tri_band, bands, tr}};
     for (const Knob &kn : knobs) {
         if (e != BM_OK) break;
@@ -1832,7 +1873,7 @@ int bm_autotune(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, const cha
     if (cache_path) {
         std::ofstream o(cache_path, std::ios::app);
         o << key << " " << best.level_px << " " << best.strip_rows << " " << best.brief_near << " " << best.tri_band
-          << " " << best.tri_groups << " " << best.brief_frame << "\n";
+          << " " << best.tri_groups << " " << best.brief_frame << " " << best.tri_split << "\n";
     }
     return BM_OK;
 }
@@ -2031,7 +2072,9 @@ MultiOp group_op(PlanBytes &pb, MultiOp::Kind kind, int a, int bk, const std::ve
 // per-pair path (bm_flow_pair) inside the same sequence.
 bool multi_eligible(const bm_ctx *c)
 {
-    return c->upsampler == BM_UPSAMPLE_BILINEAR && c->g.brief_near > 0 && c->g.jfMax >= 1;
+    // a split trilateral filter sums in another order than the multi-pair
+    // kernel: such a pair keeps its own launches
+    return c->upsampler == BM_UPSAMPLE_BILINEAR && c->g.brief_near > 0 && c->g.jfMax >= 1 && c->g.tri_split <= 1;
 }
 
 // The plan of the multi-pair mode for these buffers: every stage of every

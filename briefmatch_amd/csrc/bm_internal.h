@@ -49,6 +49,8 @@ struct Geometry {
     int tri_band;                 // max source rows per LDS band of the trilateral filter
     int tri_groups;               // trilateral kernel: 0 VALU sums, 2 / 4 MFMA sums with that many 4-row groups
     int brief_frame;              // 1: the whole-path calls up-sample into a frame and BRIEF reads it
+    int tri_split;                // work-groups per trilateral tile (shares of its source rows): 1, 2, 4 or 8
+    float *tri_part;              // their partial sums, 3 tri_split W H floats (tri_split > 1)
 };
 
 // Largest trilateral band (rows) that fits the LDS budget for half window s.

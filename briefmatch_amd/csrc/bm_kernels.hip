@@ -1393,11 +1393,35 @@ __global__ void __launch_bounds__(kThreads) k_median(const float *__restrict__ i
 constexpr int kTriTW = 64, kTriTH = 16, kTriPY = 4;
 constexpr int kTriLdsBytes = 64 * 1024;
 
+// Split of a tile's source rows over K work-groups (K = 1: the whole range):
+// part p takes rows [src0 + p n / K, src0 + (p + 1) n / K) of the n rows, and
+// writes its sums of w, w u and w v per output pixel to partial (three planes
+// of W H floats per part, in that order) for k_trilateral_finish.
+__device__ __forceinline__ void tri_part_rows(int part, int K, int &src0, int &src1)
+{
+    if (K > 1) {
+        const int n = src1 - src0, a = src0;
+        src0 = a + part * n / K;
+        src1 = a + (part + 1) * n / K;
+    }
+}
+
+__device__ __forceinline__ void tri_store_partial(float *__restrict__ partial, int part, int W, int H, size_t o,
+                                                  float sw, float su, float sv)
+{
+    const size_t WH = (size_t)W * H;
+    float *q = partial + (size_t)3 * part * WH + o;
+    q[0] = sw;
+    q[WH] = su;
+    q[2 * WH] = sv;
+}
+
 __device__ __forceinline__ void tri_tile(const float *__restrict__ Ui, const float *__restrict__ Vi,
                                          const float *__restrict__ Um, const float *__restrict__ Vm,
                                          const float *__restrict__ I, float *__restrict__ Uo, float *__restrict__ Vo,
                                          int W, int H, int s, int BR, float a2, float bI,
-                                         const float *__restrict__ spt, int blk, float4 *band)
+                                         const float *__restrict__ spt, int blk, float4 *band, int part = 0,
+                                         int K = 1, float *__restrict__ partial = nullptr)
 {
     const int pitch = kTriTW + 2 * s;
     // the column part of the spatial weight (2s entries) after the band
@@ -1419,7 +1443,8 @@ __device__ __forceinline__ void tri_tile(const float *__restrict__ Ui, const flo
         au[k] = av[k] = aw[k] = 0.0f;
     }
 
-    const int src0 = ty0 - s, src1 = ty0 + kTriTH + s;  // source rows of the tile
+    int src0 = ty0 - s, src1 = ty0 + kTriTH + s;        // source rows of the tile
+    tri_part_rows(part, K, src0, src1);                 // this work-group's share of them
     const int my0 = y0 - s, my1 = y0 + kTriPY - 1 + s;  // source rows this wave needs
     for (int band0 = src0; band0 < src1; band0 += BR) {
         __syncthreads();
@@ -1483,9 +1508,13 @@ __device__ __forceinline__ void tri_tile(const float *__restrict__ Ui, const flo
         for (int k = 0; k < kTriPY; ++k) {
             const int y = y0 + k;
             if (y < H) {
-                const float den = fmaxf(1e-5f, aw[k]);
-                Uo[(size_t)y * W + x] = au[k] / den;
-                Vo[(size_t)y * W + x] = av[k] / den;
+                if (K > 1) {
+                    tri_store_partial(partial, part, W, H, (size_t)y * W + x, aw[k], au[k], av[k]);
+                } else {
+                    const float den = fmaxf(1e-5f, aw[k]);
+                    Uo[(size_t)y * W + x] = au[k] / den;
+                    Vo[(size_t)y * W + x] = av[k] / den;
+                }
             }
         }
     }
@@ -1495,11 +1524,34 @@ __global__ void __launch_bounds__(kThreads) k_trilateral(const float *__restrict
                                                          const float *__restrict__ Um, const float *__restrict__ Vm,
                                                          const float *__restrict__ I, float *__restrict__ Uo,
                                                          float *__restrict__ Vo, int W, int H, int s, int BR,
-                                                         float a2, float bI, const float *__restrict__ spt)
+                                                         float a2, float bI, const float *__restrict__ spt, int K,
+                                                         float *__restrict__ partial)
 {
     extern __shared__ float4 band[];
     const int gxb = (W + kTriTW - 1) / kTriTW, gyb = (H + kTriTH - 1) / kTriTH;
-    tri_tile(Ui, Vi, Um, Vm, I, Uo, Vo, W, H, s, BR, a2, bI, spt, xcd_remap(blockIdx.x, gxb * gyb), band);
+    // K work-groups per tile (the parts of its source rows), adjacent in the remapped order
+    const int lb = xcd_remap(blockIdx.x, gxb * gyb * K);
+    tri_tile(Ui, Vi, Um, Vm, I, Uo, Vo, W, H, s, BR, a2, bI, spt, lb / K, band, lb % K, K, partial);
+}
+
+// The split trilateral's last step: per pixel the parts' sums, added in part
+// order, and the normalisation of the unsplit kernels.
+__global__ void __launch_bounds__(kThreads) k_trilateral_finish(const float *__restrict__ partial, int K, int W,
+                                                                int H, float *__restrict__ Uo,
+                                                                float *__restrict__ Vo)
+{
+    const size_t WH = (size_t)W * H, i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= WH) return;
+    float sw = 0.0f, su = 0.0f, sv = 0.0f;
+    for (int p = 0; p < K; ++p) {
+        const float *q = partial + (size_t)3 * p * WH + i;
+        sw += q[0];
+        su += q[WH];
+        sv += q[2 * WH];
+    }
+    const float den = fmaxf(1e-5f, sw);
+    Uo[i] = su / den;
+    Vo[i] = sv / den;
 }
 
 // Cross-trilateral filter with the three weighted sums on the matrix cores.
@@ -1556,7 +1608,8 @@ __device__ __forceinline__ void tri_mfma_tile(const float *__restrict__ Ui, cons
                                               const float *__restrict__ Um, const float *__restrict__ Vm,
                                               const float *__restrict__ I, float *__restrict__ Uo,
                                               float *__restrict__ Vo, int W, int H, int s, int BR, float a2, float bI,
-                                              const float *__restrict__ spt, int blk, float4 *band)
+                                              const float *__restrict__ spt, int blk, float4 *band, int part = 0,
+                                              int K = 1, float *__restrict__ partial = nullptr)
 {
     constexpr int TH = 4 * R;
     const int pitch = kTriTW + 2 * s;
@@ -1582,7 +1635,8 @@ __device__ __forceinline__ void tri_mfma_tile(const float *__restrict__ Ui, cons
         acc[g] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     }
 
-    const int src0 = ty0 - s, src1 = ty0 + TH + s;  // source rows of the tile
+    int src0 = ty0 - s, src1 = ty0 + TH + s;  // source rows of the tile
+    tri_part_rows(part, K, src0, src1);       // this work-group's share of them
     for (int band0 = src0; band0 < src1; band0 += BR) {
         __syncthreads();
         for (int r = wave; r < BR; r += 4) {
@@ -1612,6 +1666,20 @@ __device__ __forceinline__ void tri_mfma_tile(const float *__restrict__ Ui, cons
         }
     }
     // lane 4b+j holds, in register i, row i's sum of w * (1, u, v, I)_j * es
+    if (K > 1) {  // lanes 4b+0..2 hold the sums of w, w u, w v: one plane each
+        if (x < W && k < 3) {
+            const size_t WH = (size_t)W * H;
+            float *q = partial + ((size_t)3 * part + k) * WH;
+#pragma unroll
+            for (int g = 0; g < R; ++g)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int y = ty0 + 4 * g + i;
+                    if (y < H) q[(size_t)y * W + x] = acc[g][i];
+                }
+        }
+        return;
+    }
 #pragma unroll
     for (int g = 0; g < R; ++g) {
         float den[4];
@@ -1635,11 +1703,13 @@ __global__ void __launch_bounds__(kThreads) k_trilateral_mfma(const float *__res
                                                               const float *__restrict__ Vm,
                                                               const float *__restrict__ I, float *__restrict__ Uo,
                                                               float *__restrict__ Vo, int W, int H, int s, int BR,
-                                                              float a2, float bI, const float *__restrict__ spt)
+                                                              float a2, float bI, const float *__restrict__ spt, int K,
+                                                              float *__restrict__ partial)
 {
     extern __shared__ float4 band[];
     const int gxb = (W + kTriTW - 1) / kTriTW, gyb = (H + 4 * R - 1) / (4 * R);
-    tri_mfma_tile<R>(Ui, Vi, Um, Vm, I, Uo, Vo, W, H, s, BR, a2, bI, spt, xcd_remap(blockIdx.x, gxb * gyb), band);
+    const int lb = xcd_remap(blockIdx.x, gxb * gyb * K);
+    tri_mfma_tile<R>(Ui, Vi, Um, Vm, I, Uo, Vo, W, H, s, BR, a2, bI, spt, lb / K, band, lb % K, K, partial);
 }
 
 // plain 16-byte-per-lane copy, grid-stride
@@ -2092,20 +2162,30 @@ hipError_t launch_trilateral(const Geometry &g, const float *Ui, const float *Vi
     if (BR > g.tri_band) BR = g.tri_band;
     const size_t lds = (size_t)BR * pitch * sizeof(float4) + (size_t)2 * s * sizeof(float);
     const int gxb = (g.W + kTriTW - 1) / kTriTW;
+    // tri_split > 1: that many work-groups per tile, each over a share of its
+    // source rows, then the finishing kernel (partial sums in g.tri_part)
+    const int K = g.tri_split > 1 ? g.tri_split : 1;
+    if (K > 1 && !g.tri_part) return hipErrorInvalidValue;
+    float *part = K > 1 ? g.tri_part : nullptr;
     switch (g.tri_groups) {
     case 0:
-        hipLaunchKernelGGL(k_trilateral, dim3(gxb * ((g.H + kTriTH - 1) / kTriTH)), dim3(kThreads), lds, st, Ui, Vi,
-                           Um, Vm, I, Uo, Vo, g.W, g.H, s, BR, g.tri_a2, g.tri_b, g.tri_spatial);
+        hipLaunchKernelGGL(k_trilateral, dim3(gxb * ((g.H + kTriTH - 1) / kTriTH) * K), dim3(kThreads), lds, st, Ui,
+                           Vi, Um, Vm, I, Uo, Vo, g.W, g.H, s, BR, g.tri_a2, g.tri_b, g.tri_spatial, K, part);
         break;
     case 2:
-        hipLaunchKernelGGL(k_trilateral_mfma<2>, dim3(gxb * ((g.H + 7) / 8)), dim3(kThreads), lds, st, Ui, Vi, Um, Vm,
-                           I, Uo, Vo, g.W, g.H, s, BR, g.tri_a2, g.tri_b, g.tri_spatial);
+        hipLaunchKernelGGL(k_trilateral_mfma<2>, dim3(gxb * ((g.H + 7) / 8) * K), dim3(kThreads), lds, st, Ui, Vi, Um,
+                           Vm, I, Uo, Vo, g.W, g.H, s, BR, g.tri_a2, g.tri_b, g.tri_spatial, K, part);
         break;
     case 4:
-        hipLaunchKernelGGL(k_trilateral_mfma<4>, dim3(gxb * ((g.H + 15) / 16)), dim3(kThreads), lds, st, Ui, Vi, Um,
-                           Vm, I, Uo, Vo, g.W, g.H, s, BR, g.tri_a2, g.tri_b, g.tri_spatial);
+        hipLaunchKernelGGL(k_trilateral_mfma<4>, dim3(gxb * ((g.H + 15) / 16) * K), dim3(kThreads), lds, st, Ui, Vi,
+                           Um, Vm, I, Uo, Vo, g.W, g.H, s, BR, g.tri_a2, g.tri_b, g.tri_spatial, K, part);
         break;
     default: return hipErrorInvalidValue;
+    }
+    if (K > 1) {
+        const size_t WH = (size_t)g.W * g.H;
+        hipLaunchKernelGGL(k_trilateral_finish, dim3((unsigned)((WH + kThreads - 1) / kThreads)), dim3(kThreads), 0, st,
+                           part, K, g.W, g.H, Uo, Vo);
     }
     return hipGetLastError();
 }

@@ -1,11 +1,12 @@
 """Time one stage of a workload under chosen launch shapes.
 
     python -m briefmatch_amd.stagebench trilateral --workload c2_1920x1080_u2.9_n128 \\
-        --groups 0 2 4 --bands 8 --reps 3
+        --groups 0 2 4 --bands 8 --split 1 2 4 --reps 3
     python -m briefmatch_amd.stagebench brief --workload c2_1920x1080_u2.9_n128 --near 32 40 --reps 5
 
 trilateral: synthetic device buffers of the workload's input resolution (flow
-planes N(0, 3) px, an integer gray guide 0..255), per (kernel, band) setting.
+planes N(0, 3) px, an integer gray guide 0..255), per (kernel, band, split)
+setting.
 brief: the fused up-sampling + BRIEF kernel on the gray first frame of the
 workload's seed-0 pair, per near halo.  Each setting runs --reps untimed
 launches, then --reps timed ones, and prints ms per launch.  It is a profiling
@@ -35,16 +36,18 @@ def trilateral(args):
     bands = args.bands or [ctx.tuning().tri_band]
     for groups in args.groups:
         for band in bands:
-            ctx.set_tuning(tri_groups=groups, tri_band=band)
-            for _ in range(args.reps):
-                ctx.trilateral(*ins, Uo, Vo)
-            ctx.sync()
-            t = time.perf_counter()
-            for _ in range(args.reps):
-                ctx.trilateral(*ins, Uo, Vo)
-            ctx.sync()
-            ms = (time.perf_counter() - t) / args.reps * 1e3
-            print("trilateral %s groups %d band %2d  %.3f ms" % (args.workload, groups, band, ms), flush=True)
+            for split in args.split:
+                ctx.set_tuning(tri_groups=groups, tri_band=band, tri_split=split)
+                for _ in range(args.reps):
+                    ctx.trilateral(*ins, Uo, Vo)
+                ctx.sync()
+                t = time.perf_counter()
+                for _ in range(args.reps):
+                    ctx.trilateral(*ins, Uo, Vo)
+                ctx.sync()
+                ms = (time.perf_counter() - t) / args.reps * 1e3
+                print("trilateral %s groups %d band %2d split %d  %.3f ms" % (args.workload, groups, band, split, ms),
+                      flush=True)
 
 
 def brief(args):
@@ -75,6 +78,7 @@ def main():
     ap.add_argument("stage", choices=["trilateral", "brief"])
     ap.add_argument("--workload", default="c2_1920x1080_u2.9_n128", choices=sorted(synth.WORKLOADS))
     ap.add_argument("--groups", type=int, nargs="+", default=[0, 2, 4], help="tri_groups values")
+    ap.add_argument("--split", type=int, nargs="+", default=[1], help="tri_split values")
     ap.add_argument("--bands", type=int, nargs="*", default=None, help="tri_band values (default: the context's)")
     ap.add_argument("--near", type=int, nargs="*", default=None, help="brief_near values (default: the context's)")
     ap.add_argument("--reps", type=int, default=3)

@@ -231,7 +231,10 @@ int bm_flow_pair_static(const bm_params *params, uint64_t seed, const uint8_t *r
 /* ---- tuning ------------------------------------------------------------------ */
 
 /* Launch-shape choices of one context.  Every valid tuning computes identical
- * results; only the speed changes. */
+ * descriptors and match fields; the flow is identical too except that the
+ * trilateral filter's kernel choices (tri_groups, tri_split) add its weighted
+ * sums in different orders (agreeing to far within the 1e-3 px tolerance).
+ * Only the speed changes. */
 typedef struct bm_tuning {
     int level_px;     /* pixels per thread of the jump-flooding kernels: 1, 2 (N <= 256) or 4 (N <= 128);
                          above 256 bits they run two threads per pixel whatever the value */
@@ -246,6 +249,9 @@ typedef struct bm_tuning {
                          the BRIEF kernel then copies its tile from that frame and reads its far tests
                          from it (needs brief_near > 0); a new context takes 1 when its matched frame
                          has at least 2^21 pixels */
+    int tri_split;    /* work-groups per trilateral tile: 1, 2, 4 or 8; above 1 each sums a share of the
+                         tile's source rows into a context buffer (3 tri_split W H floats) and a
+                         finishing kernel adds the shares, which fills the device on small frames */
 } bm_tuning;
 
 int bm_get_tuning(const bm_ctx *ctx, bm_tuning *out);

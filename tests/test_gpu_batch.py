@@ -1,8 +1,10 @@
 """Tuning invariance, the autotuner and batches of uneven pairs on the GPU.
 
-Every valid tuning must give bit-identical fields and flows; a batch (per-pair
-launches or one hipGraph) must give, for pair i, exactly bm_flow_pair(ctx_i,
-pair = i), whose field in turn equals the CPU oracle."""
+Every valid tuning must give bit-identical fields and flows, except for the
+trilateral filter's kernel and split, which add the same terms in another
+order; a batch (per-pair launches or one hipGraph) must give, for pair i,
+exactly bm_flow_pair(ctx_i, pair = i), whose field in turn equals the CPU
+oracle."""
 import itertools
 
 import numpy as np
@@ -39,14 +41,15 @@ def test_tuning_does_not_change_results(N):
         for a, b in zip(ref, got):
             assert np.array_equal(a, b), (lp, strip, near, band, frame)
     ctx.set_tuning(brief_frame=0)
-    # the trilateral kernels sum in different orders: equal within the flow tolerance
+    # the trilateral kernels and splits sum in different orders: equal within the flow tolerance
     for groups in (0, 2, 4):
-        ctx.set_tuning(tri_groups=groups)
-        got = _flow(ctx, pr)
-        for a, b in zip(ref[:2], got[:2]):
-            assert np.array_equal(a, b), groups
-        for a, b in zip(ref[2:], got[2:]):
-            assert np.abs(a - b).max() <= 1e-3, groups
+        for split in (1, 2, 8):
+            ctx.set_tuning(tri_groups=groups, tri_split=split)
+            got = _flow(ctx, pr)
+            for a, b in zip(ref[:2], got[:2]):
+                assert np.array_equal(a, b), (groups, split)
+            for a, b in zip(ref[2:], got[2:]):
+                assert np.abs(a - b).max() <= 1e-3, (groups, split)
 
 
 def test_tuning_rejects_invalid():
@@ -54,6 +57,7 @@ def test_tuning_rejects_invalid():
     ctx = bm.Context(p)
     for kw in ({"level_px": 3}, {"level_px": 2}, {"level_px": 4}, {"strip_rows": 8}, {"brief_near": 12},
                {"tri_band": 0}, {"tri_groups": 1}, {"tri_groups": 8}, {"brief_frame": 2},
+               {"tri_split": 0}, {"tri_split": 3}, {"tri_split": 16},
                {"brief_near": 0, "brief_frame": 1}):
         with pytest.raises(bm.BriefMatchError):
             ctx.set_tuning(**kw)
@@ -74,8 +78,12 @@ def test_autotune_and_cache(tmp_path):
     t2 = ctx2.autotune(r1, r2, cache)                  # served from the cache
     assert t2.as_dict() == t.as_dict()
     assert len(open(cache).read().splitlines()) == 1
-    for a, b in zip(ref, _flow(ctx, pr)):
+    got = _flow(ctx, pr)
+    for a, b in zip(ref[:2], got[:2]):
         assert np.array_equal(a, b)
+    # the flow up to the trilateral filter's summation order (kernel and split)
+    for a, b in zip(ref[2:], got[2:]):
+        assert np.abs(a - b).max() <= 1e-3
 
 
 def _batch_workload(n, seed):
@@ -138,7 +146,7 @@ def test_batch_equals_per_pair_and_oracle():
 
 def test_batch_graph_follows_configuration_changes():
     """A captured batch graph bakes in launch arguments (near halo, level
-    kernel shape, up-sampler) and the BRIEF plan's layout: changing a member
+    kernel shape, up-sampler, trilateral split) and the BRIEF plan's layout: changing a member
     context's tuning, pattern or up-sampler between two graph launches must
     re-capture, so each graph result equals the per-pair launches under the
     same configuration."""
@@ -169,14 +177,17 @@ def test_batch_graph_follows_configuration_changes():
                    lambda: c.set_pattern(np.random.default_rng(3 + mode).integers(-12, 13, 4 * c.params.N)
                                          .astype(np.int16)),
                    lambda: c.set_upsampler(bm.UPSAMPLE_LANCZOS3),   # the multi mode's per-pair path for this pair
-                   lambda: c.set_upsampler(bm.UPSAMPLE_BILINEAR)]
+                   lambda: c.set_upsampler(bm.UPSAMPLE_BILINEAR),
+                   lambda: c.set_tuning(tri_split=4),   # the split filter: the multi mode's per-pair path
+                   lambda: c.set_tuning(tri_split=1)]
         for k, change in enumerate(changes):
             before = run(mode)
             change()
             got = run(mode)
             want = run(bm.BATCH_PER_PAIR)
             assert same(got, want), (mode, k)
-            assert not same(before[1:2], got[1:2]) or k == 0, (mode, k)  # pattern / up-sampler change the result
+            # pattern / up-sampler change the result
+            assert not same(before[1:2], got[1:2]) or k == 0 or k >= 4, (mode, k)
 
 
 def test_batch_schedules_and_autotune():

@@ -332,8 +332,9 @@ def test_downsample_paths(W, H, up, sigma_rad):
 
 @pytest.mark.parametrize("W,H,sS", [(150, 100, 6.0), (67, 45, 12.0), (40, 30, 20.0), (129, 17, 3.0)])
 def test_trilateral_kernels(W, H, sS):
-    """Every trilateral kernel (VALU sums; MFMA sums with 2 or 4 row groups) against
-    the oracle on ragged frames, windows up to wider than the frame."""
+    """Every trilateral kernel (VALU sums; MFMA sums with 2 or 4 row groups),
+    whole or with each tile's source rows split over 2, 4 or 8 work-groups,
+    against the oracle on ragged frames, windows up to wider than the frame."""
     say("trilateral kernels", W, H, sS)
     rng = np.random.default_rng(W * 1000 + H)
     p = bm.derive_params(W, H, 1.0, 1.0, N=32)
@@ -350,17 +351,19 @@ def test_trilateral_kernels(W, H, sS):
     ins = [bm.DeviceBuffer(array=a) for a in (U, V, Um, Vm, guide)]
     got = {}
     for groups in (0, 2, 4):
-        ctx.set_tuning(tri_groups=groups)
-        Uo, Vo = ctx.image_buffer(), ctx.image_buffer()
-        ctx.trilateral(*ins, Uo, Vo)
-        ctx.sync()
-        gU, gV = Uo.download((H, W), np.float32), Vo.download((H, W), np.float32)
-        eu, ev = np.abs(gU - tU).max(), np.abs(gV - tV).max()
-        print("[tri]", W, H, sS, groups, eu, ev, flush=True)
-        assert eu <= FLOW_TOL and ev <= FLOW_TOL, (groups, eu, ev)
-        got[groups] = (gU, gV)
-    for groups in (2, 4):
-        assert np.abs(got[groups][0] - got[0][0]).max() <= FLOW_TOL
+        for split in (1, 2, 4, 8):
+            ctx.set_tuning(tri_groups=groups, tri_split=split)
+            Uo, Vo = ctx.image_buffer(), ctx.image_buffer()
+            ctx.trilateral(*ins, Uo, Vo)
+            ctx.sync()
+            gU, gV = Uo.download((H, W), np.float32), Vo.download((H, W), np.float32)
+            eu, ev = np.abs(gU - tU).max(), np.abs(gV - tV).max()
+            print("[tri]", W, H, sS, groups, split, eu, ev, flush=True)
+            assert eu <= FLOW_TOL and ev <= FLOW_TOL, (groups, split, eu, ev)
+            got[groups, split] = (gU, gV)
+    for key, (gU, gV) in got.items():
+        # every variant sums the same terms in another order
+        assert np.abs(gU - got[0, 1][0]).max() <= FLOW_TOL and np.abs(gV - got[0, 1][1]).max() <= FLOW_TOL, key
 
 
 @pytest.mark.parametrize("W,H,sS", [(160, 120, None), (97, 61, 6.0)])
@@ -385,14 +388,15 @@ def test_trilateral_large_flow(W, H, sS):
     tU, tV = oracle.trilateral(U, V, Um, Vm, guide, p.sEPE ** 2, p.sI ** 2, p.sS ** 2)
     ins = [bm.DeviceBuffer(array=a) for a in (U, V, Um, Vm, guide)]
     for groups in (0, 2, 4):
-        ctx.set_tuning(tri_groups=groups)
-        Uo, Vo = ctx.image_buffer(), ctx.image_buffer()
-        ctx.trilateral(*ins, Uo, Vo)
-        ctx.sync()
-        eu = np.abs(Uo.download((H, W), np.float32) - tU).max()
-        ev = np.abs(Vo.download((H, W), np.float32) - tV).max()
-        print("[tri large]", W, H, sS, groups, eu, ev, flush=True)
-        assert eu <= FLOW_TOL and ev <= FLOW_TOL, (groups, eu, ev)
+        for split in (1, 4):
+            ctx.set_tuning(tri_groups=groups, tri_split=split)
+            Uo, Vo = ctx.image_buffer(), ctx.image_buffer()
+            ctx.trilateral(*ins, Uo, Vo)
+            ctx.sync()
+            eu = np.abs(Uo.download((H, W), np.float32) - tU).max()
+            ev = np.abs(Vo.download((H, W), np.float32) - tV).max()
+            print("[tri large]", W, H, sS, groups, split, eu, ev, flush=True)
+            assert eu <= FLOW_TOL and ev <= FLOW_TOL, (groups, split, eu, ev)
 
 
 @pytest.mark.parametrize("filt", [1, 0])
