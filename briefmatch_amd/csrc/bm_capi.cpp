@@ -1651,6 +1651,17 @@ int apply_tuning(bm_ctx *ctx, const bm_tuning &t)
     if (t.tri_split > 1) {  // three planes of W H partial sums per part
         const size_t need = (size_t)3 * t.tri_split * ctx->g.W * ctx->g.H;
         if (need > ctx->tri_part_cap) {
+            // the smaller buffer goes (the context's streams are idle, and a
+            // captured batch graph that holds it is re-captured: generation)
+            if (ctx->d_tri_part) {
+                const auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), (void *)ctx->d_tri_part);
+                if (it != ctx->allocs.end()) ctx->allocs.erase(it);
+                (void)hipFree(ctx->d_tri_part);
+                ctx->d_tri_part = nullptr;
+                ctx->g.tri_part = nullptr;
+                ctx->g.tri_split = 1;
+                ctx->tri_part_cap = 0;
+            }
             const int r = dev_alloc(ctx, &ctx->d_tri_part, need);
             if (r != BM_OK) return r;
             ctx->tri_part_cap = need;
