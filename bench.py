@@ -227,6 +227,12 @@ def measure_pair(a, bm, name, tune_cache=None):
 
     s_tri = int(np.ceil(2.5 * float(np.sqrt(np.float32(p.sS) * np.float32(p.sS)))))
     work = rl.stage_work(p, ctx.NSP, s_tri, tuning.get("brief_frame", 0))
+    # the per-frame stages' work is per frame; one launch may cover both frames
+    # of the pair (the gray conversion, and BRIEF in its fused mode)
+    for st in ("gray", "brief", "upsample"):
+        if st in work and prof.get(st, (0, 0))[1]:
+            frames = 2.0 * a.steps / prof[st][1]
+            work[st] = {k: v * frames for k, v in work[st].items()}
     stages = {}
     for st, (ms, n) in prof.items():
         if n == 0:

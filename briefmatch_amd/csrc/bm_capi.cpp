@@ -460,6 +460,19 @@ int describe(bm_ctx *ctx, const uint8_t *g8, uint32_t *F)
     return BM_OK;
 }
 
+// Descriptors of both frames of a pair: in the fused bilinear mode one launch
+// over both frames' tiles (small frames leave most of the device idle per
+// frame), else describe() per frame.
+int describe_pair(bm_ctx *ctx, const uint8_t *g8a, uint32_t *Fa, const uint8_t *g8b, uint32_t *Fb)
+{
+    if (ctx->upsampler == BM_UPSAMPLE_BILINEAR && !ctx->g.brief_frame && ctx->g.brief_near > 0) {
+        BM_LAUNCH(ctx, BM_STAGE_BRIEF, bm::launch_brief_pair(ctx->g, g8a, Fa, g8b, Fb, ctx->stream));
+        return BM_OK;
+    }
+    const int r = describe(ctx, g8a, Fa);
+    return r ? r : describe(ctx, g8b, Fb);
+}
+
 // Ping-pong propagation of one iteration over the context's field buffers,
 // starting from ctx->field; the random search of `iteration` is fused into the
 // last level when fuse_rs.
@@ -694,10 +707,9 @@ void bind_tables(bm_ctx *ctx)
 int run_pair(bm_ctx *ctx, const uint8_t *rgb1, const uint8_t *rgb2, uint32_t pair, float *U, float *V)
 {
     const bm::Geometry &g = ctx->g;
-    BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb1, ctx->d_g8[0], ctx->d_g32[0], ctx->stream));
-    BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb2, ctx->d_g8[1], nullptr, ctx->stream));
-    int r = describe(ctx, ctx->d_g8[0], ctx->d_F[0]);
-    if (!r) r = describe(ctx, ctx->d_g8[1], ctx->d_F[1]);
+    BM_LAUNCH(ctx, BM_STAGE_GRAY,
+              bm::launch_gray_pair(g, rgb1, ctx->d_g8[0], ctx->d_g32[0], rgb2, ctx->d_g8[1], ctx->stream));
+    int r = describe_pair(ctx, ctx->d_g8[0], ctx->d_F[0], ctx->d_g8[1], ctx->d_F[1]);
     if (r) return r;
     r = run_match(ctx, ctx->d_F[0], ctx->d_F[1], 1, nullptr, pair);
     if (r) return r;
@@ -1373,8 +1385,8 @@ int bm_flow_pair_learned(bm_ctx *ctx, bm_net *net, const uint8_t *rgb1, const ui
     if (r) return r;
     const bm::Geometry &g = ctx->g;
     const int cw = g.W / bm::net::kCell, ch = g.H / bm::net::kCell;
-    BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb1, ctx->d_g8[0], ctx->d_g32[0], ctx->stream));
-    BM_LAUNCH(ctx, BM_STAGE_GRAY, bm::launch_gray(g, rgb2, ctx->d_g8[1], nullptr, ctx->stream));
+    BM_LAUNCH(ctx, BM_STAGE_GRAY,
+              bm::launch_gray_pair(g, rgb1, ctx->d_g8[0], ctx->d_g32[0], rgb2, ctx->d_g8[1], ctx->stream));
     // the proposal (features, network, offsets) needs only the gray frames: it
     // runs on the side stream while the descriptor kernels run on the main one
     hipStream_t side = ctx->side;
@@ -1391,8 +1403,7 @@ int bm_flow_pair_learned(bm_ctx *ctx, bm_net *net, const uint8_t *rgb1, const ui
     BM_LAUNCH_ON(ctx, BM_STAGE_PROPOSAL, side,
                  launch_offsets(ctx, ctx->d_nout, ctx->d_feat, ctx->d_far, ctx->d_coff, side));
     BM_HIP(ctx, hipEventRecord(ctx->join, side));
-    r = describe(ctx, ctx->d_g8[0], ctx->d_F[0]);
-    if (!r) r = describe(ctx, ctx->d_g8[1], ctx->d_F[1]);
+    r = describe_pair(ctx, ctx->d_g8[0], ctx->d_F[0], ctx->d_g8[1], ctx->d_F[1]);
     if (r) return r;
     BM_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->join, 0));
     const int dst = ctx->field;

@@ -57,9 +57,15 @@ struct Geometry {
 int tri_max_band(int s);
 
 hipError_t launch_gray(const Geometry &g, const uint8_t *rgb, uint8_t *g8, float *g32, hipStream_t st);
+// both frames of a pair in one launch (FP32 copy of the first only)
+hipError_t launch_gray_pair(const Geometry &g, const uint8_t *rgb1, uint8_t *g8a, float *g32a, const uint8_t *rgb2,
+                            uint8_t *g8b, hipStream_t st);
 hipError_t launch_upsample(const Geometry &g, const uint8_t *g8, float *L, hipStream_t st);
 hipError_t launch_upsample_lanczos(const Geometry &g, const uint8_t *g8, float *L, hipStream_t st);
 hipError_t launch_brief(const Geometry &g, const float *L, const uint8_t *g8, uint32_t *F, hipStream_t st);
+// the fused BRIEF (bilinear from the gray frame, brief_near > 0) of both frames in one launch
+hipError_t launch_brief_pair(const Geometry &g, const uint8_t *g8a, uint32_t *Fa, const uint8_t *g8b, uint32_t *Fb,
+                             hipStream_t st);
 // rs_iteration >= 0 inserts that iteration's random search after the initial candidate.
 hipError_t launch_init(const Geometry &g, const uint32_t *F1, const uint32_t *F2, int first,
                        const uint32_t *prev_off, uint32_t *off, uint16_t *cost, int rs_iteration, uint32_t pair,

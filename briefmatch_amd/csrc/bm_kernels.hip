@@ -94,6 +94,18 @@ __global__ void __launch_bounds__(kThreads) k_gray(const uint8_t *__restrict__ r
     gray_tile(rgb, g8, g32, n, blockIdx.x);
 }
 
+// Both frames of a pair in one launch (the first also as FP32): work-groups
+// [0, nb) take the first frame, [nb, 2 nb) the second.
+__global__ void __launch_bounds__(kThreads) k_gray_pair(const uint8_t *__restrict__ rgb1, uint8_t *__restrict__ g8a,
+                                                        float *__restrict__ g32a, const uint8_t *__restrict__ rgb2,
+                                                        uint8_t *__restrict__ g8b, int n, int nb)
+{
+    if ((int)blockIdx.x < nb)
+        gray_tile(rgb1, g8a, g32a, n, blockIdx.x);
+    else
+        gray_tile(rgb2, g8b, nullptr, n, blockIdx.x - nb);
+}
+
 // ---------------------------------------------------------------------------
 // bilinear up-sampling (stand-alone; the matching path uses the fused BRIEF)
 
@@ -535,6 +547,25 @@ __global__ void __launch_bounds__(kThreads) k_brief_lds(BriefJob a)
     constexpr int TH = 4 * PY;
     const int gxb = (a.sx + kBriefTW - 1) / kBriefTW, gyb = (a.sy + TH - 1) / TH;
     brief_tile<NSP, RN, PY, GL>(a, xcd_remap(blockIdx.x, gxb * gyb), smem);
+}
+
+// The fused kernel over both frames of a pair in one launch (their jobs differ
+// only in the gray frame and the descriptors): for frames whose tiles alone
+// leave most of the device idle.
+template <int NSP, int RN, int PY>
+__global__ void __launch_bounds__(kThreads) k_brief_lds_pair(BriefJob a, const uint8_t *__restrict__ g8b,
+                                                             uint32_t *__restrict__ Fb)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int TH = 4 * PY;
+    const int gxb = (a.sx + kBriefTW - 1) / kBriefTW, gyb = (a.sy + TH - 1) / TH, tiles = gxb * gyb;
+    int b = xcd_remap(blockIdx.x, 2 * tiles);
+    if (b >= tiles) {
+        a.g8 = g8b;
+        a.F = Fb;
+        b -= tiles;
+    }
+    brief_tile<NSP, RN, PY, false>(a, b, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -1886,6 +1917,14 @@ hipError_t launch_gray(const Geometry &g, const uint8_t *rgb, uint8_t *g8, float
     return hipGetLastError();
 }
 
+hipError_t launch_gray_pair(const Geometry &g, const uint8_t *rgb1, uint8_t *g8a, float *g32a, const uint8_t *rgb2,
+                            uint8_t *g8b, hipStream_t st)
+{
+    const int n = g.W * g.H, nb = (n + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL(k_gray_pair, dim3(2 * nb), dim3(kThreads), 0, st, rgb1, g8a, g32a, rgb2, g8b, n, nb);
+    return hipGetLastError();
+}
+
 hipError_t launch_upsample_lanczos(const Geometry &g, const uint8_t *g8, float *L, hipStream_t st)
 {
     if (!g.lz_x0 || !g.lz_wx || !g.lz_y0 || !g.lz_wy) return hipErrorInvalidValue;
@@ -1934,6 +1973,20 @@ static hipError_t launch_brief_lds_t(const Geometry &g, const float *L, const ui
     return hipGetLastError();
 }
 
+template <int NSP, int RN>
+static hipError_t launch_brief_pair_t(const Geometry &g, const uint8_t *g8a, uint32_t *Fa, const uint8_t *g8b,
+                                      uint32_t *Fb, hipStream_t st)
+{
+    constexpr int PY = NSP <= 4 ? 8 : 4;
+    constexpr int TH = 4 * PY;
+    const BriefJob a{g8a, g.up_x, g.up_y, g.brief_plan, g.brief_plan_count, g.N, g.W, g.H, g.sx, g.sy, g.brief_far,
+                     g.brief_gw[PY == 8 ? 1 : 0], g.brief_gh[PY == 8 ? 1 : 0], Fa};
+    const int gxb = (g.sx + kBriefTW - 1) / kBriefTW, gyb = (g.sy + TH - 1) / TH;
+    const size_t lds = brief_lds_bytes(g, RN, TH);
+    hipLaunchKernelGGL((k_brief_lds_pair<NSP, RN, PY>), dim3(2 * gxb * gyb), dim3(kThreads), lds, st, a, g8b, Fb);
+    return hipGetLastError();
+}
+
 template <int NSP>
 static hipError_t launch_brief_lds_n(const Geometry &g, const float *L, const uint8_t *g8, uint32_t *F,
                                      hipStream_t st)
@@ -1947,6 +2000,31 @@ static hipError_t launch_brief_lds_n(const Geometry &g, const float *L, const ui
     case 48: return launch_brief_lds_t<NSP, 48>(g, L, g8, F, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_brief_pair(const Geometry &g, const uint8_t *g8a, uint32_t *Fa, const uint8_t *g8b, uint32_t *Fb,
+                             hipStream_t st)
+{
+    if (g.brief_near <= 0) return hipErrorInvalidValue;
+#define BM_PAIR_RN(NSPV)                                                                                         \
+    switch (g.brief_near) {                                                                                      \
+    case 8: return launch_brief_pair_t<NSPV, 8>(g, g8a, Fa, g8b, Fb, st);                                        \
+    case 16: return launch_brief_pair_t<NSPV, 16>(g, g8a, Fa, g8b, Fb, st);                                      \
+    case 24: return launch_brief_pair_t<NSPV, 24>(g, g8a, Fa, g8b, Fb, st);                                      \
+    case 32: return launch_brief_pair_t<NSPV, 32>(g, g8a, Fa, g8b, Fb, st);                                      \
+    case 40: return launch_brief_pair_t<NSPV, 40>(g, g8a, Fa, g8b, Fb, st);                                      \
+    case 48: return launch_brief_pair_t<NSPV, 48>(g, g8a, Fa, g8b, Fb, st);                                      \
+    default: return hipErrorInvalidValue;                                                                        \
+    }
+    switch (g.NSP) {
+    case 1: BM_PAIR_RN(1)
+    case 2: BM_PAIR_RN(2)
+    case 4: BM_PAIR_RN(4)
+    case 8: BM_PAIR_RN(8)
+    case 16: BM_PAIR_RN(16)
+    default: return hipErrorInvalidValue;
+    }
+#undef BM_PAIR_RN
 }
 
 hipError_t launch_brief(const Geometry &g, const float *L, const uint8_t *g8, uint32_t *F, hipStream_t st)

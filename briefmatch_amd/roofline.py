@@ -74,7 +74,8 @@ def stage_work(p, NSP, s_tri, brief_frame=0):
     """Algorithmic work per launch of each stage for parameters p (a bm_params
     with sx, sy, sx_in, sy_in, N, fsize), descriptor stride NSP and trilateral
     half window s_tri; brief_frame 1: the up-sampled frame is written by its
-    own launch and read by BRIEF."""
+    own launch and read by BRIEF.  Gray, BRIEF and up-sampling are per frame
+    (a caller whose launch covers both frames scales them by two)."""
     Win = p.sx_in * p.sy_in
     M = p.sx * p.sy
     b = p.N / 8.0
