@@ -13,7 +13,9 @@ x 2.4 GHz issue peak of one MI355X).
 
 `traffic` scales each kernel's mean FETCH_SIZE and WRITE_SIZE (KiB per dispatch)
 by what the same counter reads for the plain copy kernel (k_copy) of known
-size, and merges the HBM bytes per launch of each stage of that workload into
+size, and merges the HBM bytes per launch of each stage of that workload (all
+kernels of the stage: the split trilateral filter's finishing kernel counts
+with it) into
 traffic.json under "workloads" -> workload -> stage (other workloads kept),
 which bench.py reports as roofline.traffic.  Profile a run whose launch shapes
 come from an autotuner cache (bench.py --tune-cache), so each stage runs one
@@ -123,15 +125,18 @@ def memory(mem1, mem2, stats_path, out, cmd):
         f.write("\n".join(lines) + "\n")
 
 
-def _mean_by_kernel(path):
+def _mean_by_kernel(path, counts=None):
     acc = defaultdict(list)
     for r in csv.DictReader(open(path)):
         acc[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    if counts is not None:
+        counts.update({k: len(v) for k, v in acc.items()})
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
 def traffic(fetch, write, cfetch, cwrite, copy_bytes, out, out_json, cmd, workload):
-    f = _mean_by_kernel(fetch)
+    calls = {}
+    f = _mean_by_kernel(fetch, calls)
     w = _mean_by_kernel(write)
     cf = _mean_by_kernel(cfetch)
     cw = _mean_by_kernel(cwrite)
@@ -157,8 +162,12 @@ def traffic(fetch, write, cfetch, cwrite, copy_bytes, out, out_json, cmd, worklo
             continue
         fb, wb = f[k] * sf * 1024, w.get(k, 0.0) * sw * 1024
         lines.append("%-34s %14.1f %14.1f %14.1f" % (k, fb / 1e6, wb / 1e6, (fb + wb) / 1e6))
-        per_stage[stage_of(k)].append(fb + wb)
-    tj["workloads"][workload] = {st: sum(v) / len(v) for st, v in per_stage.items()}
+        per_stage[stage_of(k)].append((fb + wb, calls.get(k, 1)))
+    # a stage launch may be several kernels (the split trilateral filter and its
+    # finishing kernel): bytes of all its dispatches over the launches, counted
+    # as the dispatches of its most frequent kernel
+    tj["workloads"][workload] = {st: sum(b * n for b, n in v) / max(n for _, n in v)
+                                 for st, v in per_stage.items()}
     with open(out, "w") as fo:
         fo.write("\n".join(lines) + "\n")
     with open(out_json, "w") as fo:
