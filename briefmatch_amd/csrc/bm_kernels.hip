@@ -892,6 +892,14 @@ __device__ __forceinline__ void level_tile(const uint32_t *__restrict__ F1, cons
 // Hamming distance is the sum of the group's partial counts.  The pixel's
 // selection logic runs on each lane of its group (identical inputs, identical
 // results); lane part 0 writes.
+// The value of the other lane of an even / odd lane pair (DPP quad_perm
+// [1, 0, 3, 2]: one VALU move, no LDS); every lane of the wave is active
+// where it is used.
+__device__ __forceinline__ uint32_t lane_pair_swap(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);
+}
+
 template <int NSP, bool RS, int L>
 __device__ __forceinline__ void level_tile_lanes(const uint32_t *__restrict__ F1, const uint32_t *__restrict__ F2,
                                                  const uint32_t *__restrict__ off_in,
@@ -910,13 +918,28 @@ __device__ __forceinline__ void level_tile_lanes(const uint32_t *__restrict__ F1
     const int xq = bx * span + (int)threadIdx.x / L;
     const bool live = xq < sx;
     const int idx = y * sx + min(xq, sx - 1);
-    const uint32_t own = live ? off_in[idx] : 0u;
-    const int own_cost = live ? (int)cost_in[idx] : 0;
     const int nidx[4] = {idx - sv, idx - sh, idx + sv, idx + sh};
     bool valid[4];
     uint32_t o[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) valid[c] = live && (c >= 2 || double_prop) && (unsigned)nidx[c] < (unsigned)M;
+    uint32_t own;
+    int own_cost;
+    if constexpr (L == 2) {
+        // the pixel's field words are read once per pixel, not once per lane:
+        // lane part 0 reads its own offset and cost, part 1 the upper
+        // neighbour's offset, and the pair swap (one load instruction for two
+        // words; the loads bound this kernel's address units)
+        const uint32_t v = part == 0 ? (live ? off_in[idx] : 0u) : (valid[0] ? off_in[nidx[0]] : 0u);
+        const int cst = part == 0 && live ? (int)cost_in[idx] : 0;
+        const uint32_t w = lane_pair_swap(v);
+        own = part == 0 ? v : w;
+        o[0] = part == 0 ? w : v;
+        own_cost = part == 0 ? cst : (int)lane_pair_swap((uint32_t)cst);
+    } else {
+        own = live ? off_in[idx] : 0u;
+        own_cost = live ? (int)cost_in[idx] : 0;
+    }
     bool lane_h[4] = {false, false, false, false};
     if (p < PW) {  // uniform: a horizontal neighbour in the same wave is another group's own entry
         const int pl = ((int)threadIdx.x & 63) / L;
@@ -928,9 +951,21 @@ __device__ __forceinline__ void level_tile_lanes(const uint32_t *__restrict__ F1
             o[c] = v;
         }
     }
+    if constexpr (L == 2) {
+        // the other three neighbours the same way: part 0 the left and right,
+        // part 1 the lower one (both lanes of a pixel agree on lane_h)
+        const bool n1 = !lane_h[1] && valid[1], n3 = !lane_h[3] && valid[3];
+        const uint32_t a = part == 0 ? (n1 ? off_in[nidx[1]] : 0u) : (valid[2] ? off_in[nidx[2]] : 0u);
+        const uint32_t b = part == 0 && n3 ? off_in[nidx[3]] : 0u;
+        const uint32_t a2 = lane_pair_swap(a), b2 = lane_pair_swap(b);
+        if (!lane_h[1]) o[1] = part == 0 ? a : a2;
+        o[2] = part == 0 ? a2 : a;
+        if (!lane_h[3]) o[3] = part == 0 ? b : b2;
+    } else {
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-        if (!lane_h[c]) o[c] = valid[c] ? off_in[nidx[c]] : 0u;
+        for (int c = 0; c < 4; ++c)
+            if (!lane_h[c]) o[c] = valid[c] ? off_in[nidx[c]] : 0u;
+    }
     uint32_t rcand = 0u;
     int rtgt = 0;
     if constexpr (RS) {
