@@ -315,10 +315,12 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_rsrc(const float *p, uin
 // copied from it and the far tests read it directly, so the work-group needs
 // neither the pair tile nor the coordinate tables and no test interpolates.
 
-template <int NSP, int RN, int PY, bool GL>
+template <int NSP, int RN, int PY, bool GL, int G = 4>
 __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned char *smem)
 {
-    constexpr int TH = 4 * PY, PITCH = kBriefTW + 2 * RN, ROWS = TH + 2 * RN;
+    // G waves per work-group, wave ly taking rows ly + G k of the 64 x TH tile
+    static_assert(GL || G == 4, "the fused kernel runs four waves per work-group");
+    constexpr int TH = G * PY, PITCH = kBriefTW + 2 * RN, ROWS = TH + 2 * RN;
     float *lt = reinterpret_cast<float *>(smem);
     const int sx = a.sx, sy = a.sy;
     const int gxb = (sx + kBriefTW - 1) / kBriefTW;
@@ -329,20 +331,20 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
     const uint16_t *gt = nullptr;
     if constexpr (GL) {
         // the tile and its halo, clamped to the frame per axis (as the reference's
-        // per-axis offset clamp): element i of a thread is row ly + 4 (i / NC),
+        // per-axis offset clamp): element i of a thread is row ly + G (i / NC),
         // column lx + 64 (i % NC); FB loads in flight before their stores (the
         // copy is latency-bound)
-        constexpr int NC = (PITCH + 63) / 64, NR = (ROWS + 3) / 4, TOT = NC * NR, FB = 16;
+        constexpr int NC = (PITCH + 63) / 64, NR = (ROWS + G - 1) / G, TOT = NC * NR, FB = 16;
         for (int i0 = 0; i0 < TOT; i0 += FB) {
             float v[FB];
 #pragma unroll
             for (int q = 0; q < FB; ++q) {
-                const int i = min(i0 + q, TOT - 1), r = ly + 4 * (i / NC), c = lx + 64 * (i % NC);
+                const int i = min(i0 + q, TOT - 1), r = ly + G * (i / NC), c = lx + 64 * (i % NC);
                 v[q] = a.L[(size_t)clampi(ty0 - RN + r, 0, sy - 1) * sx + clampi(tx0 - RN + c, 0, sx - 1)];
             }
 #pragma unroll
             for (int q = 0; q < FB; ++q) {
-                const int i = i0 + q, r = ly + 4 * (i / NC), c = lx + 64 * (i % NC);
+                const int i = i0 + q, r = ly + G * (i / NC), c = lx + 64 * (i % NC);
                 if (i < TOT && r < ROWS && c < PITCH) lt[r * PITCH + c] = v[q];
             }
         }
@@ -418,7 +420,7 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
         interior = tx0 - a.Rf >= 0 && tx0 + kBriefTW - 1 + a.Rf <= sx - 1 && ty0 - a.Rf >= 0 &&
                    ty0 + TH - 1 + a.Rf <= sy - 1;
 #pragma unroll
-        for (int k = 0; k < PY; ++k) pix[k] = (uint32_t)((ty0 + ly + 4 * k) * sx + tx0 + lx) * 4u;
+        for (int k = 0; k < PY; ++k) pix[k] = (uint32_t)((ty0 + ly + G * k) * sx + tx0 + lx) * 4u;
     }
     uint32_t words[NSP][PY];
 #pragma unroll
@@ -458,8 +460,8 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
                 const uint32_t *p4 = reinterpret_cast<const uint32_t *>(lbase + r[j + 1].y);
 #pragma unroll
                 for (int k = 0; k < PY; ++k) {
-                    const uint32_t d0 = p2[4 * k * PITCH] - p1[4 * k * PITCH];
-                    const uint32_t d1 = p4[4 * k * PITCH] - p3[4 * k * PITCH];
+                    const uint32_t d0 = p2[G * k * PITCH] - p1[G * k * PITCH];
+                    const uint32_t d1 = p4[G * k * PITCH] - p3[G * k * PITCH];
                     words[w][k] = __builtin_amdgcn_alignbit(__builtin_amdgcn_alignbit(words[w][k], d0, 31), d1, 31);
                 }
             }
@@ -501,7 +503,7 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
                 uint32_t va[PY], vb[PY];
 #pragma unroll
                 for (int k = 0; k < PY; ++k) {
-                    const int y = ty0 + ly + 4 * k;
+                    const int y = ty0 + ly + G * k;
                     va[k] = __float_as_uint(a.L[(size_t)clampi(y + oy1, 0, sy - 1) * sx + xa]);
                     vb[k] = __float_as_uint(a.L[(size_t)clampi(y + oy2, 0, sy - 1) * sx + xb]);
                 }
@@ -529,7 +531,7 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
     if (x < sx) {
 #pragma unroll
         for (int k = 0; k < PY; ++k) {
-            const int y = ty0 + ly + 4 * k;
+            const int y = ty0 + ly + G * k;
             if (y < sy) {
                 uint32_t out[NSP];
 #pragma unroll
@@ -540,13 +542,13 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
     }
 }
 
-template <int NSP, int RN, int PY, bool GL>
-__global__ void __launch_bounds__(kThreads) k_brief_lds(BriefJob a)
+template <int NSP, int RN, int PY, bool GL, int G = 4>
+__global__ void __launch_bounds__(64 * G) k_brief_lds(BriefJob a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int TH = 4 * PY;
+    constexpr int TH = G * PY;
     const int gxb = (a.sx + kBriefTW - 1) / kBriefTW, gyb = (a.sy + TH - 1) / TH;
-    brief_tile<NSP, RN, PY, GL>(a, xcd_remap(blockIdx.x, gxb * gyb), smem);
+    brief_tile<NSP, RN, PY, GL, G>(a, xcd_remap(blockIdx.x, gxb * gyb), smem);
 }
 
 // The fused kernel over both frames of a pair in one launch (their jobs differ
@@ -1988,6 +1990,18 @@ static hipError_t launch_brief_t(const Geometry &g, const float *L, const uint8_
 
 // L != nullptr: the up-sampled frame is in memory (the GL kernel, LDS for
 // the near tile only); else the fused kernel interpolates from g8.
+// The frame kernel's work-group shape: G waves over a 64 x (G PY) tile, PY
+// rows per thread.  Its LDS tile limits the work-groups per CU, so more waves
+// per tile hide more of the LDS latency; fewer rows per thread also free
+// registers.  Measured (BRIEF + up-sampling per pair, best halo of each, two
+// interleaved rounds on one box): 128 bits 4 waves x 8 rows 0.745 ms, 8 x 4
+// 0.650, 16 x 2 0.657, 16 x 4 0.670; 512 bits 4 x 4 3.79 ms, 8 x 2 3.52,
+// 16 x 1 3.78, 16 x 2 2.92 (32-row tiles: the 48-px halo fits one
+// work-group of 16 waves per CU and leaves 31 far tests of 512), 16 x 4 3.54.
+template <int NSP> struct BriefFrameShape {
+    static constexpr int PY = NSP <= 4 ? 4 : 2, G = NSP <= 4 ? 8 : 16;
+};
+
 template <int NSP, int RN>
 static hipError_t launch_brief_lds_t(const Geometry &g, const float *L, const uint8_t *g8, uint32_t *F,
                                      hipStream_t st)
@@ -1997,11 +2011,14 @@ static hipError_t launch_brief_lds_t(const Geometry &g, const float *L, const ui
     BriefJob a{g8,  g.up_x, g.up_y, g.brief_plan, g.brief_plan_count, g.N, g.W, g.H, g.sx, g.sy, g.brief_far,
                g.brief_gw[PY == 8 ? 1 : 0], g.brief_gh[PY == 8 ? 1 : 0], F};
     a.L = L;
-    const int gxb = (g.sx + kBriefTW - 1) / kBriefTW, gyb = (g.sy + TH - 1) / TH;
+    const int gxb = (g.sx + kBriefTW - 1) / kBriefTW;
     if (L) {
-        const size_t lds = (size_t)(kBriefTW + 2 * RN) * (TH + 2 * RN) * sizeof(float);
-        hipLaunchKernelGGL((k_brief_lds<NSP, RN, PY, true>), dim3(gxb * gyb), dim3(kThreads), lds, st, a);
+        constexpr int PYF = BriefFrameShape<NSP>::PY, GF = BriefFrameShape<NSP>::G, THF = PYF * GF;
+        const int gyb = (g.sy + THF - 1) / THF;
+        const size_t lds = (size_t)(kBriefTW + 2 * RN) * (THF + 2 * RN) * sizeof(float);
+        hipLaunchKernelGGL((k_brief_lds<NSP, RN, PYF, true, GF>), dim3(gxb * gyb), dim3(64 * GF), lds, st, a);
     } else {
+        const int gyb = (g.sy + TH - 1) / TH;
         const size_t lds = brief_lds_bytes(g, RN, TH);
         hipLaunchKernelGGL((k_brief_lds<NSP, RN, PY, false>), dim3(gxb * gyb), dim3(kThreads), lds, st, a);
     }
