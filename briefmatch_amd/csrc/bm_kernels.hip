@@ -319,8 +319,7 @@ template <int NSP, int RN, int PY, bool GL, int G = 4>
 __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned char *smem)
 {
     // G waves per work-group, wave ly taking rows ly + G k of the 64 x TH tile
-    static_assert(GL || G == 4, "the fused kernel runs four waves per work-group");
-    constexpr int TH = G * PY, PITCH = kBriefTW + 2 * RN, ROWS = TH + 2 * RN;
+    constexpr int TH = G * PY, PITCH = kBriefTW + 2 * RN, ROWS = TH + 2 * RN, NT = 64 * G;
     float *lt = reinterpret_cast<float *>(smem);
     const int sx = a.sx, sy = a.sy;
     const int gxb = (sx + kBriefTW - 1) / kBriefTW;
@@ -362,11 +361,11 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
         const int gy1 = min((int)(a.up_y[clampi(ty0 + TH - 1 + Rf, 0, sy - 1)] >> 8) + 1, H - 1);
         const int gw = gx1 - gx0 + 1, gh = gy1 - gy0 + 1;
 
-        for (int i = threadIdx.x; i < XT; i += kThreads) {
+        for (int i = threadIdx.x; i < XT; i += NT) {
             const uint32_t u = a.up_x[clampi(tx0 - Rf + i, 0, sx - 1)];
             xt[i] = (uint32_t)((int)(u >> 8) - gx0) | ((u & 255u) << 16);
         }
-        for (int i = threadIdx.x; i < YT; i += kThreads) {
+        for (int i = threadIdx.x; i < YT; i += NT) {
             const uint32_t u = a.up_y[clampi(ty0 - Rf + i, 0, sy - 1)];
             const int y0 = (int)(u >> 8), y1 = min(y0 + 1, H - 1);
             yt[i] = make_uint2((uint32_t)((y0 - gy0) * gw * 2) | ((uint32_t)((y1 - gy0) * gw * 2) << 16),
@@ -377,22 +376,22 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
         // measured best against 2, 4, 12 and 16); the right neighbour comes from
         // the next lane, except in lane 63 and past the tile's last column.
         constexpr int FB = 8;
-        for (int r0 = ly; r0 < gh; r0 += 4 * FB) {
+        for (int r0 = ly; r0 < gh; r0 += G * FB) {
             for (int c0 = 0; c0 < gw; c0 += 64) {
                 const int c = c0 + lx, x = gx0 + min(c, gw - 1);
                 uint32_t v[FB], n[FB];
     #pragma unroll
-                for (int q = 0; q < FB; ++q) v[q] = a.g8[(size_t)(gy0 + min(r0 + 4 * q, gh - 1)) * W + x];
+                for (int q = 0; q < FB; ++q) v[q] = a.g8[(size_t)(gy0 + min(r0 + G * q, gh - 1)) * W + x];
     #pragma unroll
                 for (int q = 0; q < FB; ++q) n[q] = (uint32_t)__shfl_down((int)v[q], 1, 64);
                 if (lx == 63 || c + 1 >= gw) {
     #pragma unroll
                     for (int q = 0; q < FB; ++q)
-                        n[q] = a.g8[(size_t)(gy0 + min(r0 + 4 * q, gh - 1)) * W + min(x + 1, W - 1)];
+                        n[q] = a.g8[(size_t)(gy0 + min(r0 + G * q, gh - 1)) * W + min(x + 1, W - 1)];
                 }
     #pragma unroll
                 for (int q = 0; q < FB; ++q)
-                    if (c < gw && r0 + 4 * q < gh) gtw[(r0 + 4 * q) * gw + c] = (uint16_t)(v[q] | (n[q] << 8));
+                    if (c < gw && r0 + G * q < gh) gtw[(r0 + G * q) * gw + c] = (uint16_t)(v[q] | (n[q] << 8));
             }
         }
         __syncthreads();
@@ -400,7 +399,7 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
             const uint32_t xv = xt[c - RN + Rf];
             const int x0 = (int)(xv & 0xffffu);
             const uint32_t wx = weight_pair(xv >> 16);
-            for (int r = ly; r < ROWS; r += 4)
+            for (int r = ly; r < ROWS; r += G)
                 lt[r * PITCH + c] = (float)bil_lds(gtw, x0, wx, yt[r - RN + Rf]) * (1.0f / 65536.0f);
         }
         __syncthreads();
@@ -521,8 +520,8 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
             const uint32_t wxa = weight_pair(xa >> 16), wxb = weight_pair(xb >> 16);
 #pragma unroll
             for (int k = 0; k < PY; ++k) {
-                const uint32_t va = bil_lds(gt, xa0, wxa, ycol[4 * k + oy1]);
-                const uint32_t vb = bil_lds(gt, xb0, wxb, ycol[4 * k + oy2]);
+                const uint32_t va = bil_lds(gt, xa0, wxa, ycol[G * k + oy1]);
+                const uint32_t vb = bil_lds(gt, xb0, wxb, ycol[G * k + oy2]);
                 words[w][k] |= ((vb - va) >> 31) << r0.z;
             }
         }
@@ -554,12 +553,12 @@ __global__ void __launch_bounds__(64 * G) k_brief_lds(BriefJob a)
 // The fused kernel over both frames of a pair in one launch (their jobs differ
 // only in the gray frame and the descriptors): for frames whose tiles alone
 // leave most of the device idle.
-template <int NSP, int RN, int PY>
-__global__ void __launch_bounds__(kThreads) k_brief_lds_pair(BriefJob a, const uint8_t *__restrict__ g8b,
-                                                             uint32_t *__restrict__ Fb)
+template <int NSP, int RN, int PY, int G>
+__global__ void __launch_bounds__(64 * G) k_brief_lds_pair(BriefJob a, const uint8_t *__restrict__ g8b,
+                                                           uint32_t *__restrict__ Fb)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int TH = 4 * PY;
+    constexpr int TH = G * PY;
     const int gxb = (a.sx + kBriefTW - 1) / kBriefTW, gyb = (a.sy + TH - 1) / TH, tiles = gxb * gyb;
     int b = xcd_remap(blockIdx.x, 2 * tiles);
     if (b >= tiles) {
@@ -567,7 +566,7 @@ __global__ void __launch_bounds__(kThreads) k_brief_lds_pair(BriefJob a, const u
         a.F = Fb;
         b -= tiles;
     }
-    brief_tile<NSP, RN, PY, false>(a, b, smem);
+    brief_tile<NSP, RN, PY, false, G>(a, b, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -2002,6 +2001,13 @@ template <int NSP> struct BriefFrameShape {
     static constexpr int PY = NSP <= 4 ? 4 : 2, G = NSP <= 4 ? 8 : 16;
 };
 
+// The fused kernel's: 8 waves over the same tiles as its 4-wave form (64 x 32
+// up to 128 bits, 64 x 16 above), whose LDS plan (pair tile, tables) is sized
+// by the tile height; the multi-pair kernel keeps 4 waves.
+template <int NSP> struct BriefFusedShape {
+    static constexpr int G = 8, PY = (NSP <= 4 ? 32 : 16) / G;
+};
+
 template <int NSP, int RN>
 static hipError_t launch_brief_lds_t(const Geometry &g, const float *L, const uint8_t *g8, uint32_t *F,
                                      hipStream_t st)
@@ -2018,9 +2024,11 @@ static hipError_t launch_brief_lds_t(const Geometry &g, const float *L, const ui
         const size_t lds = (size_t)(kBriefTW + 2 * RN) * (THF + 2 * RN) * sizeof(float);
         hipLaunchKernelGGL((k_brief_lds<NSP, RN, PYF, true, GF>), dim3(gxb * gyb), dim3(64 * GF), lds, st, a);
     } else {
+        constexpr int PYU = BriefFusedShape<NSP>::PY, GU = BriefFusedShape<NSP>::G;
+        static_assert(PYU * GU == TH, "the fused kernel's tile height is its LDS plan's");
         const int gyb = (g.sy + TH - 1) / TH;
         const size_t lds = brief_lds_bytes(g, RN, TH);
-        hipLaunchKernelGGL((k_brief_lds<NSP, RN, PY, false>), dim3(gxb * gyb), dim3(kThreads), lds, st, a);
+        hipLaunchKernelGGL((k_brief_lds<NSP, RN, PYU, false, GU>), dim3(gxb * gyb), dim3(64 * GU), lds, st, a);
     }
     return hipGetLastError();
 }
@@ -2031,11 +2039,14 @@ static hipError_t launch_brief_pair_t(const Geometry &g, const uint8_t *g8a, uin
 {
     constexpr int PY = NSP <= 4 ? 8 : 4;
     constexpr int TH = 4 * PY;
+    constexpr int PYU = BriefFusedShape<NSP>::PY, GU = BriefFusedShape<NSP>::G;
+    static_assert(PYU * GU == TH, "the fused kernel's tile height is its LDS plan's");
     const BriefJob a{g8a, g.up_x, g.up_y, g.brief_plan, g.brief_plan_count, g.N, g.W, g.H, g.sx, g.sy, g.brief_far,
                      g.brief_gw[PY == 8 ? 1 : 0], g.brief_gh[PY == 8 ? 1 : 0], Fa};
     const int gxb = (g.sx + kBriefTW - 1) / kBriefTW, gyb = (g.sy + TH - 1) / TH;
     const size_t lds = brief_lds_bytes(g, RN, TH);
-    hipLaunchKernelGGL((k_brief_lds_pair<NSP, RN, PY>), dim3(2 * gxb * gyb), dim3(kThreads), lds, st, a, g8b, Fb);
+    hipLaunchKernelGGL((k_brief_lds_pair<NSP, RN, PYU, GU>), dim3(2 * gxb * gyb), dim3(64 * GU), lds, st, a, g8b,
+                       Fb);
     return hipGetLastError();
 }
 
