@@ -248,7 +248,7 @@ typedef struct bm_tuning {
                          (interpolated from the gray frame in LDS); 1 = written to a context frame first,
                          the BRIEF kernel then copies its tile from that frame and reads its far tests
                          from it (needs brief_near > 0); a new context takes 1 when its matched frame
-                         has at least 2^21 pixels */
+                         has at least 2^21 pixels, with brief_near 48 at 512 bits where the plan fits */
     int tri_split;    /* work-groups per trilateral tile: 1, 2, 4 or 8; above 1 each sums a share of the
                          tile's source rows into a context buffer (3 tri_split W H floats) and a
                          finishing kernel adds the shares, which fills the device on small frames */
