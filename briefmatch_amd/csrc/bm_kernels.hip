@@ -309,6 +309,15 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_rsrc(const float *p, uin
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), 0, (int)n, 0x00020000);
 }
 
+// Bytes of an M-float frame from float offset o on (o may be negative: a test
+// offset up and left), saturated to the 32-bit range of a buffer resource;
+// the largest byte offset a load takes, 4 (M - 1), stays below it.
+__device__ __forceinline__ uint32_t frame_bytes_from(int M, int o)
+{
+    const uint64_t n = ((uint64_t)(int64_t)M - (uint64_t)(int64_t)o) * 4u;
+    return n > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)n;
+}
+
 // One 64 x TH tile b (row-major tile index) of the fused BRIEF; smem is the
 // work-group's dynamic LDS.  GL: the up-sampled frame is in global memory
 // (a.L, written by the up-sampler before): the tile and its near halo are
@@ -481,8 +490,8 @@ __device__ __forceinline__ void brief_tile(const BriefJob &a, int b, unsigned ch
                     const int ox1 = (int16_t)(r0.x & 0xffffu), oy1 = (int16_t)(r0.x >> 16);
                     const int ox2 = (int16_t)(r0.y & 0xffffu), oy2 = (int16_t)(r0.y >> 16);
                     const int M = sx * sy, o1 = oy1 * sx + ox1, o2 = oy2 * sx + ox2;
-                    const __amdgpu_buffer_rsrc_t r1 = frame_rsrc(a.L + o1, (uint32_t)(M - o1) * 4u);
-                    const __amdgpu_buffer_rsrc_t r2 = frame_rsrc(a.L + o2, (uint32_t)(M - o2) * 4u);
+                    const __amdgpu_buffer_rsrc_t r1 = frame_rsrc(a.L + o1, frame_bytes_from(M, o1));
+                    const __amdgpu_buffer_rsrc_t r2 = frame_rsrc(a.L + o2, frame_bytes_from(M, o2));
                     uint32_t va[PY], vb[PY];
 #pragma unroll
                     for (int k = 0; k < PY; ++k) {
