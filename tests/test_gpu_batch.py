@@ -52,6 +52,24 @@ def test_tuning_does_not_change_results(N):
                 assert np.abs(a - b).max() <= 1e-3, (groups, split)
 
 
+@pytest.mark.parametrize("wl,frame,near", [("c1_320x240_u1.5_n32", 0, None), ("c2_1920x1080_u2.9_n128", 1, 32),
+                                           ("c3_1280x720_u4.9_n512", 1, 48)])
+def test_default_tuning_by_size(wl, frame, near):
+    """A new context's BRIEF mode follows its matched frame: fused below 2^21
+    pixels, the up-sampled frame in memory above, with the 48-px near halo at
+    512 bits (where the pattern reaches past 32 px); the trilateral filter is
+    not split until a tuning asks for it."""
+    w = synth.WORKLOADS[wl]
+    p = bm.derive_params(w["W"], w["H"], w["up"], w["up"], N=w["N"])
+    ctx = bm.Context(p, seed=1)
+    t = ctx.tuning()
+    assert t.brief_frame == frame, t.as_dict()
+    if near is not None:
+        assert t.brief_near == near, t.as_dict()
+    assert t.tri_split == 1
+    ctx.close()
+
+
 def test_tuning_rejects_invalid():
     p = bm.derive_params(64, 48, 2.0, 2.0, N=512)
     ctx = bm.Context(p)
