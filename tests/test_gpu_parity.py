@@ -98,6 +98,35 @@ def test_gray_upsample_brief(W, H, up, N):
     assert np.array_equal(ctx.download_desc(Ff), oF)
 
 
+@pytest.mark.parametrize("W,H,up,N", [(23, 9, 1.0, 512), (17, 5, 1.0, 100), (30, 7, 1.3, 256), (70, 3, 1.0, 32)])
+def test_brief_frames_smaller_than_a_tile(W, H, up, N):
+    """Frames narrower or shorter than one work-group tile (64 x 16 / 32 rows,
+    up to 16 waves), with a user pattern whose offsets (up to 20 px) reach past
+    every border: both BRIEF kernels clamp each sample to the frame as the
+    oracle does."""
+    say("brief tiny frames", W, H, up, N)
+    pr = _pair(W, H, seed=W * H + N)
+    p = bm.derive_params(W, H, up, up, N=N)
+    ctx = bm.Context(p, seed=5)
+    pat = np.random.default_rng(N + W).integers(-20, 21, 4 * N).astype(np.int16)
+    ctx.set_pattern(pat)
+    rgb = bm.DeviceBuffer(array=pr.frame1)
+    g8 = bm.DeviceBuffer(W * H)
+    ctx.gray(rgb, g8=g8)
+    L = bm.DeviceBuffer(p.sx * p.sy * 4)
+    ctx.upsample(g8, L)
+    oL = oracle.upsample(oracle.gray(pr.frame1), p.sx, p.sy)
+    oF = _pad(oracle.brief(oL, pat, N), ctx.NSP)
+    for near in (8, 16, 24):
+        ctx.set_tuning(brief_near=near)
+        F, Ff = ctx.desc_buffer(), ctx.desc_buffer()
+        ctx.brief(L, F)
+        ctx.brief_gray(g8, Ff)
+        ctx.sync()
+        assert np.array_equal(ctx.download_desc(F), oF), ("frame", near)
+        assert np.array_equal(ctx.download_desc(Ff), oF), ("fused", near)
+
+
 @pytest.mark.parametrize("N", [72, 128])
 def test_brief_pattern_halo_boundaries(N):
     """User patterns whose tests sit exactly at the near halo, one pixel beyond
