@@ -864,10 +864,11 @@ int bm_ctx_create(const bm_params *params, uint64_t seed, bm_ctx **out)
         plan_brief(ctx, pattern_rmax(ctx), 32);
         bm_tuning t = current_tuning(ctx);
         t.brief_near = 48;
-        if (fits && apply_tuning(ctx, t) != BM_OK) {
+        const int rt = fits ? apply_tuning(ctx, t) : BM_OK;
+        if (rt != BM_OK) {
             std::string msg = ctx->err;
             bm_ctx_destroy(ctx);
-            return fail(nullptr, BM_E_ALLOC, msg);
+            return fail(nullptr, rt, msg);
         }
     }
     *out = ctx;
