@@ -28,7 +28,7 @@
 namespace bm {
 namespace {
 
-__device__ const int32_t d_ndtri_q16[4096] = {
+__device__ __align__(16) const int32_t d_ndtri_q16[4096] = {
 #include "bm_ndtri_q16.inc"
 };
 
@@ -585,7 +585,21 @@ struct RSArgs {
     int enabled;
     uint32_t iteration, pair, key0, key1;
     int64_t sigma_q16;
+    const int32_t *ndtri;  // the inverse-normal table as the kernel reads it (set on the device)
 };
+
+// The inverse-normal table copied into the work-group's LDS: a random-search
+// candidate looks it up at two random indices per pixel, gathers that would
+// otherwise take the address units these kernels are bound by.  Every thread
+// of the work-group calls it before any returns.
+__device__ __forceinline__ const int32_t *ndtri_to_lds(int32_t *t)
+{
+    const int4 *src = reinterpret_cast<const int4 *>(d_ndtri_q16);
+    int4 *dst = reinterpret_cast<int4 *>(t);
+    for (int i = threadIdx.x; i < 4096 / 4; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+    return t;
+}
 
 template <int NSP>
 __device__ __forceinline__ void random_candidate(const uint32_t (&f1)[NSP], const uint32_t *__restrict__ F2,
@@ -594,8 +608,8 @@ __device__ __forceinline__ void random_candidate(const uint32_t (&f1)[NSP], cons
 {
     uint32_t c0 = (uint32_t)idx, c1 = rs.iteration, c2 = rs.pair, c3 = 0u;
     philox4x32_10(c0, c1, c2, c3, rs.key0, rs.key1);
-    int nx = round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c0 >> 20]);
-    int ny = round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c1 >> 20]);
+    int nx = round_q32(rs.sigma_q16 * (int64_t)rs.ndtri[c0 >> 20]);
+    int ny = round_q32(rs.sigma_q16 * (int64_t)rs.ndtri[c1 >> 20]);
     nx = clampi(nx, -x, sx - 1 - x);
     ny = clampi(ny, -y, sy - 1 - y);
     const uint32_t cand = pack_off(nx, ny);
@@ -680,8 +694,8 @@ __device__ __forceinline__ void init_tile_lanes(const uint32_t *__restrict__ F1,
     if constexpr (RS) {
         uint32_t c0 = (uint32_t)idx, c1 = rs.iteration, c2 = rs.pair, c3 = 0u;
         philox4x32_10(c0, c1, c2, c3, rs.key0, rs.key1);
-        const int rx = clampi(round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c0 >> 20]), -x, sx - 1 - x);
-        const int ry = clampi(round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c1 >> 20]), -y, sy - 1 - y);
+        const int rx = clampi(round_q32(rs.sigma_q16 * (int64_t)rs.ndtri[c0 >> 20]), -x, sx - 1 - x);
+        const int ry = clampi(round_q32(rs.sigma_q16 * (int64_t)rs.ndtri[c1 >> 20]), -y, sy - 1 - y);
         rcand = pack_off(rx, ry);
         rtgt = idx + ry * sx + rx;
     }
@@ -735,6 +749,8 @@ __global__ void __launch_bounds__(kThreads) k_init_lanes(const uint32_t *__restr
                                                          uint32_t *__restrict__ off, uint16_t *__restrict__ cost,
                                                          int sx, int sy, int strip_rows, RSArgs rs)
 {
+    __shared__ __align__(16) int32_t nd[RS ? 4096 : 4];
+    if constexpr (RS) rs.ndtri = ndtri_to_lds(nd);
     constexpr int span = kThreads / L;
     const int gx = (sx + span - 1) / span;
     init_tile_lanes<NSP, RS, L>(F1, F2, first, prev, off, cost, sx, sy, strip_rows, rs,
@@ -747,6 +763,8 @@ __global__ void __launch_bounds__(kThreads) k_init(const uint32_t *__restrict__ 
                                                    uint32_t *__restrict__ off, uint16_t *__restrict__ cost,
                                                    int sx, int sy, int strip_rows, RSArgs rs)
 {
+    __shared__ __align__(16) int32_t nd[RS ? 4096 : 4];
+    if constexpr (RS) rs.ndtri = ndtri_to_lds(nd);
     const int gx = (sx + kThreads - 1) / kThreads;
     init_tile<NSP, RS>(F1, F2, first, prev, off, cost, sx, sy, strip_rows, rs, xcd_remap(blockIdx.x, gx * sy));
 }
@@ -816,8 +834,8 @@ __device__ __forceinline__ void level_tile(const uint32_t *__restrict__ F1, cons
             uint32_t c0 = (uint32_t)idx[q], c1 = rs.iteration, c2 = rs.pair, c3 = 0u;
             philox4x32_10(c0, c1, c2, c3, rs.key0, rs.key1);
             const int x = min(xq[q], sx - 1);
-            const int nx = clampi(round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c0 >> 20]), -x, sx - 1 - x);
-            const int ny = clampi(round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c1 >> 20]), -y, sy - 1 - y);
+            const int nx = clampi(round_q32(rs.sigma_q16 * (int64_t)rs.ndtri[c0 >> 20]), -x, sx - 1 - x);
+            const int ny = clampi(round_q32(rs.sigma_q16 * (int64_t)rs.ndtri[c1 >> 20]), -y, sy - 1 - y);
             rcand[q] = pack_off(nx, ny);
             rtgt[q] = idx[q] + ny * sx + nx;
         }
@@ -982,8 +1000,8 @@ __device__ __forceinline__ void level_tile_lanes(const uint32_t *__restrict__ F1
         uint32_t c0 = (uint32_t)idx, c1 = rs.iteration, c2 = rs.pair, c3 = 0u;
         philox4x32_10(c0, c1, c2, c3, rs.key0, rs.key1);
         const int x = min(xq, sx - 1);
-        const int nx = clampi(round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c0 >> 20]), -x, sx - 1 - x);
-        const int ny = clampi(round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c1 >> 20]), -y, sy - 1 - y);
+        const int nx = clampi(round_q32(rs.sigma_q16 * (int64_t)rs.ndtri[c0 >> 20]), -x, sx - 1 - x);
+        const int ny = clampi(round_q32(rs.sigma_q16 * (int64_t)rs.ndtri[c1 >> 20]), -y, sy - 1 - y);
         rcand = pack_off(nx, ny);
         rtgt = idx + ny * sx + nx;
     }
@@ -1086,6 +1104,8 @@ __global__ void __launch_bounds__(kThreads) k_level_lanes(const uint32_t *__rest
                                                           uint16_t *__restrict__ cost_out, int sx, int sy, int p,
                                                           int pd, int double_prop, int strip_rows, RSArgs rs)
 {
+    __shared__ __align__(16) int32_t nd[RS ? 4096 : 4];
+    if constexpr (RS) rs.ndtri = ndtri_to_lds(nd);
     constexpr int span = kThreads / L;
     const int gx = (sx + span - 1) / span;
     level_tile_lanes<NSP, RS, L>(F1, F2, off_in, cost_in, off_out, cost_out, sx, sy, p, pd, double_prop, strip_rows,
@@ -1100,6 +1120,8 @@ __global__ void __launch_bounds__(kThreads) k_level(const uint32_t *__restrict__
                                                     uint16_t *__restrict__ cost_out, int sx, int sy, int p,
                                                     int pd, int double_prop, int strip_rows, RSArgs rs)
 {
+    __shared__ __align__(16) int32_t nd[RS ? 4096 : 4];
+    if constexpr (RS) rs.ndtri = ndtri_to_lds(nd);
     const int gx = (sx + kThreads * PX - 1) / (kThreads * PX);
     level_tile<NSP, RS, PX>(F1, F2, off_in, cost_in, off_out, cost_out, sx, sy, p, pd, double_prop, strip_rows, rs,
                             xcd_remap(blockIdx.x, gx * sy));
@@ -1112,6 +1134,8 @@ __global__ void __launch_bounds__(kThreads) k_random_search(const uint32_t *__re
                                                             uint16_t *__restrict__ cost, int sx, int sy,
                                                             int strip_rows, RSArgs rs)
 {
+    __shared__ __align__(16) int32_t nd[4096];
+    rs.ndtri = ndtri_to_lds(nd);
     const int gx = (sx + kThreads - 1) / kThreads;
     int bx, y;
     strip_block(xcd_remap(blockIdx.x, gx * sy), gx, sy, strip_rows, bx, y);
@@ -1143,6 +1167,8 @@ __global__ void __launch_bounds__(kThreads) k_init_proposal(const uint32_t *__re
                                                             int H, int sx, int sy, int strip_rows, RSArgs rs,
                                                             uint32_t *__restrict__ off, uint16_t *__restrict__ cost)
 {
+    __shared__ __align__(16) int32_t nd[RS ? 4096 : 4];
+    if constexpr (RS) rs.ndtri = ndtri_to_lds(nd);
     // row segments of 256 pixels, strip-major and XCD-contiguous, so the
     // candidate gathers of the segments sharing an L2 overlap
     const int gx = (sx + kThreads - 1) / kThreads;
@@ -1168,8 +1194,8 @@ __global__ void __launch_bounds__(kThreads) k_init_proposal(const uint32_t *__re
     if constexpr (RS) {
         uint32_t c0 = (uint32_t)i, c1 = rs.iteration, c2 = rs.pair, c3 = 0u;
         philox4x32_10(c0, c1, c2, c3, rs.key0, rs.key1);
-        const int rx = clampi(round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c0 >> 20]), -x, sx - 1 - x);
-        const int ry = clampi(round_q32(rs.sigma_q16 * (int64_t)d_ndtri_q16[c1 >> 20]), -y, sy - 1 - y);
+        const int rx = clampi(round_q32(rs.sigma_q16 * (int64_t)rs.ndtri[c0 >> 20]), -x, sx - 1 - x);
+        const int ry = clampi(round_q32(rs.sigma_q16 * (int64_t)rs.ndtri[c1 >> 20]), -y, sy - 1 - y);
         rcand = pack_off(rx, ry);
         load_desc<NSP>(F2, (uint32_t)(i + ry * sx + rx), fr);
     }
@@ -1830,7 +1856,7 @@ __global__ void __launch_bounds__(kThreads) k_brief_multi(const BriefJob *__rest
     })
 }
 
-__device__ __forceinline__ RSArgs job_rs(const MatchJob &J, int rs_iteration)
+__device__ __forceinline__ RSArgs job_rs(const MatchJob &J, int rs_iteration, const int32_t *ndtri)
 {
     RSArgs rs;
     rs.enabled = rs_iteration >= 0;
@@ -1839,6 +1865,7 @@ __device__ __forceinline__ RSArgs job_rs(const MatchJob &J, int rs_iteration)
     rs.key0 = J.key0;
     rs.key1 = J.key1;
     rs.sigma_q16 = J.sigma_q16;
+    rs.ndtri = ndtri;
     return rs;
 }
 
@@ -1848,9 +1875,11 @@ __global__ void __launch_bounds__(kThreads) k_init_multi(const MatchJob *__restr
                                                          const int *__restrict__ start, int njobs, int slots,
                                                          int rs_iteration)
 {
+    __shared__ __align__(16) int32_t nd[RS ? 4096 : 4];
+    const int32_t *ndtri = RS ? ndtri_to_lds(nd) : d_ndtri_q16;
     BM_MULTI_TILES({
         const MatchJob &J = jobs[j_];
-        const RSArgs rs = job_rs(J, rs_iteration);
+        const RSArgs rs = job_rs(J, rs_iteration, ndtri);
         init_tile<NSP, RS>(J.F1, J.F2, 1, nullptr, J.off[0], J.cost[0], J.sx, J.sy, J.strip_rows, rs, lt_);
     })
 }
@@ -1861,9 +1890,11 @@ __global__ void __launch_bounds__(kThreads) k_level_multi(const MatchJob *__rest
                                                           const int *__restrict__ start, int njobs, int slots, int p,
                                                           int pd, int src, int rs_iteration)
 {
+    __shared__ __align__(16) int32_t nd[RS ? 4096 : 4];
+    const int32_t *ndtri = RS ? ndtri_to_lds(nd) : d_ndtri_q16;
     BM_MULTI_TILES({
         const MatchJob &J = jobs[j_];
-        const RSArgs rs = job_rs(J, rs_iteration);
+        const RSArgs rs = job_rs(J, rs_iteration, ndtri);
         level_tile<NSP, RS, PX>(J.F1, J.F2, J.off[src], J.cost[src], J.off[1 - src], J.cost[1 - src], J.sx, J.sy,
                                 p, pd, J.double_prop, J.strip_rows, rs, lt_);
     })
@@ -2141,6 +2172,7 @@ static RSArgs make_rs(const Geometry &g, int enabled, uint32_t iteration, uint32
     rs.key0 = (uint32_t)g.seed;
     rs.key1 = (uint32_t)(g.seed >> 32);
     rs.sigma_q16 = g.rs_sigma_q16;
+    rs.ndtri = nullptr;  // each kernel points it at its LDS copy
     return rs;
 }
 
